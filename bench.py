@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark: FP32 nonlinear forward modelling on one MI355X.
+
+Workload (one "step" = one full shot): 400 x 400 x 250-cell seeded layered
+model (padded 328 x 472 x 468), d = 12.5 m, explosive 10 Hz Ricker source,
+2.5 s = 626 coarse samples x 8 fine steps (5000 fine steps), 200 x 200
+surface receivers recording all 9 components. Inputs (medium, positions,
+source traces) are resident in HBM before the timed region.
+
+Prints one JSON line with the throughput in Gcell-updates/s, the roofline of
+the stencil kernels (algorithmic bytes / HIP-event-timed launch duration vs
+8 TB/s) and the CPU baseline (the FP64 CPU restatement of the scheme timed on
+a bounded slice of the 100^3 homogeneous workload on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_F32 = 140       # 9 c + 9 o reads, 9 n writes, 8 medium reads (x4 B)
+BYTES_VEL_F32 = 60             # velocity kernel: 6 c + 3 o + 3 medium reads, 3 writes
+BYTES_STRESS_F32 = 80          # stress kernel: 3 c + 6 o + 5 medium reads, 6 writes
+METRIC = "Gcell-updates/s & %HBM peak; FWI-gradient shots/s batched vs plain, 1 GPU"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_setup(workload):
+    import torch
+    from elasticisolib_3d_amd import propagator as P
+    from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
+    from elasticisolib_3d_amd.models import source_traces
+
+    w = workload
+    a = w.axes
+    t0 = time.time()
+    med = P.scale_medium(w.rho, w.lame, w.mu, w.dtw)
+    src_geom = ShotGeometry(a, w.src_xyz[:, 2], w.src_xyz[:, 0], w.src_xyz[:, 1])
+    rec_geom = ShotGeometry(a, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
+    src_reg = scale_regular_traces(med, src_geom, src_geom.to_regular(source_traces(w)), a.dx * a.dy * a.dz)
+    log(f"[bench] host set-up {time.time() - t0:.1f}s; receiver points per grid {rec_geom.counts()}")
+    prop = P.Propagator(a.nz, a.nx, a.ny, w.nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos, "f32")
+    dev = "cuda"
+    prop.set_medium(torch.tensor(med, dtype=torch.float32, device=dev))
+    del med
+    spos = [torch.tensor(p, device=dev) for p in src_geom.positions()]
+    rpos = [torch.tensor(p, device=dev) for p in rec_geom.positions()]
+    src = [torch.tensor(s, dtype=torch.float32, device=dev) for s in src_reg]
+    rec = [torch.zeros((n, w.nts), dtype=torch.float32, device=dev) for n in
+           [rec_geom.counts()[g] for g in (1, 2, 3, 0, 0, 0, 4, 5, 6)]]
+    torch.cuda.synchronize()
+    return prop, spos, src, rpos, rec
+
+
+def cpu_baseline(target_s: float = 15.0):
+    """FP64 CPU restatement of the scheme on a bounded slice of the 100^3
+    homogeneous workload (the first coarse samples of the 0.6 s record)."""
+    from oracle import oracle as O
+    from elasticisolib_3d_amd.models import homogeneous_100
+    from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
+    from elasticisolib_3d_amd.models import source_traces
+
+    O.build()
+    w = homogeneous_100()
+    a = w.axes
+
+    def run(nts):
+        g = O.Grid(a.nz, a.nx, a.ny, nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos)
+        med = O.lib_scale_medium(w.rho, w.lame, w.mu, w.dtw)
+        sg = ShotGeometry(a, w.src_xyz[:, 2], w.src_xyz[:, 0], w.src_xyz[:, 1])
+        rg = ShotGeometry(a, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
+        tr = [t[:, :nts] for t in source_traces(w)]
+        src = scale_regular_traces(med, sg, sg.to_regular(tr), a.dx * a.dy * a.dz)
+        t0 = time.perf_counter()
+        O.lib_nl_forward(g, med, src, sg.positions(), rg.positions())
+        return time.perf_counter() - t0
+
+    probe_nts = 3
+    t_probe = run(probe_nts)
+    per_step = t_probe / ((probe_nts - 1) * w.sub)
+    n_fine = max(8, int(target_s / max(per_step, 1e-6)))
+    nts = min(w.nts, n_fine // w.sub + 1)
+    t = run(nts)
+    steps = (nts - 1) * w.sub
+    cells = w.interior_cells * steps
+    return {
+        "value": round(cells / t / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": (f"FP64 CPU restatement, 100^3 homogeneous workload (padded {a.nz}x{a.nx}x{a.ny}), "
+                   f"first {nts} of {w.nts} coarse samples = {steps} fine steps, {t:.1f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
+    ap.add_argument("--nts", type=int, default=626, help="coarse samples (626 = 2.5 s)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "shots would be distributed one process per GPU, which this build does not launch.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from elasticisolib_3d_amd.models import layered_400
+
+    t0 = time.time()
+    w = layered_400(nts=args.nts)
+    log(f"[bench] model {w.name} padded {w.axes.nz}x{w.axes.nx}x{w.axes.ny} built in {time.time() - t0:.1f}s")
+    prop, spos, src, rpos, rec = gpu_setup(w)
+
+    for i in range(args.warmup):
+        prop.nl_forward(spos, src, rpos, rec_out=rec)
+        torch.cuda.synchronize()
+        log(f"[bench] warmup shot {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = 0.0
+    for i in range(args.steps):
+        prop.nl_forward(spos, src, rpos, rec_out=rec)
+        ms, steps_done, cells = prop.last_timing()
+        ev_ms += ms
+        log(f"[bench] timed shot {i + 1}/{args.steps}: loop {ms:.1f} ms")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cell_updates = float(w.interior_cells) * w.fine_steps * args.steps
+    gcells = cell_updates / wall / 1e9
+
+    # per-kernel launch durations with HIP events on the propagator's stream
+    kv, ks = prop.time_step_kernels(args.kernel_reps)
+    step_ms = kv + ks
+    ach = BYTES_PER_CELL_F32 * w.interior_cells / (step_ms * 1e-3) / 1e9
+    finite = all(bool(torch.isfinite(r).all()) for r in rec)
+    norm = float(sum(float((r.double() ** 2).sum()) for r in rec) ** 0.5)
+
+    out = {
+        "metric": METRIC,
+        "value": round(gcells, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "FP32 nonlinear forward modelling, one shot per step: seeded 12-layer model 400x400x250 "
+                        "cells (padded 328x472x468, 30-cell sponge), d=12.5 m, explosive 10 Hz Ricker, "
+                        f"{w.nts} coarse samples x {w.sub} fine steps, 200x200 surface receivers, 9 components",
+            "interior_cells": w.interior_cells,
+            "fine_steps": w.fine_steps,
+            "event_loop_ms_per_step": round(ev_ms / args.steps, 2),
+            "data_finite": finite,
+            "data_l2": norm,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_velocity + k_stress (one fine step)",
+            "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_cell": BYTES_PER_CELL_F32,
+            "k_velocity_ms": round(kv, 4),
+            "k_stress_ms": round(ks, 4),
+            "k_velocity_GBs": round(BYTES_VEL_F32 * w.interior_cells / (kv * 1e-3) / 1e9, 1),
+            "k_stress_GBs": round(BYTES_STRESS_F32 * w.interior_cells / (ks * 1e-3) / 1e9, 1),
+        },
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # reported, never silently replaced
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+"""Acquisition geometry on the 7 staggered grids.
+
+Counterpart of the reference's ``spaceInterpGpu_3D`` objects and of the
+Python geometry builders ``buildSourceGeometry_3D`` / ``buildReceiversGeometry_3D``
+(python_float/Elastic_iso_float_3D.py:21-383). Weights and unique point
+lists come from the library's C++ routine ``ela_interp_linear``; the trace
+mapping between irregular devices and regular grid points is a sparse
+matrix product.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, Sequence
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+
+GRID_NAMES = ("centre", "x", "y", "z", "xz", "xy", "yz")
+# Half-cell origin shifts (z, x, y) of each grid (Elastic_iso_float_3D.py:97-110).
+GRID_SHIFT = ((0, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 0), (1, 1, 0), (0, 1, 1), (1, 0, 1))
+COMP_GRID = (1, 2, 3, 0, 0, 0, 4, 5, 6)
+
+
+@dataclass
+class PaddedAxes:
+    """Padded model axes with the par-file padding parameters."""
+
+    nz: int
+    nx: int
+    ny: int
+    oz: float
+    ox: float
+    oy: float
+    dz: float
+    dx: float
+    dy: float
+    fat: int
+    z_pad_minus: int
+    z_pad_plus: int
+    x_pad_minus: int
+    x_pad_plus: int
+    y_pad: int
+    tolerance: float = 1e-6
+
+    @property
+    def min_pad(self) -> int:
+        """Sponge width (fdParamElastic_3D.cpp:44-55, 88; freeSurface=0)."""
+        return min(min(self.z_pad_minus, self.z_pad_plus), min(self.x_pad_minus, self.x_pad_plus))
+
+    def grid(self, g: int) -> "PaddedAxes":
+        sz, sx, sy = GRID_SHIFT[g]
+        return PaddedAxes(self.nz, self.nx, self.ny, self.oz - 0.5 * self.dz * sz, self.ox - 0.5 * self.dx * sx,
+                          self.oy - 0.5 * self.dy * sy, self.dz, self.dx, self.dy, self.fat, self.z_pad_minus,
+                          self.z_pad_plus, self.x_pad_minus, self.x_pad_plus, self.y_pad, self.tolerance)
+
+    def c(self) -> _lib.ElaInterpAxes:
+        return _lib.ElaInterpAxes(self.nz, self.nx, self.ny, self.oz, self.ox, self.oy, self.dz, self.dx, self.dy,
+                                  self.fat, self.z_pad_minus, self.z_pad_plus, self.x_pad_minus, self.x_pad_plus,
+                                  self.y_pad, self.tolerance)
+
+
+class SpaceInterp:
+    """Linear interpolation of one device set onto one grid (spaceInterpGpu_3D)."""
+
+    def __init__(self, axes: PaddedAxes, z, x, y):
+        lib = _lib.load()
+        z = np.ascontiguousarray(z, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        n = z.shape[0]
+        nbr = np.empty(8 * max(n, 1), np.int64)
+        w = np.empty(8 * max(n, 1), np.float64)
+        uniq = np.empty(8 * max(n, 1), np.int64)
+        nu = ctypes.c_int64()
+        ax = axes.c()
+        rc = lib.ela_interp_linear(ctypes.byref(ax), n, z.ctypes.data, x.ctypes.data, y.ctypes.data, nbr.ctypes.data,
+                                   w.ctypes.data, uniq.ctypes.data, ctypes.byref(nu))
+        if rc != _lib.ELA_OK:
+            raise ValueError("acquisition device out of bounds (or its interpolation stencil leaves the padded interior)")
+        self.n_dev = n
+        self.pos_unique = uniq[: nu.value].copy()
+        self.nbr = nbr[: 8 * n].reshape(n, 8)
+        self.weight = w[: 8 * n].reshape(n, 8)
+        rows = np.repeat(np.arange(n), 8)
+        # M: [n_dev, n_reg]; forward (reg -> irreg) is M @ reg, adjoint is M.T @ irreg
+        self.M = sp.csr_matrix((self.weight.ravel(), (rows, self.nbr.ravel())), shape=(n, nu.value))
+
+    @property
+    def n_reg(self) -> int:
+        return len(self.pos_unique)
+
+    def forward(self, reg: np.ndarray) -> np.ndarray:
+        """regular [n_reg, nt] -> irregular [n_dev, nt] (spaceInterpGpu_3D.cpp:140-157)."""
+        return np.asarray(self.M @ reg)
+
+    def adjoint(self, irreg: np.ndarray) -> np.ndarray:
+        """irregular [n_dev, nt] -> regular [n_reg, nt] (spaceInterpGpu_3D.cpp:159-175)."""
+        return np.asarray(self.M.T @ irreg)
+
+
+class ShotGeometry:
+    """The 7 grid interpolators of one device set (one shot's sources or receivers)."""
+
+    def __init__(self, axes: PaddedAxes, z, x, y):
+        self.grids: List[SpaceInterp] = [SpaceInterp(axes.grid(g), z, x, y) for g in range(7)]
+        self.n_dev = len(np.atleast_1d(z))
+
+    def positions(self) -> List[np.ndarray]:
+        return [g.pos_unique for g in self.grids]
+
+    def counts(self) -> List[int]:
+        return [g.n_reg for g in self.grids]
+
+    def to_regular(self, traces: Sequence[np.ndarray]) -> List[np.ndarray]:
+        """9 irregular [n_dev, nt] -> 9 regular [n_reg(grid), nt]."""
+        return [self.grids[COMP_GRID[c]].adjoint(traces[c]) for c in range(9)]
+
+    def to_irregular(self, traces: Sequence[np.ndarray]) -> List[np.ndarray]:
+        return [self.grids[COMP_GRID[c]].forward(traces[c]) for c in range(9)]
+
+
+def scale_regular_traces(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.ndarray], dxyz: float):
+    """Medium scaling applied to injected traces: forces by 2dtw/rho, normal
+    moments through the (lambda+2mu, lambda) block, shear moments by 2dtw mu,
+    all divided by the cell volume (nonlinearPropElasticGpu_3D.cpp:200-277;
+    the nonlinear adjoint applies it to the receiver traces, :481-558)."""
+    m = med8.reshape(8, -1)
+    pos = geom.positions()
+    out = [None] * 9
+    out[0] = reg[0] * m[0][pos[1]][:, None]
+    out[1] = reg[1] * m[1][pos[2]][:, None]
+    out[2] = reg[2] * m[2][pos[3]][:, None]
+    l2m = m[3][pos[0]][:, None]
+    lam = m[4][pos[0]][:, None]
+    out[3] = reg[3] * l2m + (reg[4] + reg[5]) * lam
+    out[4] = reg[4] * l2m + (reg[3] + reg[5]) * lam
+    out[5] = reg[5] * l2m + (reg[3] + reg[4]) * lam
+    out[6] = reg[6] * m[5][pos[4]][:, None]
+    out[7] = reg[7] * m[6][pos[5]][:, None]
+    out[8] = reg[8] * m[7][pos[6]][:, None]
+    inv = 1.0 / dxyz
+    return [o * inv for o in out]

@@ -1,0 +1,816 @@
+// Host driver and C ABI of the gfx950 elastic propagator (see include/elastic_iso_3d.h).
+//
+// One handle = one grid/time axis on one device: the reference's
+// init*/allocate*/prop*/deallocate* functions and their file-scope state
+// (elastic_iso_float/include/varElaDeclare_3D.h:51-86) become fields of the
+// handle. The time loops follow the reference's orders:
+//   nonlinear forward  nonlinearPropElasticGpuFunctions_3D.cu:909-925
+//   nonlinear adjoint  nonlinearPropElasticGpuFunctions_3D.cu:1330-1345
+//   Born forward       BornElasticGpuFunctions_3D.cu:1268-1407
+//   Born adjoint       BornElasticGpuFunctions_3D.cu:2147-2281
+// with the sponge fused into the stencil launches and the coarse-time
+// wavefield of the Born operators kept resident in HBM instead of pinned host
+// memory (one HBM3E device holds 288 GB).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/elastic_iso_3d.h"
+#include "ela_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ELA_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+#define CHK(expr)              \
+  do {                         \
+    int r_ = (expr);           \
+    if (r_ != ELA_OK) return r_; \
+  } while (0)
+
+const double kC[4] = {1.196289062541883, -0.079752604188901, 0.009570312506634, -6.975446437140719e-04};
+const int kCompGrid[9] = {1, 2, 3, 0, 0, 0, 4, 5, 6};
+
+// Device buffer owned by the library.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int alloc(size_t b) {
+    if (b <= bytes && p) return ELA_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return ELA_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) return fail(ELA_ERR_MEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+    bytes = b;
+    return ELA_OK;
+  }
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct ela_handle {
+  ela_config cfg;
+  int64_t n_model = 0;      // nz*nx*ny
+  int64_t n_pad = 0;        // per-array stride (rounded up to 64 elements)
+  int esize = 4;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf wave;              // 2 x 9 arrays (p0, p1)
+  DevBuf med;               // 8 arrays
+  DevBuf taper;             // min_pad values
+  DevBuf scratch_tm;        // time-major trace buffer
+  DevBuf coarse;            // Born coarse wavefield [nts][3][n_pad] and slices
+  DevBuf ss;                // Born secondary sources 2 x 9 arrays
+  DevBuf staging;           // device copies for host-memory calls
+  bool have_medium = false;
+  std::vector<double> filt; // 2*(sub+1)
+  double last_ms = 0.0;
+  int64_t last_steps = 0;
+  int by_x = 8;             // BX template choice
+  int ychunk = 32;
+};
+
+namespace {
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+template <typename T>
+T* arr(DevBuf& b, int64_t stride, int k) {
+  return reinterpret_cast<T*>(b.p) + stride * k;
+}
+
+int check_cfg(const ela_config* c) {
+  if (!c) return fail(ELA_ERR_ARG, "null config");
+  if (c->nz < 2 * ela::FAT + 1 || c->nx < 2 * ela::FAT + 1 || c->ny < 2 * ela::FAT + 1)
+    return fail(ELA_ERR_ARG, "grid %lld x %lld x %lld too small (need > 2*FAT per axis)",
+                (long long)c->nz, (long long)c->nx, (long long)c->ny);
+  if (c->nz > (1 << 30) || c->nx > (1 << 30) || c->ny > (1 << 30))
+    return fail(ELA_ERR_ARG, "grid axis too large");
+  if (c->nts < 2) return fail(ELA_ERR_ARG, "nts must be >= 2");
+  if (c->sub < 1) return fail(ELA_ERR_ARG, "sub must be >= 1");
+  if (c->precision != ELA_FP32 && c->precision != ELA_FP64)
+    return fail(ELA_ERR_ARG, "precision must be ELA_FP32 or ELA_FP64");
+  if (!(c->dz > 0 && c->dx > 0 && c->dy > 0 && c->dts > 0)) return fail(ELA_ERR_ARG, "non-positive spacing");
+  if (c->min_pad < 0) return fail(ELA_ERR_ARG, "negative min_pad");
+  const int64_t half = std::min(c->nz, c->nx) / 2;
+  if (c->min_pad > half) return fail(ELA_ERR_ARG, "min_pad larger than half the grid");
+  return ELA_OK;
+}
+
+int check_points(const ela_handle* h, const ela_points* p, const char* what) {
+  if (!p) return fail(ELA_ERR_ARG, "null %s point list", what);
+  for (int g = 0; g < 7; ++g) {
+    if (p->n[g] < 0) return fail(ELA_ERR_ARG, "%s grid %d: negative count", what, g);
+    if (p->n[g] > 0 && !p->pos[g]) return fail(ELA_ERR_ARG, "%s grid %d: null positions", what, g);
+  }
+  (void)h;
+  return ELA_OK;
+}
+
+// Host-side validation of position values (host pointers only): every point
+// must be an interior cell, so that injection/recording never leaves the grid.
+int check_positions_host(const ela_handle* h, const ela_points* p, const char* what) {
+  const ela_config& c = h->cfg;
+  for (int g = 0; g < 7; ++g)
+    for (int64_t k = 0; k < p->n[g]; ++k) {
+      const int64_t v = p->pos[g][k];
+      if (v < 0 || v >= h->n_model) return fail(ELA_ERR_ARG, "%s grid %d point %lld out of the grid", what, g, (long long)k);
+      const int64_t iy = v / (c.nz * c.nx), r = v % (c.nz * c.nx), ix = r / c.nz, iz = r % c.nz;
+      if (iz < ela::FAT || iz >= c.nz - ela::FAT || ix < ela::FAT || ix >= c.nx - ela::FAT || iy < ela::FAT ||
+          iy >= c.ny - ela::FAT)
+        return fail(ELA_ERR_ARG, "%s grid %d point %lld lies in the FAT layer", what, g, (long long)k);
+    }
+  return ELA_OK;
+}
+
+template <typename T>
+void set_coefs(const ela_handle* h, T* cx, T* cy, T* cz) {
+  for (int k = 0; k < 4; ++k) {
+    cz[k] = T(kC[k] / h->cfg.dz);
+    cx[k] = T(kC[k] / h->cfg.dx);
+    cy[k] = T(kC[k] / h->cfg.dx);  // reference: y coefficients use dx
+  }
+}
+
+struct Tiles {
+  int nbz, nbx, nby, nblocks;
+};
+
+Tiles tiles_for(const ela_handle* h, int bx, int ychunk) {
+  Tiles t;
+  const int64_t iz = h->cfg.nz - 2 * ela::FAT, ix = h->cfg.nx - 2 * ela::FAT, iy = h->cfg.ny - 2 * ela::FAT;
+  t.nbz = (int)((iz + ela::TZ - 1) / ela::TZ);
+  t.nbx = (int)((ix + bx - 1) / bx);
+  t.nby = (int)((iy + ychunk - 1) / ychunk);
+  t.nblocks = t.nbz * t.nbx * t.nby;
+  return t;
+}
+
+template <typename T>
+ela::StepArgs<T> base_args(ela_handle* h, T* o[9], T* c[9]) {
+  ela::StepArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  for (int k = 0; k < 9; ++k) { a.o[k] = o[k]; a.c[k] = c[k]; }
+  for (int k = 0; k < 8; ++k) a.m[k] = arr<T>(h->med, h->n_pad, k);
+  set_coefs<T>(h, a.cx, a.cy, a.cz);
+  a.taper = reinterpret_cast<const T*>(h->taper.p);
+  a.min_pad = (int)h->cfg.min_pad;
+  a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
+  a.ys = h->cfg.nz * h->cfg.nx;
+  a.ychunk = h->ychunk;
+  return a;
+}
+
+// which: bit 0 = velocity kernel, bit 1 = stress kernel
+template <typename T, int BX, int MODE, bool SS, bool ACCV, bool ACCS>
+int launch_step_bx(ela_handle* h, ela::StepArgs<T> a, int which) {
+  Tiles t = tiles_for(h, BX, a.ychunk);
+  a.nbz = t.nbz; a.nbx = t.nbx; a.nby = t.nby;
+  dim3 block(ela::TZ, BX);
+  if (which & 1) hipLaunchKernelGGL((ela::k_velocity<T, BX, MODE, SS, ACCV>), dim3(t.nblocks), block, 0, h->stream, a);
+  if (which & 2) hipLaunchKernelGGL((ela::k_stress<T, BX, MODE, SS, ACCS>), dim3(t.nblocks), block, 0, h->stream, a);
+  return ELA_OK;
+}
+
+template <typename T, int MODE, bool SS, bool ACCV, bool ACCS>
+int launch_step(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
+  switch (h->by_x) {
+    case 4: return launch_step_bx<T, 4, MODE, SS, ACCV, ACCS>(h, a, which);
+    case 16: return launch_step_bx<T, 16, MODE, SS, ACCV, ACCS>(h, a, which);
+    default: return launch_step_bx<T, 8, MODE, SS, ACCV, ACCS>(h, a, which);
+  }
+}
+
+
+
+template <typename T>
+int fill(ela_handle* h, void* p, int64_t n) {
+  if (n <= 0) return ELA_OK;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL((ela::k_fill<T>), dim3((unsigned)blocks), dim3(256), 0, h->stream, (T*)p, n, T(0));
+  return ELA_OK;
+}
+
+// Point-list description on the device for one call.
+template <typename T>
+struct PointSet {
+  ela::PointArgs<T> a;
+  int64_t total = 0;
+};
+
+template <typename T>
+PointSet<T> make_points(ela_handle* h, const ela_points* pts, T* const field[9], T* const tr[9],
+                        bool time_major, int64_t tm_cols) {
+  PointSet<T> s;
+  std::memset(&s.a, 0, sizeof(s.a));
+  int64_t off = 0;
+  for (int c = 0; c < 9; ++c) {
+    const int g = kCompGrid[c];
+    s.a.pos[c] = pts->pos[g];
+    s.a.n[c] = pts->n[g];
+    s.a.off[c] = off;
+    s.a.field[c] = field[c];
+    s.a.tr[c] = time_major ? tr[0] + off : tr[c];
+    off += pts->n[g];
+  }
+  s.a.off[9] = off;
+  s.total = off;
+  s.a.tr_stride_pt = time_major ? 1 : h->cfg.nts;
+  s.a.tr_stride_t = time_major ? tm_cols : 1;
+  s.a.taper = reinterpret_cast<const T*>(h->taper.p);
+  s.a.min_pad = (int)h->cfg.min_pad;
+  s.a.nz = (int)h->cfg.nz; s.a.nx = (int)h->cfg.nx; s.a.ny = (int)h->cfg.ny;
+  return s;
+}
+
+template <typename T, bool ACC>
+void launch_inject(ela_handle* h, ela::PointArgs<T> a, int its, double fa, double fb) {
+  if (a.off[9] == 0) return;
+  a.its = its; a.fa = T(fa); a.fb = T(fb);
+  const unsigned nb = (unsigned)((a.off[9] + 255) / 256);
+  hipLaunchKernelGGL((ela::k_inject<T, ACC>), dim3(nb), dim3(256), 0, h->stream, a);
+}
+
+template <typename T>
+void launch_record(ela_handle* h, ela::PointArgs<T> a, int its, double fa, double fb) {
+  if (a.off[9] == 0) return;
+  a.its = its; a.fa = T(fa); a.fb = T(fb);
+  const unsigned nb = (unsigned)((a.off[9] + 255) / 256);
+  hipLaunchKernelGGL((ela::k_record<T>), dim3(nb), dim3(256), 0, h->stream, a);
+}
+
+template <typename T>
+void launch_transpose(ela_handle* h, const T* in, T* out, int64_t rows, int64_t cols, int64_t ld_in, int64_t ld_out) {
+  if (rows <= 0 || cols <= 0) return;
+  dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+  hipLaunchKernelGGL((ela::k_transpose<T>), grid, dim3(256), 0, h->stream, in, out, rows, cols, ld_in, ld_out);
+}
+
+// ---- argument staging: host pointers are copied to library device buffers.
+struct Staged {
+  const int64_t* pos_src[7];
+  const int64_t* pos_rec[7];
+  void* tr_src[9];
+  void* tr_rec[9];
+  void* dmod = nullptr;
+};
+
+int64_t trace_elems(const ela_points* p, int64_t nts) {
+  int64_t n = 0;
+  for (int c = 0; c < 9; ++c) n += p->n[kCompGrid[c]] * nts;
+  return n;
+}
+
+int stage_in(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp, const ela_traces* rt,
+             const void* dmod, bool copy_src, bool copy_rec, bool copy_dmod, Staged& s, int mem) {
+  if (mem == ELA_MEM_DEVICE) {
+    for (int g = 0; g < 7; ++g) { s.pos_src[g] = sp->pos[g]; s.pos_rec[g] = rp->pos[g]; }
+    for (int c = 0; c < 9; ++c) { s.tr_src[c] = st->tr[c]; s.tr_rec[c] = rt->tr[c]; }
+    s.dmod = const_cast<void*>(dmod);
+    return ELA_OK;
+  }
+  const int64_t nts = h->cfg.nts;
+  int64_t npos = 0;
+  for (int g = 0; g < 7; ++g) npos += sp->n[g] + rp->n[g];
+  const int64_t nsrc = trace_elems(sp, nts), nrec = trace_elems(rp, nts);
+  const int64_t ndm = dmod ? 8 * h->n_pad : 0;
+  const size_t bytes = npos * 8 + (nsrc + nrec + ndm) * h->esize + 64 * 256;  // 256-byte rounding of <= 33 pieces
+  CHK(h->staging.alloc(bytes));
+  char* base = (char*)h->staging.p;
+  size_t off = 0;
+  auto take = [&](size_t b) { char* p = base + off; off += (b + 255) / 256 * 256; return p; };
+  for (int g = 0; g < 7; ++g) {
+    int64_t* d = (int64_t*)take(sp->n[g] * 8);
+    if (sp->n[g]) HIPCHK(hipMemcpyAsync(d, sp->pos[g], sp->n[g] * 8, hipMemcpyHostToDevice, h->stream));
+    s.pos_src[g] = d;
+    int64_t* e = (int64_t*)take(rp->n[g] * 8);
+    if (rp->n[g]) HIPCHK(hipMemcpyAsync(e, rp->pos[g], rp->n[g] * 8, hipMemcpyHostToDevice, h->stream));
+    s.pos_rec[g] = e;
+  }
+  for (int c = 0; c < 9; ++c) {
+    const int64_t ns = sp->n[kCompGrid[c]] * nts, nr = rp->n[kCompGrid[c]] * nts;
+    s.tr_src[c] = take(ns * h->esize);
+    if (copy_src && ns) HIPCHK(hipMemcpyAsync(s.tr_src[c], st->tr[c], ns * h->esize, hipMemcpyHostToDevice, h->stream));
+    s.tr_rec[c] = take(nr * h->esize);
+    if (copy_rec && nr) HIPCHK(hipMemcpyAsync(s.tr_rec[c], rt->tr[c], nr * h->esize, hipMemcpyHostToDevice, h->stream));
+  }
+  if (dmod) {
+    s.dmod = take(8 * h->n_pad * h->esize);
+    if (copy_dmod)
+      for (int k = 0; k < 8; ++k)
+        HIPCHK(hipMemcpyAsync((char*)s.dmod + k * h->n_pad * h->esize, (const char*)dmod + k * h->n_model * h->esize,
+                              h->n_model * h->esize, hipMemcpyHostToDevice, h->stream));
+  }
+  return ELA_OK;
+}
+
+int stage_out_traces(ela_handle* h, const ela_points* p, void* const dev[9], const ela_traces* host) {
+  for (int c = 0; c < 9; ++c) {
+    const int64_t n = p->n[kCompGrid[c]] * h->cfg.nts;
+    if (n) HIPCHK(hipMemcpyAsync(host->tr[c], dev[c], n * h->esize, hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return ELA_OK;
+}
+
+// ---- the fine-step loops --------------------------------------------------
+
+template <typename T>
+struct Wave {
+  T* p0[9];
+  T* p1[9];
+  void swap() { for (int k = 0; k < 9; ++k) std::swap(p0[k], p1[k]); }
+};
+
+template <typename T>
+Wave<T> wave_ptrs(ela_handle* h) {
+  Wave<T> w;
+  for (int k = 0; k < 9; ++k) {
+    w.p0[k] = arr<T>(h->wave, h->n_pad, k);
+    w.p1[k] = arr<T>(h->wave, h->n_pad, 9 + k);
+  }
+  return w;
+}
+
+int64_t interior_cells(const ela_handle* h) {
+  return (h->cfg.nz - 2 * ela::FAT) * (h->cfg.nx - 2 * ela::FAT) * (h->cfg.ny - 2 * ela::FAT);
+}
+
+template <typename T>
+int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, const ela_points* rp_in,
+           const ela_traces* rt_in, bool adjoint, int mem) {
+  if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
+  CHK(check_points(h, sp_in, "source"));
+  CHK(check_points(h, rp_in, "receiver"));
+  if (mem == ELA_MEM_HOST) {
+    CHK(check_positions_host(h, sp_in, "source"));
+    CHK(check_positions_host(h, rp_in, "receiver"));
+  }
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1;
+  Staged s;
+  CHK(stage_in(h, sp_in, st_in, rp_in, rt_in, nullptr, !adjoint, adjoint, false, s, mem));
+  ela_points sp = *sp_in, rp = *rp_in;
+  for (int g = 0; g < 7; ++g) { sp.pos[g] = s.pos_src[g]; rp.pos[g] = s.pos_rec[g]; }
+
+  // time-major receiver buffer [nts][nrec_total]
+  int64_t nrec_tot = 0;
+  for (int c = 0; c < 9; ++c) nrec_tot += rp.n[kCompGrid[c]];
+  CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
+  T* tm = (T*)h->scratch_tm.p;
+
+  Wave<T> w = wave_ptrs<T>(h);
+  CHK(fill<T>(h, h->wave.p, 18 * h->n_pad));
+  T* src_tr[9];
+  T* rec_tr[9];
+  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)s.tr_src[c]; rec_tr[c] = (T*)s.tr_rec[c]; }
+  T* tmv[9];
+  for (int c = 0; c < 9; ++c) tmv[c] = tm;
+
+  // the receiver-side point set (time-major), and the source-side set ([n][nts])
+  if (!adjoint) {
+    CHK(fill<T>(h, tm, nrec_tot * nts));
+  } else {
+    // data -> time major
+    int64_t off = 0;
+    for (int c = 0; c < 9; ++c) {
+      const int64_t n = rp.n[kCompGrid[c]];
+      launch_transpose<T>(h, rec_tr[c], tm + off, n, nts, nts, nrec_tot);
+      off += n;
+    }
+    for (int c = 0; c < 9; ++c) CHK(fill<T>(h, src_tr[c], sp.n[kCompGrid[c]] * nts));
+  }
+
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  const double* f = h->filt.data();
+  if (!adjoint) {
+    for (int64_t its = 0; its < nts - 1; ++its)
+      for (int64_t it2 = 1; it2 <= sub; ++it2) {
+        ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+        CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a)));
+        PointSet<T> S = make_points<T>(h, &sp, w.p0, src_tr, false, 0);
+        launch_inject<T, false>(h, S.a, (int)its, f[it2 - 1], f[hh + it2 - 1]);
+        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
+        launch_record<T>(h, R.a, (int)its, f[it2], f[hh + it2]);
+        w.swap();
+      }
+  } else {
+    for (int64_t its = nts - 2; its >= 0; --its)
+      for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
+        ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+        CHK((launch_step<T, ela::MODE_ADJ, false, false, false>(h, a)));
+        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
+        launch_inject<T, false>(h, R.a, (int)its, f[it2 + 1], f[hh + it2 + 1]);
+        PointSet<T> S = make_points<T>(h, &sp, w.p0, src_tr, false, 0);
+        launch_record<T>(h, S.a, (int)its, f[it2], f[hh + it2]);
+        w.swap();
+      }
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipGetLastError());
+  h->last_steps = (nts - 1) * sub;
+
+  if (!adjoint) {
+    int64_t off = 0;
+    for (int c = 0; c < 9; ++c) {
+      const int64_t n = rp.n[kCompGrid[c]];
+      launch_transpose<T>(h, tm + off, rec_tr[c], nts, n, nrec_tot, nts);
+      off += n;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  if (mem == ELA_MEM_HOST) {
+    if (!adjoint) CHK(stage_out_traces(h, rp_in, s.tr_rec, rt_in));
+    else CHK(stage_out_traces(h, sp_in, s.tr_src, st_in));
+  }
+  return ELA_OK;
+}
+
+// Pass 1 of the Born operators: propagate the background source and keep the
+// coarse-time vx, vy, vz (interpWavefieldVxVyVz_3D accumulation,
+// BornElasticGpuFunctions_3D.cu:1268-1316) resident in HBM as V[nts][3][n_pad].
+template <typename T>
+int born_pass1(ela_handle* h, const ela_points& sp, T* const src_tr[9], T* V) {
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1;
+  const int64_t np = h->n_pad;
+  Wave<T> w = wave_ptrs<T>(h);
+  CHK(fill<T>(h, h->wave.p, 18 * np));
+  CHK(fill<T>(h, V, (int64_t)nts * 3 * np));
+  const double* f = h->filt.data();
+  for (int64_t its = 0; its < nts - 1; ++its) {
+    T* L = V + its * 3 * np;
+    T* R = V + (its + 1) * 3 * np;
+    for (int64_t it2 = 1; it2 <= sub; ++it2) {
+      ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+      for (int k = 0; k < 3; ++k) { a.accl[k] = L + k * np; a.accr[k] = R + k * np; }
+      a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
+      CHK((launch_step<T, ela::MODE_FWD, false, true, false>(h, a)));
+      PointSet<T> S = make_points<T>(h, &sp, w.p0, src_tr, false, 0);
+      for (int k = 0; k < 3; ++k) { S.a.accl[k] = a.accl[k]; S.a.accr[k] = a.accr[k]; }
+      S.a.accfl = a.accfl; S.a.accfr = a.accfr;
+      launch_inject<T, true>(h, S.a, (int)its, f[it2 - 1], f[hh + it2 - 1]);
+      w.swap();
+    }
+  }
+  return ELA_OK;
+}
+
+template <typename T>
+ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* V, int64_t its, const T* dm, T* dmo, T* const ss[9]) {
+  ela::ImagingArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  const int64_t np = h->n_pad, nts = h->cfg.nts;
+  auto slice = [&](int64_t t, int k) -> const T* { return (t < 0 || t >= nts) ? nullptr : V + t * 3 * np + k * np; };
+  for (int k = 0; k < 3; ++k) { a.o[k] = slice(its - 1, k); a.c[k] = slice(its, k); a.n[k] = slice(its + 1, k); }
+  for (int k = 0; k < 8; ++k) {
+    if (dm) a.dm[k] = dm + k * np;
+    if (dmo) a.dmo[k] = dmo + k * np;
+  }
+  for (int k = 0; k < 9; ++k) a.ss[k] = ss[k];
+  set_coefs<T>(h, a.cx, a.cy, a.cz);
+  a.dts_inv = T(1.0 / (2.0 * h->cfg.dts));
+  a.its = (int)its; a.nts = (int)nts;
+  a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
+  a.ys = h->cfg.nz * h->cfg.nx;
+  a.ychunk = h->ychunk;
+  Tiles t = tiles_for(h, 8, a.ychunk);
+  a.nbz = t.nbz; a.nbx = t.nbx; a.nby = t.nby;
+  return a;
+}
+
+template <typename T, bool ADJ>
+void launch_imaging(ela_handle* h, const ela::ImagingArgs<T>& a) {
+  Tiles t = tiles_for(h, 8, a.ychunk);
+  hipLaunchKernelGGL((ela::k_imaging<T, 8, ADJ>), dim3(t.nblocks), dim3(ela::TZ, 8), 0, h->stream, a);
+}
+
+template <typename T>
+int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, const void* dmod_in,
+             void* dmod_out, const ela_points* rp_in, const ela_traces* rt_in, bool adjoint, int mem) {
+  if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
+  if (h->cfg.nts < 3) return fail(ELA_ERR_ARG, "Born operators need nts >= 3");
+  CHK(check_points(h, sp_in, "source"));
+  CHK(check_points(h, rp_in, "receiver"));
+  if (mem == ELA_MEM_HOST) {
+    CHK(check_positions_host(h, sp_in, "source"));
+    CHK(check_positions_host(h, rp_in, "receiver"));
+  }
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, np = h->n_pad;
+  Staged s;
+  CHK(stage_in(h, sp_in, st_in, rp_in, rt_in, adjoint ? dmod_out : dmod_in, true, adjoint, true, s, mem));
+  ela_points sp = *sp_in, rp = *rp_in;
+  for (int g = 0; g < 7; ++g) { sp.pos[g] = s.pos_src[g]; rp.pos[g] = s.pos_rec[g]; }
+  T* src_tr[9];
+  T* rec_tr[9];
+  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)s.tr_src[c]; rec_tr[c] = (T*)s.tr_rec[c]; }
+  // Device perturbation arrays: with device memory the caller's arrays are
+  // [8][n_model]; the kernels index them with the n_pad stride, so copy into a
+  // padded buffer when the two differ.
+  T* dm = (T*)s.dmod;
+  const bool dev_unpadded = (mem == ELA_MEM_DEVICE) && (np != h->n_model);
+  DevBuf dmpad;
+  if (dev_unpadded) {
+    CHK(dmpad.alloc(8 * np * sizeof(T)));
+    for (int k = 0; k < 8; ++k)
+      HIPCHK(hipMemcpyAsync((T*)dmpad.p + k * np, (const T*)s.dmod + k * h->n_model, h->n_model * sizeof(T),
+                            hipMemcpyDeviceToDevice, h->stream));
+    dm = (T*)dmpad.p;
+  }
+
+  CHK(h->coarse.alloc((size_t)nts * 3 * np * sizeof(T)));
+  T* V = (T*)h->coarse.p;
+  CHK(h->ss.alloc(18 * np * sizeof(T)));
+  T* ssL[9];
+  T* ssR[9];
+  for (int k = 0; k < 9; ++k) { ssL[k] = arr<T>(h->ss, np, k); ssR[k] = arr<T>(h->ss, np, 9 + k); }
+
+  int64_t nrec_tot = 0;
+  for (int c = 0; c < 9; ++c) nrec_tot += rp.n[kCompGrid[c]];
+  CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
+  T* tm = (T*)h->scratch_tm.p;
+  T* tmv[9];
+  for (int c = 0; c < 9; ++c) tmv[c] = tm;
+
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  CHK(born_pass1<T>(h, sp, src_tr, V));
+  Wave<T> w = wave_ptrs<T>(h);
+  CHK(fill<T>(h, h->wave.p, 18 * np));
+  CHK(fill<T>(h, h->ss.p, 18 * np));
+  const double* f = h->filt.data();
+
+  if (!adjoint) {
+    CHK(fill<T>(h, tm, nrec_tot * nts));
+    launch_imaging<T, false>(h, imaging_args<T>(h, V, 0, dm, nullptr, ssL));
+    for (int64_t its = 0; its < nts - 1; ++its) {
+      launch_imaging<T, false>(h, imaging_args<T>(h, V, its + 1, dm, nullptr, ssR));
+      for (int64_t it2 = 1; it2 <= sub; ++it2) {
+        ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+        for (int k = 0; k < 9; ++k) { a.ssl[k] = ssL[k]; a.ssr[k] = ssR[k]; }
+        a.ssfl = T(f[it2 - 1]); a.ssfr = T(f[hh + it2 - 1]);
+        CHK((launch_step<T, ela::MODE_FWD, true, false, false>(h, a)));
+        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
+        launch_record<T>(h, R.a, (int)its, f[it2], f[hh + it2]);
+        w.swap();
+      }
+      for (int k = 0; k < 9; ++k) std::swap(ssL[k], ssR[k]);
+    }
+  } else {
+    int64_t off = 0;
+    for (int c = 0; c < 9; ++c) {
+      const int64_t n = rp.n[kCompGrid[c]];
+      launch_transpose<T>(h, rec_tr[c], tm + off, n, nts, nts, nrec_tot);
+      off += n;
+    }
+    for (int64_t its = nts - 2; its >= 0; --its) {
+      for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
+        ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+        for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
+        a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
+        CHK((launch_step<T, ela::MODE_BORN_ADJ, false, true, true>(h, a)));
+        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
+        for (int k = 0; k < 9; ++k) { R.a.accl[k] = ssL[k]; R.a.accr[k] = ssR[k]; }
+        R.a.accfl = a.accfl; R.a.accfr = a.accfr;
+        launch_inject<T, true>(h, R.a, (int)its, f[it2 + 1], f[hh + it2 + 1]);
+        w.swap();
+      }
+      launch_imaging<T, true>(h, imaging_args<T>(h, V, its + 1, nullptr, dm, ssR));
+      for (int k = 0; k < 9; ++k) std::swap(ssL[k], ssR[k]);
+      for (int k = 0; k < 9; ++k) CHK(fill<T>(h, ssL[k], np));
+    }
+    launch_imaging<T, true>(h, imaging_args<T>(h, V, 0, nullptr, dm, ssR));
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipGetLastError());
+  h->last_steps = 2 * (nts - 1) * sub;
+
+  if (!adjoint) {
+    int64_t off = 0;
+    for (int c = 0; c < 9; ++c) {
+      const int64_t n = rp.n[kCompGrid[c]];
+      launch_transpose<T>(h, tm + off, rec_tr[c], nts, n, nrec_tot, nts);
+      off += n;
+    }
+    HIPCHK(hipGetLastError());
+    if (mem == ELA_MEM_HOST) CHK(stage_out_traces(h, rp_in, s.tr_rec, rt_in));
+  } else {
+    if (dev_unpadded)
+      for (int k = 0; k < 8; ++k)
+        HIPCHK(hipMemcpyAsync((T*)s.dmod + k * h->n_model, dm + k * np, h->n_model * sizeof(T),
+                              hipMemcpyDeviceToDevice, h->stream));
+    if (mem == ELA_MEM_HOST) {
+      for (int k = 0; k < 8; ++k)
+        HIPCHK(hipMemcpyAsync((char*)dmod_out + k * h->n_model * sizeof(T), dm + k * np, h->n_model * sizeof(T),
+                              hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  if (dev_unpadded) HIPCHK(hipStreamSynchronize(h->stream));  // before dmpad is freed
+  return ELA_OK;
+}
+
+template <typename T>
+int time_kernels(ela_handle* h, int reps, double* ms_v, double* ms_s) {
+  Wave<T> w = wave_ptrs<T>(h);
+  ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+  float tv = 0.f, ts = 0.f;
+  CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a, 3)));  // warm caches and code
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a, 1)));
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  HIPCHK(hipEventElapsedTime(&tv, h->ev0, h->ev1));
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a, 2)));
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  HIPCHK(hipEventElapsedTime(&ts, h->ev0, h->ev1));
+  HIPCHK(hipGetLastError());
+  *ms_v = tv / reps;
+  *ms_s = ts / reps;
+  return ELA_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* ela_last_error(void) { return g_err.c_str(); }
+
+const char* ela_version(void) { return "elastic_iso_3d gfx950 0.1"; }
+
+int ela_create(const ela_config* cfg, ela_handle** out) {
+  if (!out) return fail(ELA_ERR_ARG, "null output handle");
+  *out = nullptr;
+  CHK(check_cfg(cfg));
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(ELA_ERR_ARG, "device %d not present (%d devices)", cfg->device, ndev);
+  DeviceGuard dg(cfg->device);
+  ela_handle* h = new ela_handle();
+  h->cfg = *cfg;
+  h->esize = cfg->precision;
+  h->n_model = cfg->nz * cfg->nx * cfg->ny;
+  h->n_pad = round_up(h->n_model, 64);
+  auto bail = [&](int rc) { delete h; return rc; };
+  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(ELA_ERR_HIP, "stream create failed"));
+  h->stream = h->own_stream;
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return bail(fail(ELA_ERR_HIP, "event create failed"));
+  int rc = h->wave.alloc(18 * h->n_pad * h->esize);
+  if (rc) return bail(rc);
+  rc = h->med.alloc(8 * h->n_pad * h->esize);
+  if (rc) return bail(rc);
+  // time filter (nonlinearPropElasticGpuFunctions_3D.cu:224-241) and taper (:248-255)
+  const int64_t sub = cfg->sub, hh = sub + 1, ntw = (cfg->nts - 1) * sub + 1;
+  h->filt.resize(2 * hh);
+  const double sc = 1.0 / std::sqrt(double(ntw) / double(cfg->nts));
+  for (int64_t i = 0; i < hh; ++i) {
+    const double a = 1.0 - double(i) / double(sub);
+    h->filt[i] = a * sc;
+    h->filt[hh + i] = (1.0 - a) * sc;
+  }
+  const int64_t mp = cfg->min_pad;
+  if (mp > 0) {
+    std::vector<double> tap(mp);
+    for (int64_t d = 0; d < mp; ++d)
+      tap[d] = cfg->alpha_cos + (1.0 - cfg->alpha_cos) * std::cos(M_PI / double(mp) * double(mp - d));
+    rc = h->taper.alloc(mp * h->esize);
+    if (rc) return bail(rc);
+    if (h->esize == 4) {
+      std::vector<float> tf(tap.begin(), tap.end());
+      if (hipMemcpy(h->taper.p, tf.data(), mp * 4, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy"));
+    } else {
+      if (hipMemcpy(h->taper.p, tap.data(), mp * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy"));
+    }
+  }
+  h->by_x = cfg->precision == ELA_FP64 ? 8 : 8;
+  *out = h;
+  return ELA_OK;
+}
+
+int ela_destroy(ela_handle* h) {
+  if (!h) return ELA_OK;
+  DeviceGuard dg(h->cfg.device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  delete h;
+  return ELA_OK;
+}
+
+int ela_set_stream(ela_handle* h, void* s) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  h->stream = s ? (hipStream_t)s : h->own_stream;
+  return ELA_OK;
+}
+
+int ela_synchronize(ela_handle* h) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  DeviceGuard dg(h->cfg.device);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return ELA_OK;
+}
+
+int ela_set_medium(ela_handle* h, const void* med8, int mem) {
+  if (!h || !med8) return fail(ELA_ERR_ARG, "null argument");
+  DeviceGuard dg(h->cfg.device);
+  const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  for (int k = 0; k < 8; ++k)
+    HIPCHK(hipMemcpyAsync((char*)h->med.p + k * h->n_pad * h->esize, (const char*)med8 + k * h->n_model * h->esize,
+                          h->n_model * h->esize, kind, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->have_medium = true;
+  return ELA_OK;
+}
+
+int ela_nl_forward(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp, ela_traces* rt,
+                   int mem) {
+  if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
+  DeviceGuard dg(h->cfg.device);
+  return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, false, mem) : nl_run<double>(h, sp, st, rp, rt, false, mem);
+}
+
+int ela_nl_adjoint(ela_handle* h, const ela_points* sp, ela_traces* st, const ela_points* rp, const ela_traces* rt,
+                   int mem) {
+  if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
+  DeviceGuard dg(h->cfg.device);
+  return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, true, mem) : nl_run<double>(h, sp, st, rp, rt, true, mem);
+}
+
+int ela_born_forward(ela_handle* h, const ela_points* sp, const ela_traces* st, const void* dmod8,
+                     const ela_points* rp, ela_traces* rt, int mem) {
+  if (!h || !st || !rt || !dmod8) return fail(ELA_ERR_ARG, "null argument");
+  DeviceGuard dg(h->cfg.device);
+  return h->esize == 4 ? born_run<float>(h, sp, st, dmod8, nullptr, rp, rt, false, mem)
+                       : born_run<double>(h, sp, st, dmod8, nullptr, rp, rt, false, mem);
+}
+
+int ela_born_adjoint(ela_handle* h, const ela_points* sp, const ela_traces* st, void* dmod8, const ela_points* rp,
+                     const ela_traces* rt, int mem) {
+  if (!h || !st || !rt || !dmod8) return fail(ELA_ERR_ARG, "null argument");
+  DeviceGuard dg(h->cfg.device);
+  return h->esize == 4 ? born_run<float>(h, sp, st, nullptr, dmod8, rp, rt, true, mem)
+                       : born_run<double>(h, sp, st, nullptr, dmod8, rp, rt, true, mem);
+}
+
+int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* ms_stress) {
+  if (!h || reps < 1 || !ms_velocity || !ms_stress) return fail(ELA_ERR_ARG, "bad argument");
+  if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
+  DeviceGuard dg(h->cfg.device);
+  return h->esize == 4 ? time_kernels<float>(h, reps, ms_velocity, ms_stress)
+                       : time_kernels<double>(h, reps, ms_velocity, ms_stress);
+}
+
+int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_t* cells) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  float t = 0.f;
+  DeviceGuard dg(h->cfg.device);
+  HIPCHK(hipEventSynchronize(h->ev1));
+  HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  if (ms) *ms = t;
+  if (fine_steps) *fine_steps = h->last_steps;
+  if (cells) *cells = interior_cells(h);
+  return ELA_OK;
+}
+
+}  // extern "C"

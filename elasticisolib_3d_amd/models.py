@@ -1,0 +1,174 @@
+"""Synthetic models, wavelets and acquisition layouts of the benchmark workloads.
+
+Recipes follow the workload definitions of this project's benchmark (a
+homogeneous 100^3 model, a seeded 12-layer model on 400x400x250 cells, ...).
+Padding follows the reference's padding tool
+(elastic_iso_float/src/padElasticFileGpuMain_3D.cpp:31-155): edge replication,
+with zPadPlus / xPadPlus chosen so that the padded interior is a multiple of
+the block size; the y pads are symmetric. The Ricker wavelet is the
+reference's (python_float/waveletFloatMain.py:88-111) and the (Vp, Vs, rho)
+-> (rho, lambda, mu) conversion is ElasticConv_3D type 1
+(seis_utils/python/elasticParamConvertModule_3D.py:23-47).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .acquisition import PaddedAxes
+
+FAT = 4
+
+
+def ricker(nts: int, dts: float, f_dom: float, delay: float) -> np.ndarray:
+    alpha = (math.pi * f_dom) ** 2
+    t = np.arange(nts) * dts - delay
+    return (1.0 - 2.0 * alpha * t * t) * np.exp(-alpha * t * t)
+
+
+def vpvsrho_to_rholamemu(vp, vs, rho):
+    return rho, rho * (vp * vp - 2.0 * vs * vs), rho * vs * vs
+
+
+def pad_sizes(n: int, pad: int, block: int = 16):
+    """(padMinus, padPlus) so that pad + n + padPlus is a multiple of `block`."""
+    total = math.ceil(float(2 * pad + n) / float(block)) * block
+    return pad, total - n - pad
+
+
+def pad_model(m: np.ndarray, z_pad: int, x_pad: int, y_pad: int, fat: int = FAT, block: int = 16):
+    """Edge-replicating padding of a [..., ny, nx, nz] model.
+
+    Returns the padded array and (zPadMinus, zPadPlus, xPadMinus, xPadPlus, yPad)."""
+    ny, nx, nz = m.shape[-3:]
+    zm, zp = pad_sizes(nz, z_pad, block)
+    xm, xp = pad_sizes(nx, x_pad, block)
+    widths = [(0, 0)] * (m.ndim - 3) + [(fat + y_pad, fat + y_pad), (fat + xm, fat + xp), (fat + zm, fat + zp)]
+    return np.pad(m, widths, mode="edge"), (zm, zp, xm, xp, y_pad)
+
+
+def gaussian_random_field(shape, sigma_cells: float, rng: np.random.Generator) -> np.ndarray:
+    from scipy.ndimage import gaussian_filter
+    g = gaussian_filter(rng.standard_normal(shape).astype(np.float32), sigma=sigma_cells, mode="reflect")
+    return g / max(float(np.max(np.abs(g))), 1e-30)
+
+
+def layered_model(nz: int, nx: int, ny: int, seed: int, n_layers: int = 12, vp_range=(1800.0, 4500.0),
+                  pert: float = 0.05, sigma: float = 8.0):
+    """Seeded 1-D layered (Vp, Vs, rho) with a smoothed random perturbation.
+
+    Vp of equal-thickness layers: sorted U(vp_range), increasing with depth;
+    Vs = Vp/sqrt(3) * U(0.9, 1.0) per layer; Vp and Vs scaled by
+    (1 + pert * g / max|g|) with g a Gaussian-smoothed N(0,1) field; Gardner
+    density rho = 310 Vp^0.25. Arrays are [ny, nx, nz] float64."""
+    rng = np.random.default_rng(seed)
+    vp_l = np.sort(rng.uniform(vp_range[0], vp_range[1], n_layers))
+    vs_l = vp_l / math.sqrt(3.0) * rng.uniform(0.9, 1.0, n_layers)
+    layer = np.minimum((np.arange(nz) * n_layers) // nz, n_layers - 1)
+    vp1, vs1 = vp_l[layer], vs_l[layer]
+    g = gaussian_random_field((ny, nx, nz), sigma, rng) if pert > 0 else np.zeros((ny, nx, nz), np.float32)
+    scale = 1.0 + pert * g.astype(np.float64)
+    vp = vp1[None, None, :] * scale
+    vs = vs1[None, None, :] * scale
+    rho = 310.0 * vp ** 0.25
+    return vp, vs, rho
+
+
+@dataclass
+class Workload:
+    """A padded model with its time axis and one shot's acquisition."""
+
+    name: str
+    rho: np.ndarray          # [ny, nx, nz] padded
+    lame: np.ndarray
+    mu: np.ndarray
+    axes: PaddedAxes
+    nts: int
+    dts: float
+    sub: int                 # fine steps per coarse sample (2 x par sub)
+    alpha_cos: float
+    src_xyz: np.ndarray      # [n, 3] (x, y, z) model coordinates [m]
+    rec_xyz: np.ndarray      # [m, 3]
+    wavelet: np.ndarray      # [nts]
+    source_type: str         # "explosive" or "fz"
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def dtw(self) -> float:
+        return self.dts / self.sub
+
+    @property
+    def interior_cells(self) -> int:
+        a = self.axes
+        return (a.nz - 2 * FAT) * (a.nx - 2 * FAT) * (a.ny - 2 * FAT)
+
+    @property
+    def fine_steps(self) -> int:
+        return (self.nts - 1) * self.sub
+
+
+def _axes(nz, nx, ny, d, pads, fat=FAT, o=(0.0, 0.0, 0.0)):
+    zm, zp, xm, xp, yp = pads
+    return PaddedAxes(nz=nz, nx=nx, ny=ny, oz=o[0] - (fat + zm) * d, ox=o[1] - (fat + xm) * d,
+                      oy=o[2] - (fat + yp) * d, dz=d, dx=d, dy=d, fat=fat, z_pad_minus=zm, z_pad_plus=zp,
+                      x_pad_minus=xm, x_pad_plus=xp, y_pad=yp)
+
+
+def homogeneous_100(nts: int = 151) -> Workload:
+    """100^3 cells, d = 10 m, 20-cell sponge, Vp 2500, Vs 1440, rho 2000; vertical
+    point force at the centre, 15 Hz Ricker (delay 1/f), dts = 4 ms with 4
+    fine sub-steps per coarse sample of the par file (8 after doubling),
+    0.6 s; 81 receivers on a line at z = 5 cells."""
+    n, d, pad = 100, 10.0, 20
+    vp = np.full((n, n, n), 2500.0)
+    vs = np.full((n, n, n), 1440.0)
+    rho0 = np.full((n, n, n), 2000.0)
+    rho, lam, mu = vpvsrho_to_rholamemu(vp, vs, rho0)
+    m, pads = pad_model(np.stack([rho, lam, mu]), pad, pad, pad)
+    ny, nx, nz = m.shape[1:]
+    ax = _axes(nz, nx, ny, d, pads)
+    dts = 0.004
+    src = np.array([[50 * d, 50 * d, 50 * d]])
+    rec = np.array([[x * d, 50 * d, 5 * d] for x in range(10, 91)])
+    return Workload("homogeneous_100", m[0], m[1], m[2], ax, nts, dts, 8, 0.99, src, rec,
+                    ricker(nts, dts, 15.0, 1.0 / 15.0), "fz")
+
+
+def layered_400(nts: int = 626, seed: int = 2024, n_rec_axis: int = 200, sigma: float = 8.0) -> Workload:
+    """400 x 400 x 250 cells (x, y, z), d = 12.5 m, 12-layer seeded model with a
+    5% smoothed perturbation (sigma 8 cells), 30-cell sponge; explosive 10 Hz
+    Ricker at the centre, z = 10 cells; 2.5 s (626 samples of 4 ms, 8 fine
+    steps each); 200 x 200 surface receivers every 2 cells."""
+    nxm, nym, nzm, d, pad = 400, 400, 250, 12.5, 30
+    vp, vs, rho0 = layered_model(nzm, nxm, nym, seed, sigma=sigma)
+    rho, lam, mu = vpvsrho_to_rholamemu(vp, vs, rho0)
+    del vp, vs, rho0
+    m, pads = pad_model(np.stack([rho, lam, mu]), pad, pad, pad)
+    del rho, lam, mu
+    ny, nx, nz = m.shape[1:]
+    ax = _axes(nz, nx, ny, d, pads)
+    dts = 0.004
+    src = np.array([[200 * d, 200 * d, 10 * d]])
+    step = 2
+    xs = np.arange(n_rec_axis) * step * d
+    gx, gy = np.meshgrid(xs, xs, indexing="ij")
+    rec = np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], axis=1)
+    return Workload("layered_400x400x250", m[0], m[1], m[2], ax, nts, dts, 8, 0.99, src, rec,
+                    ricker(nts, dts, 10.0, 0.1), "explosive")
+
+
+def source_traces(w: Workload) -> List[np.ndarray]:
+    """9 irregular source traces [n_src, nts] for the workload's source type."""
+    n = w.src_xyz.shape[0]
+    tr = [np.zeros((n, w.nts)) for _ in range(9)]
+    if w.source_type == "explosive":
+        for c in (3, 4, 5):
+            tr[c][:] = w.wavelet
+    elif w.source_type == "fz":
+        tr[2][:] = w.wavelet
+    else:
+        raise ValueError(w.source_type)
+    return tr

@@ -1,0 +1,229 @@
+"""Regular-grid propagator: a Python handle over the C ABI.
+
+This is the counterpart of the reference's GPU layer
+(elastic_iso_float/src/nonlinearPropElasticGpuFunctions_3D.cu,
+elastic_iso_float/src/BornElasticGpuFunctions_3D.cu): it propagates traces
+that already live on the regular FD grids. Arrays may be numpy arrays (host
+memory, synchronous) or torch tensors on the GPU (device memory, queued on
+torch's current stream).
+
+Layouts (shared with the reference):
+  - medium:    [8, ny, nx, nz]  rhoxDtw rhoyDtw rhozDtw lamb2MuDtw lambDtw muxzDtw muxyDtw muyzDtw
+  - positions: 7 int64 arrays   grids centre X Y Z XZ XY YZ (flat [ny][nx][nz] indices)
+  - traces:    9 arrays [nReg(grid of component), nts], components vx vy vz sxx syy szz sxz sxy syz
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import ElaConfig, ElaPoints, ElaTraces, check
+
+COMP_GRID = (1, 2, 3, 0, 0, 0, 4, 5, 6)
+COMPONENTS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxz", "sxy", "syz")
+GRIDS = ("centre", "x", "y", "z", "xz", "xy", "yz")
+
+
+def _is_torch(x) -> bool:
+    return hasattr(x, "data_ptr") and hasattr(x, "is_cuda")
+
+
+class Propagator:
+    """One propagator handle (one grid, one time axis, one device)."""
+
+    def __init__(self, nz: int, nx: int, ny: int, nts: int, sub: int, dz: float, dx: float, dy: float,
+                 dts: float, min_pad: int, alpha_cos: float = 0.99, precision: str = "f32", device: int = 0):
+        if precision not in ("f32", "f64"):
+            raise ValueError("precision must be 'f32' or 'f64'")
+        self.lib = _lib.load()
+        self.dtype = np.float32 if precision == "f32" else np.float64
+        self.precision = precision
+        self.shape = (ny, nx, nz)
+        self.nts = int(nts)
+        self.sub = int(sub)
+        self.cfg = ElaConfig(nz, nx, ny, nts, sub, dz, dx, dy, dts, min_pad, alpha_cos,
+                             _lib.ELA_FP32 if precision == "f32" else _lib.ELA_FP64, device)
+        h = ctypes.c_void_p()
+        check(self.lib.ela_create(ctypes.byref(self.cfg), ctypes.byref(h)), "ela_create")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ela_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    def _mem_of(self, *xs) -> int:
+        kinds = {_is_torch(x) and x.is_cuda for x in xs if x is not None}
+        if len(kinds) > 1:
+            raise ValueError("mix of host and device arrays in one call")
+        dev = kinds.pop() if kinds else False
+        if dev:
+            import torch
+            self.lib.ela_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+            return _lib.ELA_MEM_DEVICE
+        self.lib.ela_set_stream(self.h, None)
+        return _lib.ELA_MEM_HOST
+
+    def _ptr(self, x, dtype=None):
+        if _is_torch(x):
+            if not x.is_contiguous():
+                raise ValueError("device arrays must be contiguous")
+            if dtype is not None:
+                want = {np.float32: "torch.float32", np.float64: "torch.float64", np.int64: "torch.int64"}[dtype]
+                if str(x.dtype) != want:
+                    raise ValueError(f"expected {want}, got {x.dtype}")
+            return ctypes.c_void_p(x.data_ptr())
+        if not (x.flags["C_CONTIGUOUS"] and (dtype is None or x.dtype == dtype)):
+            raise ValueError("host arrays must be C-contiguous with the handle dtype")
+        return ctypes.c_void_p(x.ctypes.data)
+
+    def _points(self, pos: Sequence) -> ElaPoints:
+        if len(pos) != 7:
+            raise ValueError("need 7 position lists (centre X Y Z XZ XY YZ)")
+        p = ElaPoints()
+        for g in range(7):
+            a = pos[g]
+            if not _is_torch(a):
+                a = np.ascontiguousarray(a, dtype=np.int64)
+                self._keep.append(a)
+            n = a.shape[0] if len(a.shape) else 0
+            p.n[g] = int(n)
+            p.pos[g] = self._ptr(a, np.int64).value if n else None
+        return p
+
+    def _traces(self, tr: Sequence, counts) -> ElaTraces:
+        if len(tr) != 9:
+            raise ValueError("need 9 trace arrays")
+        t = ElaTraces()
+        for c in range(9):
+            a = tr[c]
+            n = counts[COMP_GRID[c]]
+            if tuple(a.shape) != (n, self.nts):
+                raise ValueError(f"component {COMPONENTS[c]}: expected shape {(n, self.nts)}, got {tuple(a.shape)}")
+            t.tr[c] = self._ptr(a, self.dtype).value if n else None
+        return t
+
+    def _alloc_traces(self, counts, like_device: bool):
+        if like_device:
+            import torch
+            td = torch.float32 if self.dtype == np.float32 else torch.float64
+            return [torch.zeros((counts[COMP_GRID[c]], self.nts), dtype=td, device="cuda") for c in range(9)]
+        return [np.zeros((counts[COMP_GRID[c]], self.nts), dtype=self.dtype) for c in range(9)]
+
+    @staticmethod
+    def _counts(pos):
+        return [int(p.shape[0]) for p in pos]
+
+    # ------------------------------------------------------------------
+    def set_medium(self, med8) -> None:
+        if tuple(med8.shape) != (8,) + self.shape:
+            raise ValueError(f"medium must have shape {(8,) + self.shape}")
+        mem = self._mem_of(med8)
+        if mem == _lib.ELA_MEM_HOST:
+            med8 = np.ascontiguousarray(med8, dtype=self.dtype)
+        check(self.lib.ela_set_medium(self.h, self._ptr(med8, self.dtype), mem), "ela_set_medium")
+
+    def nl_forward(self, src_pos, src_traces, rec_pos, rec_out=None):
+        mem = self._mem_of(*src_traces, *(rec_out or []))
+        sp, rp = self._points(src_pos), self._points(rec_pos)
+        rc = self._counts(rec_pos)
+        if mem == _lib.ELA_MEM_HOST:
+            src_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in src_traces]
+        rec = rec_out if rec_out is not None else self._alloc_traces(rc, mem == _lib.ELA_MEM_DEVICE)
+        st, rt = self._traces(src_traces, self._counts(src_pos)), self._traces(rec, rc)
+        check(self.lib.ela_nl_forward(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(rp), ctypes.byref(rt), mem),
+              "ela_nl_forward")
+        self._keep.clear()
+        return rec
+
+    def nl_adjoint(self, src_pos, rec_traces, rec_pos, src_out=None):
+        mem = self._mem_of(*rec_traces, *(src_out or []))
+        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sc = self._counts(src_pos)
+        if mem == _lib.ELA_MEM_HOST:
+            rec_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in rec_traces]
+        src = src_out if src_out is not None else self._alloc_traces(sc, mem == _lib.ELA_MEM_DEVICE)
+        st, rt = self._traces(src, sc), self._traces(rec_traces, self._counts(rec_pos))
+        check(self.lib.ela_nl_adjoint(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(rp), ctypes.byref(rt), mem),
+              "ela_nl_adjoint")
+        self._keep.clear()
+        return src
+
+    def born_forward(self, src_pos, src_traces, dmod8, rec_pos, rec_out=None):
+        mem = self._mem_of(*src_traces, dmod8)
+        if mem == _lib.ELA_MEM_HOST:
+            src_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in src_traces]
+            dmod8 = np.ascontiguousarray(dmod8, dtype=self.dtype)
+        if tuple(dmod8.shape) != (8,) + self.shape:
+            raise ValueError("perturbation must have shape (8, ny, nx, nz)")
+        sp, rp = self._points(src_pos), self._points(rec_pos)
+        rc = self._counts(rec_pos)
+        rec = rec_out if rec_out is not None else self._alloc_traces(rc, mem == _lib.ELA_MEM_DEVICE)
+        st, rt = self._traces(src_traces, self._counts(src_pos)), self._traces(rec, rc)
+        check(self.lib.ela_born_forward(self.h, ctypes.byref(sp), ctypes.byref(st), self._ptr(dmod8, self.dtype),
+                                        ctypes.byref(rp), ctypes.byref(rt), mem), "ela_born_forward")
+        self._keep.clear()
+        return rec
+
+    def born_adjoint(self, src_pos, src_traces, rec_traces, rec_pos, dmod8=None):
+        """Returns dmod8 with the image ADDED (allocated zero if None)."""
+        mem = self._mem_of(*src_traces, *rec_traces, dmod8)
+        if mem == _lib.ELA_MEM_HOST:
+            src_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in src_traces]
+            rec_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in rec_traces]
+            if dmod8 is None:
+                dmod8 = np.zeros((8,) + self.shape, self.dtype)
+        elif dmod8 is None:
+            import torch
+            dmod8 = torch.zeros((8,) + self.shape, dtype=src_traces[0].dtype, device="cuda")
+        sp, rp = self._points(src_pos), self._points(rec_pos)
+        st, rt = self._traces(src_traces, self._counts(src_pos)), self._traces(rec_traces, self._counts(rec_pos))
+        check(self.lib.ela_born_adjoint(self.h, ctypes.byref(sp), ctypes.byref(st), self._ptr(dmod8, self.dtype),
+                                        ctypes.byref(rp), ctypes.byref(rt), mem), "ela_born_adjoint")
+        self._keep.clear()
+        return dmod8
+
+    def time_step_kernels(self, reps: int = 50):
+        """Mean launch duration [ms] of the velocity and stress stencil kernels
+        (HIP events on the handle's stream; overwrites the wavefields)."""
+        mv, ms = ctypes.c_double(), ctypes.c_double()
+        self._mem_of()
+        check(self.lib.ela_time_step_kernels(self.h, int(reps), ctypes.byref(mv), ctypes.byref(ms)),
+              "ela_time_step_kernels")
+        return mv.value, ms.value
+
+    def synchronize(self):
+        check(self.lib.ela_synchronize(self.h), "ela_synchronize")
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        steps = ctypes.c_int64()
+        cells = ctypes.c_int64()
+        check(self.lib.ela_last_timing(self.h, ctypes.byref(ms), ctypes.byref(steps), ctypes.byref(cells)),
+              "ela_last_timing")
+        return ms.value, steps.value, cells.value
+
+
+def scale_medium(rho, lame, mu, dtw):
+    """Stagger and scale (rho, lambda, mu) [ny, nx, nz] into the 8 propagation
+    arrays, through the library's host routine (fdParamElastic_3D.cpp:130-183)."""
+    lib = _lib.load()
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    lame = np.ascontiguousarray(lame, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    ny, nx, nz = rho.shape
+    out = np.empty((8, ny, nx, nz), np.float64)
+    check(lib.ela_scale_medium(nz, nx, ny, float(dtw), rho.ctypes.data, lame.ctypes.data, mu.ctypes.data,
+                               out.ctypes.data), "ela_scale_medium")
+    return out

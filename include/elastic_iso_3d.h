@@ -1,0 +1,177 @@
+/*
+ * elastic_iso_3d.h -- C ABI of the MI355X (gfx950) elastic isotropic 3-D
+ * staggered-grid propagator.
+ *
+ * This is the GPU layer that the reference's C++ operator classes call. Each
+ * entry point below names the reference function it replaces (paths relative
+ * to elastic_iso_lib_3d/):
+ *
+ *   ela_create / ela_destroy      initNonlinearElasticGpu_3D / initBornElasticGpu_3D
+ *                                 (elastic_iso_float/include/nonlinearPropElasticGpuFunctions_3D.h:9,
+ *                                  elastic_iso_float/include/BornElasticGpuFunctions_3D.h:9)
+ *                                 and deallocate*ElasticGpu_3D (same headers, :11)
+ *   ela_set_medium                allocateNonlinearElasticGpu_3D / allocateBornElasticGpu_3D
+ *                                 (nonlinearPropElasticGpuFunctions_3D.h:10, BornElasticGpuFunctions_3D.h:10)
+ *   ela_nl_forward                propElasticFwdGpu_3D   (nonlinearPropElasticGpuFunctions_3D.h:14)
+ *   ela_nl_adjoint                propElasticAdjGpu_3D   (nonlinearPropElasticGpuFunctions_3D.h:18)
+ *   ela_born_forward              BornElasticFwdGpu_3D   (BornElasticGpuFunctions_3D.h:17)
+ *   ela_born_adjoint              BornElasticAdjGpu_3D   (BornElasticGpuFunctions_3D.h:22)
+ *   ela_scale_medium              the staggering/scaling of fdParamElastic_3D's constructor
+ *                                 (elastic_iso_float/src/fdParamElastic_3D.cpp:130-183)
+ *   ela_interp_linear             spaceInterpGpu_3D's constructor, linear method
+ *                                 (elastic_iso_float/include/spaceInterpGpu_3D.h:41)
+ *
+ * Differences from the reference interface, all deliberate:
+ *   - state lives in an opaque handle (no file-scope globals indexed by iGpu);
+ *   - errors are returned as status codes with ela_last_error(), never exit();
+ *   - FP32 and FP64 are one library (precision chosen per handle), not two builds;
+ *   - pointers may be host or device memory (ELA_MEM_HOST / ELA_MEM_DEVICE);
+ *     with device pointers the call is asynchronous on the handle's stream.
+ *
+ * Conventions shared with the reference:
+ *   - wavefield and medium arrays are [ny][nx][nz] with z fastest, padded grid
+ *     including the FAT=4 layer;
+ *   - the 8 scaled medium arrays are, in order, rhoxDtw rhoyDtw rhozDtw
+ *     lamb2MuDtw lambDtw muxzDtw muxyDtw muyzDtw (2*dtw/rho_x, ..., 2*dtw*(lambda+2mu),
+ *     2*dtw*lambda, 2*dtw*mu_xz, ...);
+ *   - regular-grid traces are [nReg][nts] (time fastest), one array per
+ *     component in the order vx vy vz sxx syy szz sxz sxy syz;
+ *   - regular-grid positions are flat int64 indices, one list per grid in the
+ *     order centre X Y Z XZ XY YZ; components map to grids as
+ *     vx->X vy->Y vz->Z sxx,syy,szz->centre sxz->XZ sxy->XY syz->YZ;
+ *   - positions must be unique within a grid (as produced by the reference's
+ *     spaceInterpGpu_3D::convertIrregToReg) and inside the FD interior;
+ *   - `sub` is the number of fine steps per coarse sample, i.e. the doubled
+ *     value the reference propagates with (fdParamElastic_3D.cpp:29-36).
+ */
+#ifndef ELASTIC_ISO_3D_H
+#define ELASTIC_ISO_3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define ELA_OK 0
+#define ELA_ERR_ARG 1     /* invalid argument or inconsistent sizes */
+#define ELA_ERR_HIP 2     /* a HIP runtime call failed */
+#define ELA_ERR_MEM 3     /* device allocation failed */
+#define ELA_ERR_STATE 4   /* call out of order (e.g. no medium set) */
+
+#define ELA_MEM_HOST 0
+#define ELA_MEM_DEVICE 1
+
+#define ELA_FP32 4
+#define ELA_FP64 8
+
+typedef struct ela_config {
+  int64_t nz, nx, ny;   /* padded grid sizes (including FAT) */
+  int64_t nts;          /* number of coarse time samples */
+  int64_t sub;          /* fine steps per coarse sample (2 * par "sub") */
+  double dz, dx, dy;    /* spacings [m] (y derivatives use dx, as the reference) */
+  double dts;           /* coarse sampling [s]; dtw = dts / sub */
+  int64_t min_pad;      /* sponge width: min(zPadMinus, zPadPlus, xPadMinus, xPadPlus) */
+  double alpha_cos;     /* sponge parameter alphaCos (reference default 0.99) */
+  int32_t precision;    /* ELA_FP32 or ELA_FP64 */
+  int32_t device;       /* HIP device ordinal; the library sets it for its own calls */
+} ela_config;
+
+/* Regular-grid point lists for the 7 grids. */
+typedef struct ela_points {
+  const int64_t* pos[7];
+  int64_t n[7];
+} ela_points;
+
+/* One trace array per component; element type follows the handle precision. */
+typedef struct ela_traces {
+  void* tr[9];
+} ela_traces;
+
+typedef struct ela_handle ela_handle;
+
+/* Last error message of the calling thread ("" if none). */
+const char* ela_last_error(void);
+
+/* Library version string. */
+const char* ela_version(void);
+
+/* Create / destroy a propagator for one grid and time axis. */
+int ela_create(const ela_config* cfg, ela_handle** out);
+int ela_destroy(ela_handle* h);
+
+/* Use a caller-owned hipStream_t (NULL = the handle's own stream). */
+int ela_set_stream(ela_handle* h, void* hip_stream);
+/* Wait for all work queued on the handle's stream. */
+int ela_synchronize(ela_handle* h);
+
+/* Upload the 8 scaled medium arrays (8 * nz*nx*ny values, element type of
+ * the handle precision). */
+int ela_set_medium(ela_handle* h, const void* med8, int mem);
+
+/* Host-side staggering and scaling of (rho, lambda, mu) into the 8 arrays of
+ * ela_set_medium (fdParamElastic_3D.cpp:130-183). Double precision in and out;
+ * dtw = dts / sub. */
+int ela_scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* rho,
+                     const double* lame, const double* mu, double* med8);
+
+/* Nonlinear forward: source traces -> receiver traces. `rec` is overwritten. */
+int ela_nl_forward(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
+                   const ela_points* rec_pos, ela_traces* rec, int mem);
+
+/* Nonlinear adjoint: receiver traces -> source traces. `src` is overwritten. */
+int ela_nl_adjoint(ela_handle* h, const ela_points* src_pos, ela_traces* src,
+                   const ela_points* rec_pos, const ela_traces* rec, int mem);
+
+/* Born forward: the 8 scaled, staggered perturbations (drhox drhoy drhoz
+ * dlame dmu dmuxz dmuxy dmuyz, each nz*nx*ny) -> receiver traces
+ * (overwritten). `src` are the medium-scaled source traces of the background
+ * shot. */
+int ela_born_forward(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
+                     const void* dmod8, const ela_points* rec_pos, ela_traces* rec, int mem);
+
+/* Born adjoint: receiver traces -> the 8 perturbation images, ACCUMULATED
+ * into dmod8 (as the reference, which copies the incoming arrays to the
+ * device and adds to them). */
+int ela_born_adjoint(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
+                     void* dmod8, const ela_points* rec_pos, const ela_traces* rec, int mem);
+
+/* Axes of one acquisition grid (its origin already shifted by half a cell on
+ * the staggered axes, as python_float/Elastic_iso_float_3D.py:97-110 does),
+ * with the padding parameters of the par file. */
+typedef struct ela_interp_axes {
+  int64_t nz, nx, ny;
+  double oz, ox, oy;
+  double dz, dx, dy;
+  int64_t fat, z_pad_minus, z_pad_plus, x_pad_minus, x_pad_plus, y_pad;
+  double tolerance;     /* par "errorTolerance" (reference default 1e-6) */
+} ela_interp_axes;
+
+/* Trilinear interpolation weights of n_dev devices on one grid
+ * (spaceInterpGpu_3D linear method: elastic_iso_float/src/spaceInterpGpu_3D.cpp:13-79,
+ * 113-138, 177-279). Outputs: nbr[8*n_dev] (index of each neighbour in the
+ * unique list), weight[8*n_dev], unique_pos[<= 8*n_dev] (flat grid indices in
+ * first-seen order), *n_unique. Returns ELA_ERR_ARG if a device or one of its
+ * neighbours falls outside the padded interior. */
+int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const double* z, const double* x,
+                      const double* y, int64_t* nbr, double* weight, int64_t* unique_pos,
+                      int64_t* n_unique);
+
+/* Time of the last propagation's fine-step loop on the GPU, measured with
+ * HIP events on the handle's stream [ms], and the number of fine steps and
+ * interior cells it advanced. */
+int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_t* cells);
+
+/* Diagnostic timing: launch the velocity and the stress stencil kernels of a
+ * forward fine step `reps` times each on the handle's current wavefields and
+ * return the mean duration of one launch of each [ms] (HIP events on the
+ * handle's stream). The wavefields are overwritten; no counterpart in the
+ * reference. */
+int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* ms_stress);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ELASTIC_ISO_3D_H */
