@@ -36,6 +36,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM-side bytes per fine step of the stencil pair from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by the
+    plain-copy calibration), or None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "stencil_layered_400_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_step:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def gpu_setup(workload):
     import torch
     from elasticisolib_3d_amd import propagator as P
@@ -112,7 +126,18 @@ def main():
     ap.add_argument("--nts", type=int, default=626, help="coarse samples (626 = 2.5 s)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (for counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "shots would be distributed one process per GPU, which this build does not launch.",
@@ -181,7 +206,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "bytes_per_cell": BYTES_PER_CELL_F32,
             "k_velocity_ms": round(kv, 4),
             "k_stress_ms": round(ks, 4),

@@ -23,7 +23,8 @@ EXPORTS = (
     "ela_last_error", "ela_version", "ela_create", "ela_destroy", "ela_set_stream",
     "ela_synchronize", "ela_set_medium", "ela_scale_medium", "ela_nl_forward",
     "ela_nl_adjoint", "ela_born_forward", "ela_born_adjoint", "ela_last_timing",
-    "ela_interp_linear", "ela_time_step_kernels",
+    "ela_interp_linear", "ela_time_step_kernels", "ela_set_tiling", "ela_get_tiling",
+    "ela_set_tile_order",
 )
 
 
@@ -100,6 +101,9 @@ def load() -> ctypes.CDLL:
     lib.ela_born_forward.argtypes = [vp, P, Tr, vp, P, Tr, i32]
     lib.ela_born_adjoint.argtypes = [vp, P, Tr, vp, P, Tr, i32]
     lib.ela_interp_linear.argtypes = [ctypes.POINTER(ElaInterpAxes), i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)]
+    lib.ela_set_tiling.argtypes = [vp, i32, i32]
+    lib.ela_set_tile_order.argtypes = [vp, i32]
+    lib.ela_get_tiling.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.ela_time_step_kernels.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.ela_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     for name in EXPORTS:

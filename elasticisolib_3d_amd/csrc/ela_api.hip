@@ -11,9 +11,17 @@
 // with the sponge fused into the stencil launches and the coarse-time
 // wavefield of the Born operators kept resident in HBM instead of pinned host
 // memory (one HBM3E device holds 288 GB).
+//
+// Device layout. The caller's arrays are [ny][nx][nz]. Inside the library
+// every 3-D array (wavefields, medium, perturbations, coarse wavefields) is
+// stored with a row pitch nzi that is a whole number of 128-byte cache lines
+// and with z = 0 at element zoff, where zoff + FAT is a line boundary: the
+// 64-cell z tiles of the stencil then read whole lines. Arrays are converted
+// at the boundary (2-D copies), point lists by a small kernel.
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -46,9 +54,9 @@ int fail(int code, const char* fmt, ...) {
                   __FILE__, __LINE__);                                                   \
   } while (0)
 
-#define CHK(expr)              \
-  do {                         \
-    int r_ = (expr);           \
+#define CHK(expr)                \
+  do {                           \
+    int r_ = (expr);             \
     if (r_ != ELA_OK) return r_; \
   } while (0)
 
@@ -70,15 +78,22 @@ struct DevBuf {
     bytes = b;
     return ELA_OK;
   }
-  ~DevBuf() { if (p) (void)hipFree(p); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
 };
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 }  // namespace
 
 struct ela_handle {
   ela_config cfg;
-  int64_t n_model = 0;      // nz*nx*ny
-  int64_t n_pad = 0;        // per-array stride (rounded up to 64 elements)
+  int64_t n_model = 0;      // nz*nx*ny (reference layout)
+  int64_t nzi = 0;          // internal row pitch
+  int64_t zoff = 0;         // internal z origin
+  int64_t n_int = 0;        // ny*nx*nzi
+  int64_t stride = 0;       // per-array stride in the library's buffers (n_int rounded up)
   int esize = 4;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
@@ -87,20 +102,20 @@ struct ela_handle {
   DevBuf med;               // 8 arrays
   DevBuf taper;             // min_pad values
   DevBuf scratch_tm;        // time-major trace buffer
-  DevBuf coarse;            // Born coarse wavefield [nts][3][n_pad] and slices
+  DevBuf coarse;            // Born coarse wavefield [nts][3][stride]
   DevBuf ss;                // Born secondary sources 2 x 9 arrays
-  DevBuf staging;           // device copies for host-memory calls
+  DevBuf dmod;              // Born perturbations / images, 8 arrays
+  DevBuf pos;               // converted positions (int32) and an error flag
+  DevBuf staging;           // device copies of host-memory arguments
   bool have_medium = false;
   std::vector<double> filt; // 2*(sub+1)
-  double last_ms = 0.0;
   int64_t last_steps = 0;
-  int by_x = 8;             // BX template choice
-  int ychunk = 32;
+  int by_x = 16;            // x extent of a tile
+  int ychunk = 8;           // y planes per block
+  int remap = 0;            // 1: contiguous tile ranges per XCD
 };
 
 namespace {
-
-int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 template <typename T>
 T* arr(DevBuf& b, int64_t stride, int k) {
@@ -110,44 +125,25 @@ T* arr(DevBuf& b, int64_t stride, int k) {
 int check_cfg(const ela_config* c) {
   if (!c) return fail(ELA_ERR_ARG, "null config");
   if (c->nz < 2 * ela::FAT + 1 || c->nx < 2 * ela::FAT + 1 || c->ny < 2 * ela::FAT + 1)
-    return fail(ELA_ERR_ARG, "grid %lld x %lld x %lld too small (need > 2*FAT per axis)",
-                (long long)c->nz, (long long)c->nx, (long long)c->ny);
-  if (c->nz > (1 << 30) || c->nx > (1 << 30) || c->ny > (1 << 30))
-    return fail(ELA_ERR_ARG, "grid axis too large");
+    return fail(ELA_ERR_ARG, "grid %lld x %lld x %lld too small (need > 2*FAT per axis)", (long long)c->nz,
+                (long long)c->nx, (long long)c->ny);
   if (c->nts < 2) return fail(ELA_ERR_ARG, "nts must be >= 2");
   if (c->sub < 1) return fail(ELA_ERR_ARG, "sub must be >= 1");
   if (c->precision != ELA_FP32 && c->precision != ELA_FP64)
     return fail(ELA_ERR_ARG, "precision must be ELA_FP32 or ELA_FP64");
   if (!(c->dz > 0 && c->dx > 0 && c->dy > 0 && c->dts > 0)) return fail(ELA_ERR_ARG, "non-positive spacing");
   if (c->min_pad < 0) return fail(ELA_ERR_ARG, "negative min_pad");
-  const int64_t half = std::min(c->nz, c->nx) / 2;
-  if (c->min_pad > half) return fail(ELA_ERR_ARG, "min_pad larger than half the grid");
+  if (c->min_pad > std::min(c->nz, c->nx) / 2) return fail(ELA_ERR_ARG, "min_pad larger than half the grid");
   return ELA_OK;
 }
 
-int check_points(const ela_handle* h, const ela_points* p, const char* what) {
+int check_points(const ela_points* p, const char* what) {
   if (!p) return fail(ELA_ERR_ARG, "null %s point list", what);
   for (int g = 0; g < 7; ++g) {
     if (p->n[g] < 0) return fail(ELA_ERR_ARG, "%s grid %d: negative count", what, g);
+    if (p->n[g] > (int64_t)1 << 31) return fail(ELA_ERR_ARG, "%s grid %d: too many points", what, g);
     if (p->n[g] > 0 && !p->pos[g]) return fail(ELA_ERR_ARG, "%s grid %d: null positions", what, g);
   }
-  (void)h;
-  return ELA_OK;
-}
-
-// Host-side validation of position values (host pointers only): every point
-// must be an interior cell, so that injection/recording never leaves the grid.
-int check_positions_host(const ela_handle* h, const ela_points* p, const char* what) {
-  const ela_config& c = h->cfg;
-  for (int g = 0; g < 7; ++g)
-    for (int64_t k = 0; k < p->n[g]; ++k) {
-      const int64_t v = p->pos[g][k];
-      if (v < 0 || v >= h->n_model) return fail(ELA_ERR_ARG, "%s grid %d point %lld out of the grid", what, g, (long long)k);
-      const int64_t iy = v / (c.nz * c.nx), r = v % (c.nz * c.nx), ix = r / c.nz, iz = r % c.nz;
-      if (iz < ela::FAT || iz >= c.nz - ela::FAT || ix < ela::FAT || ix >= c.nx - ela::FAT || iy < ela::FAT ||
-          iy >= c.ny - ela::FAT)
-        return fail(ELA_ERR_ARG, "%s grid %d point %lld lies in the FAT layer", what, g, (long long)k);
-    }
   return ELA_OK;
 }
 
@@ -179,13 +175,15 @@ ela::StepArgs<T> base_args(ela_handle* h, T* o[9], T* c[9]) {
   ela::StepArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   for (int k = 0; k < 9; ++k) { a.o[k] = o[k]; a.c[k] = c[k]; }
-  for (int k = 0; k < 8; ++k) a.m[k] = arr<T>(h->med, h->n_pad, k);
+  for (int k = 0; k < 8; ++k) a.m[k] = arr<T>(h->med, h->stride, k);
   set_coefs<T>(h, a.cx, a.cy, a.cz);
   a.taper = reinterpret_cast<const T*>(h->taper.p);
   a.min_pad = (int)h->cfg.min_pad;
   a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
-  a.ys = h->cfg.nz * h->cfg.nx;
+  a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
+  a.ys = h->cfg.nx * h->nzi;
   a.ychunk = h->ychunk;
+  a.remap = h->remap;
   return a;
 }
 
@@ -204,51 +202,112 @@ template <typename T, int MODE, bool SS, bool ACCV, bool ACCS>
 int launch_step(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
   switch (h->by_x) {
     case 4: return launch_step_bx<T, 4, MODE, SS, ACCV, ACCS>(h, a, which);
-    case 16: return launch_step_bx<T, 16, MODE, SS, ACCV, ACCS>(h, a, which);
-    default: return launch_step_bx<T, 8, MODE, SS, ACCV, ACCS>(h, a, which);
+    case 8: return launch_step_bx<T, 8, MODE, SS, ACCV, ACCS>(h, a, which);
+    default: return launch_step_bx<T, 16, MODE, SS, ACCV, ACCS>(h, a, which);
   }
 }
-
-
 
 template <typename T>
 int fill(ela_handle* h, void* p, int64_t n) {
   if (n <= 0) return ELA_OK;
-  int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL((ela::k_fill<T>), dim3((unsigned)blocks), dim3(256), 0, h->stream, (T*)p, n, T(0));
   return ELA_OK;
 }
 
-// Point-list description on the device for one call.
-template <typename T>
-struct PointSet {
-  ela::PointArgs<T> a;
-  int64_t total = 0;
+// 3-D array [k][ny][nx][nz] (caller) <-> library layout, k = 0..n_arrays-1.
+int copy_in_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem) {
+  const size_t es = h->esize;
+  const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  for (int k = 0; k < n_arrays; ++k)
+    HIPCHK(hipMemcpy2DAsync((char*)dst + (k * h->stride + h->zoff) * es, h->nzi * es,
+                            (const char*)src + k * h->n_model * es, h->cfg.nz * es, h->cfg.nz * es,
+                            h->cfg.nx * h->cfg.ny, kind, h->stream));
+  return ELA_OK;
+}
+
+int copy_out_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem) {
+  const size_t es = h->esize;
+  const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  for (int k = 0; k < n_arrays; ++k)
+    HIPCHK(hipMemcpy2DAsync((char*)dst + k * h->n_model * es, h->cfg.nz * es,
+                            (const char*)src + (k * h->stride + h->zoff) * es, h->nzi * es, h->cfg.nz * es,
+                            h->cfg.nx * h->cfg.ny, kind, h->stream));
+  return ELA_OK;
+}
+
+// ---- point lists ------------------------------------------------------------
+
+struct DevPoints {
+  const int32_t* pos[7];
+  int64_t n[7];
 };
 
+// Copies (host) or reads (device) the caller's int64 positions, converts them
+// to internal int32 offsets on the GPU and checks that every point lies in the
+// FD interior.
+int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, int mem, DevPoints& ds, DevPoints& dr) {
+  int64_t total = 0;
+  for (int g = 0; g < 7; ++g) total += sp->n[g] + rp->n[g];
+  // layout: [int64 staging (host calls)] [int32 converted] [int flag]
+  const size_t b64 = round_up(total * 8, 256), b32 = round_up(total * 4, 256);
+  CHK(h->pos.alloc(b64 + b32 + 256));
+  char* base = (char*)h->pos.p;
+  int64_t* s64 = (int64_t*)base;
+  int32_t* s32 = (int32_t*)(base + b64);
+  int* flag = (int*)(base + b64 + b32);
+  HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), h->stream));
+  int64_t off = 0;
+  for (int side = 0; side < 2; ++side) {
+    const ela_points* p = side ? rp : sp;
+    DevPoints& d = side ? dr : ds;
+    for (int g = 0; g < 7; ++g) {
+      const int64_t n = p->n[g];
+      d.n[g] = n;
+      d.pos[g] = s32 + off;
+      if (n) {
+        const int64_t* src = p->pos[g];
+        if (mem == ELA_MEM_HOST) {
+          HIPCHK(hipMemcpyAsync(s64 + off, p->pos[g], n * 8, hipMemcpyHostToDevice, h->stream));
+          src = s64 + off;
+        }
+        hipLaunchKernelGGL(ela::k_convert_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src,
+                           s32 + off, n, (int)h->cfg.nz, (int)h->cfg.nx, (int)h->cfg.ny, (int)h->nzi, (int)h->zoff,
+                           flag);
+      }
+      off += n;
+    }
+  }
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (bad) return fail(ELA_ERR_ARG, "a source or receiver position lies outside the FD interior");
+  return ELA_OK;
+}
+
 template <typename T>
-PointSet<T> make_points(ela_handle* h, const ela_points* pts, T* const field[9], T* const tr[9],
-                        bool time_major, int64_t tm_cols) {
-  PointSet<T> s;
-  std::memset(&s.a, 0, sizeof(s.a));
+ela::PointArgs<T> make_points(ela_handle* h, const DevPoints& pts, T* const field[9], T* const tr[9],
+                              bool time_major, int64_t tm_cols) {
+  ela::PointArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
   int64_t off = 0;
   for (int c = 0; c < 9; ++c) {
     const int g = kCompGrid[c];
-    s.a.pos[c] = pts->pos[g];
-    s.a.n[c] = pts->n[g];
-    s.a.off[c] = off;
-    s.a.field[c] = field[c];
-    s.a.tr[c] = time_major ? tr[0] + off : tr[c];
-    off += pts->n[g];
+    a.pos[c] = pts.pos[g];
+    a.n[c] = pts.n[g];
+    a.off[c] = off;
+    a.field[c] = field[c];
+    a.tr[c] = time_major ? tr[0] + off : tr[c];
+    off += pts.n[g];
   }
-  s.a.off[9] = off;
-  s.total = off;
-  s.a.tr_stride_pt = time_major ? 1 : h->cfg.nts;
-  s.a.tr_stride_t = time_major ? tm_cols : 1;
-  s.a.taper = reinterpret_cast<const T*>(h->taper.p);
-  s.a.min_pad = (int)h->cfg.min_pad;
-  s.a.nz = (int)h->cfg.nz; s.a.nx = (int)h->cfg.nx; s.a.ny = (int)h->cfg.ny;
-  return s;
+  a.off[9] = off;
+  a.tr_stride_pt = time_major ? 1 : h->cfg.nts;
+  a.tr_stride_t = time_major ? tm_cols : 1;
+  a.taper = reinterpret_cast<const T*>(h->taper.p);
+  a.min_pad = (int)h->cfg.min_pad;
+  a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
+  a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
+  return a;
 }
 
 template <typename T, bool ACC>
@@ -274,14 +333,7 @@ void launch_transpose(ela_handle* h, const T* in, T* out, int64_t rows, int64_t 
   hipLaunchKernelGGL((ela::k_transpose<T>), grid, dim3(256), 0, h->stream, in, out, rows, cols, ld_in, ld_out);
 }
 
-// ---- argument staging: host pointers are copied to library device buffers.
-struct Staged {
-  const int64_t* pos_src[7];
-  const int64_t* pos_rec[7];
-  void* tr_src[9];
-  void* tr_rec[9];
-  void* dmod = nullptr;
-};
+// ---- trace staging for host-memory calls ------------------------------------
 
 int64_t trace_elems(const ela_points* p, int64_t nts) {
   int64_t n = 0;
@@ -289,45 +341,27 @@ int64_t trace_elems(const ela_points* p, int64_t nts) {
   return n;
 }
 
-int stage_in(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp, const ela_traces* rt,
-             const void* dmod, bool copy_src, bool copy_rec, bool copy_dmod, Staged& s, int mem) {
+// Device pointers for the 9 source and 9 receiver trace arrays: the caller's
+// (device calls) or library staging copies (host calls).
+int stage_traces(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp,
+                 const ela_traces* rt, bool copy_src, bool copy_rec, int mem, void* dsrc[9], void* drec[9]) {
   if (mem == ELA_MEM_DEVICE) {
-    for (int g = 0; g < 7; ++g) { s.pos_src[g] = sp->pos[g]; s.pos_rec[g] = rp->pos[g]; }
-    for (int c = 0; c < 9; ++c) { s.tr_src[c] = st->tr[c]; s.tr_rec[c] = rt->tr[c]; }
-    s.dmod = const_cast<void*>(dmod);
+    for (int c = 0; c < 9; ++c) { dsrc[c] = st->tr[c]; drec[c] = rt->tr[c]; }
     return ELA_OK;
   }
   const int64_t nts = h->cfg.nts;
-  int64_t npos = 0;
-  for (int g = 0; g < 7; ++g) npos += sp->n[g] + rp->n[g];
-  const int64_t nsrc = trace_elems(sp, nts), nrec = trace_elems(rp, nts);
-  const int64_t ndm = dmod ? 8 * h->n_pad : 0;
-  const size_t bytes = npos * 8 + (nsrc + nrec + ndm) * h->esize + 64 * 256;  // 256-byte rounding of <= 33 pieces
+  const size_t bytes = (trace_elems(sp, nts) + trace_elems(rp, nts)) * h->esize + 18 * 256;
   CHK(h->staging.alloc(bytes));
   char* base = (char*)h->staging.p;
   size_t off = 0;
-  auto take = [&](size_t b) { char* p = base + off; off += (b + 255) / 256 * 256; return p; };
-  for (int g = 0; g < 7; ++g) {
-    int64_t* d = (int64_t*)take(sp->n[g] * 8);
-    if (sp->n[g]) HIPCHK(hipMemcpyAsync(d, sp->pos[g], sp->n[g] * 8, hipMemcpyHostToDevice, h->stream));
-    s.pos_src[g] = d;
-    int64_t* e = (int64_t*)take(rp->n[g] * 8);
-    if (rp->n[g]) HIPCHK(hipMemcpyAsync(e, rp->pos[g], rp->n[g] * 8, hipMemcpyHostToDevice, h->stream));
-    s.pos_rec[g] = e;
-  }
   for (int c = 0; c < 9; ++c) {
     const int64_t ns = sp->n[kCompGrid[c]] * nts, nr = rp->n[kCompGrid[c]] * nts;
-    s.tr_src[c] = take(ns * h->esize);
-    if (copy_src && ns) HIPCHK(hipMemcpyAsync(s.tr_src[c], st->tr[c], ns * h->esize, hipMemcpyHostToDevice, h->stream));
-    s.tr_rec[c] = take(nr * h->esize);
-    if (copy_rec && nr) HIPCHK(hipMemcpyAsync(s.tr_rec[c], rt->tr[c], nr * h->esize, hipMemcpyHostToDevice, h->stream));
-  }
-  if (dmod) {
-    s.dmod = take(8 * h->n_pad * h->esize);
-    if (copy_dmod)
-      for (int k = 0; k < 8; ++k)
-        HIPCHK(hipMemcpyAsync((char*)s.dmod + k * h->n_pad * h->esize, (const char*)dmod + k * h->n_model * h->esize,
-                              h->n_model * h->esize, hipMemcpyHostToDevice, h->stream));
+    dsrc[c] = base + off;
+    off += round_up(ns * h->esize, 256);
+    drec[c] = base + off;
+    off += round_up(nr * h->esize, 256);
+    if (copy_src && ns) HIPCHK(hipMemcpyAsync(dsrc[c], st->tr[c], ns * h->esize, hipMemcpyHostToDevice, h->stream));
+    if (copy_rec && nr) HIPCHK(hipMemcpyAsync(drec[c], rt->tr[c], nr * h->esize, hipMemcpyHostToDevice, h->stream));
   }
   return ELA_OK;
 }
@@ -337,7 +371,6 @@ int stage_out_traces(ela_handle* h, const ela_points* p, void* const dev[9], con
     const int64_t n = p->n[kCompGrid[c]] * h->cfg.nts;
     if (n) HIPCHK(hipMemcpyAsync(host->tr[c], dev[c], n * h->esize, hipMemcpyDeviceToHost, h->stream));
   }
-  HIPCHK(hipStreamSynchronize(h->stream));
   return ELA_OK;
 }
 
@@ -347,15 +380,17 @@ template <typename T>
 struct Wave {
   T* p0[9];
   T* p1[9];
-  void swap() { for (int k = 0; k < 9; ++k) std::swap(p0[k], p1[k]); }
+  void swap() {
+    for (int k = 0; k < 9; ++k) std::swap(p0[k], p1[k]);
+  }
 };
 
 template <typename T>
 Wave<T> wave_ptrs(ela_handle* h) {
   Wave<T> w;
   for (int k = 0; k < 9; ++k) {
-    w.p0[k] = arr<T>(h->wave, h->n_pad, k);
-    w.p1[k] = arr<T>(h->wave, h->n_pad, 9 + k);
+    w.p0[k] = arr<T>(h->wave, h->stride, k);
+    w.p1[k] = arr<T>(h->wave, h->stride, 9 + k);
   }
   return w;
 }
@@ -364,41 +399,40 @@ int64_t interior_cells(const ela_handle* h) {
   return (h->cfg.nz - 2 * ela::FAT) * (h->cfg.nx - 2 * ela::FAT) * (h->cfg.ny - 2 * ela::FAT);
 }
 
+int64_t total_points(const DevPoints& p) {
+  int64_t n = 0;
+  for (int c = 0; c < 9; ++c) n += p.n[kCompGrid[c]];
+  return n;
+}
+
 template <typename T>
 int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, const ela_points* rp_in,
            const ela_traces* rt_in, bool adjoint, int mem) {
   if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
-  CHK(check_points(h, sp_in, "source"));
-  CHK(check_points(h, rp_in, "receiver"));
-  if (mem == ELA_MEM_HOST) {
-    CHK(check_positions_host(h, sp_in, "source"));
-    CHK(check_positions_host(h, rp_in, "receiver"));
-  }
+  CHK(check_points(sp_in, "source"));
+  CHK(check_points(rp_in, "receiver"));
   const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1;
-  Staged s;
-  CHK(stage_in(h, sp_in, st_in, rp_in, rt_in, nullptr, !adjoint, adjoint, false, s, mem));
-  ela_points sp = *sp_in, rp = *rp_in;
-  for (int g = 0; g < 7; ++g) { sp.pos[g] = s.pos_src[g]; rp.pos[g] = s.pos_rec[g]; }
-
-  // time-major receiver buffer [nts][nrec_total]
-  int64_t nrec_tot = 0;
-  for (int c = 0; c < 9; ++c) nrec_tot += rp.n[kCompGrid[c]];
-  CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
-  T* tm = (T*)h->scratch_tm.p;
-
-  Wave<T> w = wave_ptrs<T>(h);
-  CHK(fill<T>(h, h->wave.p, 18 * h->n_pad));
+  DevPoints sp, rp;
+  CHK(convert_points(h, sp_in, rp_in, mem, sp, rp));
+  void* ds[9];
+  void* dr[9];
+  CHK(stage_traces(h, sp_in, st_in, rp_in, rt_in, !adjoint, adjoint, mem, ds, dr));
   T* src_tr[9];
   T* rec_tr[9];
-  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)s.tr_src[c]; rec_tr[c] = (T*)s.tr_rec[c]; }
+  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)ds[c]; rec_tr[c] = (T*)dr[c]; }
+
+  // time-major receiver buffer [nts][nrec_total]: coalesced recording
+  const int64_t nrec_tot = total_points(rp);
+  CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
+  T* tm = (T*)h->scratch_tm.p;
   T* tmv[9];
   for (int c = 0; c < 9; ++c) tmv[c] = tm;
 
-  // the receiver-side point set (time-major), and the source-side set ([n][nts])
+  Wave<T> w = wave_ptrs<T>(h);
+  CHK(fill<T>(h, h->wave.p, 18 * h->stride));
   if (!adjoint) {
     CHK(fill<T>(h, tm, nrec_tot * nts));
   } else {
-    // data -> time major
     int64_t off = 0;
     for (int c = 0; c < 9; ++c) {
       const int64_t n = rp.n[kCompGrid[c]];
@@ -415,10 +449,9 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
       for (int64_t it2 = 1; it2 <= sub; ++it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
         CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a)));
-        PointSet<T> S = make_points<T>(h, &sp, w.p0, src_tr, false, 0);
-        launch_inject<T, false>(h, S.a, (int)its, f[it2 - 1], f[hh + it2 - 1]);
-        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
-        launch_record<T>(h, R.a, (int)its, f[it2], f[hh + it2]);
+        launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), (int)its, f[it2 - 1],
+                                f[hh + it2 - 1]);
+        launch_record<T>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2], f[hh + it2]);
         w.swap();
       }
   } else {
@@ -426,10 +459,9 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
       for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
         CHK((launch_step<T, ela::MODE_ADJ, false, false, false>(h, a)));
-        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
-        launch_inject<T, false>(h, R.a, (int)its, f[it2 + 1], f[hh + it2 + 1]);
-        PointSet<T> S = make_points<T>(h, &sp, w.p0, src_tr, false, 0);
-        launch_record<T>(h, S.a, (int)its, f[it2], f[hh + it2]);
+        launch_inject<T, false>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2 + 1],
+                                f[hh + it2 + 1]);
+        launch_record<T>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), (int)its, f[it2], f[hh + it2]);
         w.swap();
       }
   }
@@ -447,19 +479,20 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
   }
   HIPCHK(hipGetLastError());
   if (mem == ELA_MEM_HOST) {
-    if (!adjoint) CHK(stage_out_traces(h, rp_in, s.tr_rec, rt_in));
-    else CHK(stage_out_traces(h, sp_in, s.tr_src, st_in));
+    if (!adjoint) CHK(stage_out_traces(h, rp_in, dr, rt_in));
+    else CHK(stage_out_traces(h, sp_in, ds, st_in));
+    HIPCHK(hipStreamSynchronize(h->stream));
   }
   return ELA_OK;
 }
 
 // Pass 1 of the Born operators: propagate the background source and keep the
 // coarse-time vx, vy, vz (interpWavefieldVxVyVz_3D accumulation,
-// BornElasticGpuFunctions_3D.cu:1268-1316) resident in HBM as V[nts][3][n_pad].
+// BornElasticGpuFunctions_3D.cu:1268-1316) resident in HBM as V[nts][3][stride].
 template <typename T>
-int born_pass1(ela_handle* h, const ela_points& sp, T* const src_tr[9], T* V) {
+int born_pass1(ela_handle* h, const DevPoints& sp, T* const src_tr[9], T* V) {
   const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1;
-  const int64_t np = h->n_pad;
+  const int64_t np = h->stride;
   Wave<T> w = wave_ptrs<T>(h);
   CHK(fill<T>(h, h->wave.p, 18 * np));
   CHK(fill<T>(h, V, (int64_t)nts * 3 * np));
@@ -472,10 +505,10 @@ int born_pass1(ela_handle* h, const ela_points& sp, T* const src_tr[9], T* V) {
       for (int k = 0; k < 3; ++k) { a.accl[k] = L + k * np; a.accr[k] = R + k * np; }
       a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
       CHK((launch_step<T, ela::MODE_FWD, false, true, false>(h, a)));
-      PointSet<T> S = make_points<T>(h, &sp, w.p0, src_tr, false, 0);
-      for (int k = 0; k < 3; ++k) { S.a.accl[k] = a.accl[k]; S.a.accr[k] = a.accr[k]; }
-      S.a.accfl = a.accfl; S.a.accfr = a.accfr;
-      launch_inject<T, true>(h, S.a, (int)its, f[it2 - 1], f[hh + it2 - 1]);
+      ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
+      for (int k = 0; k < 3; ++k) { S.accl[k] = a.accl[k]; S.accr[k] = a.accr[k]; }
+      S.accfl = a.accfl; S.accfr = a.accfr;
+      launch_inject<T, true>(h, S, (int)its, f[it2 - 1], f[hh + it2 - 1]);
       w.swap();
     }
   }
@@ -486,7 +519,7 @@ template <typename T>
 ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* V, int64_t its, const T* dm, T* dmo, T* const ss[9]) {
   ela::ImagingArgs<T> a;
   std::memset(&a, 0, sizeof(a));
-  const int64_t np = h->n_pad, nts = h->cfg.nts;
+  const int64_t np = h->stride, nts = h->cfg.nts;
   auto slice = [&](int64_t t, int k) -> const T* { return (t < 0 || t >= nts) ? nullptr : V + t * 3 * np + k * np; };
   for (int k = 0; k < 3; ++k) { a.o[k] = slice(its - 1, k); a.c[k] = slice(its, k); a.n[k] = slice(its + 1, k); }
   for (int k = 0; k < 8; ++k) {
@@ -498,8 +531,10 @@ ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* V, int64_t its, const T
   a.dts_inv = T(1.0 / (2.0 * h->cfg.dts));
   a.its = (int)its; a.nts = (int)nts;
   a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
-  a.ys = h->cfg.nz * h->cfg.nx;
+  a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
+  a.ys = h->cfg.nx * h->nzi;
   a.ychunk = h->ychunk;
+  a.remap = h->remap;
   Tiles t = tiles_for(h, 8, a.ychunk);
   a.nbz = t.nbz; a.nbx = t.nbx; a.nby = t.nby;
   return a;
@@ -516,33 +551,23 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
              void* dmod_out, const ela_points* rp_in, const ela_traces* rt_in, bool adjoint, int mem) {
   if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
   if (h->cfg.nts < 3) return fail(ELA_ERR_ARG, "Born operators need nts >= 3");
-  CHK(check_points(h, sp_in, "source"));
-  CHK(check_points(h, rp_in, "receiver"));
-  if (mem == ELA_MEM_HOST) {
-    CHK(check_positions_host(h, sp_in, "source"));
-    CHK(check_positions_host(h, rp_in, "receiver"));
-  }
-  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, np = h->n_pad;
-  Staged s;
-  CHK(stage_in(h, sp_in, st_in, rp_in, rt_in, adjoint ? dmod_out : dmod_in, true, adjoint, true, s, mem));
-  ela_points sp = *sp_in, rp = *rp_in;
-  for (int g = 0; g < 7; ++g) { sp.pos[g] = s.pos_src[g]; rp.pos[g] = s.pos_rec[g]; }
+  CHK(check_points(sp_in, "source"));
+  CHK(check_points(rp_in, "receiver"));
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, np = h->stride;
+  DevPoints sp, rp;
+  CHK(convert_points(h, sp_in, rp_in, mem, sp, rp));
+  void* ds[9];
+  void* dr[9];
+  CHK(stage_traces(h, sp_in, st_in, rp_in, rt_in, true, adjoint, mem, ds, dr));
   T* src_tr[9];
   T* rec_tr[9];
-  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)s.tr_src[c]; rec_tr[c] = (T*)s.tr_rec[c]; }
-  // Device perturbation arrays: with device memory the caller's arrays are
-  // [8][n_model]; the kernels index them with the n_pad stride, so copy into a
-  // padded buffer when the two differ.
-  T* dm = (T*)s.dmod;
-  const bool dev_unpadded = (mem == ELA_MEM_DEVICE) && (np != h->n_model);
-  DevBuf dmpad;
-  if (dev_unpadded) {
-    CHK(dmpad.alloc(8 * np * sizeof(T)));
-    for (int k = 0; k < 8; ++k)
-      HIPCHK(hipMemcpyAsync((T*)dmpad.p + k * np, (const T*)s.dmod + k * h->n_model, h->n_model * sizeof(T),
-                            hipMemcpyDeviceToDevice, h->stream));
-    dm = (T*)dmpad.p;
-  }
+  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)ds[c]; rec_tr[c] = (T*)dr[c]; }
+
+  // perturbations (forward) / incoming images (adjoint, accumulated) in the library layout
+  CHK(h->dmod.alloc(8 * np * sizeof(T)));
+  T* dm = (T*)h->dmod.p;
+  CHK(fill<T>(h, dm, 8 * np));
+  CHK(copy_in_3d(h, dm, adjoint ? dmod_out : dmod_in, 8, mem));
 
   CHK(h->coarse.alloc((size_t)nts * 3 * np * sizeof(T)));
   T* V = (T*)h->coarse.p;
@@ -551,8 +576,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
   T* ssR[9];
   for (int k = 0; k < 9; ++k) { ssL[k] = arr<T>(h->ss, np, k); ssR[k] = arr<T>(h->ss, np, 9 + k); }
 
-  int64_t nrec_tot = 0;
-  for (int c = 0; c < 9; ++c) nrec_tot += rp.n[kCompGrid[c]];
+  const int64_t nrec_tot = total_points(rp);
   CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
   T* tm = (T*)h->scratch_tm.p;
   T* tmv[9];
@@ -575,8 +599,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
         for (int k = 0; k < 9; ++k) { a.ssl[k] = ssL[k]; a.ssr[k] = ssR[k]; }
         a.ssfl = T(f[it2 - 1]); a.ssfr = T(f[hh + it2 - 1]);
         CHK((launch_step<T, ela::MODE_FWD, true, false, false>(h, a)));
-        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
-        launch_record<T>(h, R.a, (int)its, f[it2], f[hh + it2]);
+        launch_record<T>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2], f[hh + it2]);
         w.swap();
       }
       for (int k = 0; k < 9; ++k) std::swap(ssL[k], ssR[k]);
@@ -594,10 +617,10 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
         for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
         a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
         CHK((launch_step<T, ela::MODE_BORN_ADJ, false, true, true>(h, a)));
-        PointSet<T> R = make_points<T>(h, &rp, w.p0, tmv, true, nrec_tot);
-        for (int k = 0; k < 9; ++k) { R.a.accl[k] = ssL[k]; R.a.accr[k] = ssR[k]; }
-        R.a.accfl = a.accfl; R.a.accfr = a.accfr;
-        launch_inject<T, true>(h, R.a, (int)its, f[it2 + 1], f[hh + it2 + 1]);
+        ela::PointArgs<T> R = make_points<T>(h, rp, w.p0, tmv, true, nrec_tot);
+        for (int k = 0; k < 9; ++k) { R.accl[k] = ssL[k]; R.accr[k] = ssR[k]; }
+        R.accfl = a.accfl; R.accfr = a.accfr;
+        launch_inject<T, true>(h, R, (int)its, f[it2 + 1], f[hh + it2 + 1]);
         w.swap();
       }
       launch_imaging<T, true>(h, imaging_args<T>(h, V, its + 1, nullptr, dm, ssR));
@@ -618,20 +641,11 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
       off += n;
     }
     HIPCHK(hipGetLastError());
-    if (mem == ELA_MEM_HOST) CHK(stage_out_traces(h, rp_in, s.tr_rec, rt_in));
+    if (mem == ELA_MEM_HOST) CHK(stage_out_traces(h, rp_in, dr, rt_in));
   } else {
-    if (dev_unpadded)
-      for (int k = 0; k < 8; ++k)
-        HIPCHK(hipMemcpyAsync((T*)s.dmod + k * h->n_model, dm + k * np, h->n_model * sizeof(T),
-                              hipMemcpyDeviceToDevice, h->stream));
-    if (mem == ELA_MEM_HOST) {
-      for (int k = 0; k < 8; ++k)
-        HIPCHK(hipMemcpyAsync((char*)dmod_out + k * h->n_model * sizeof(T), dm + k * np, h->n_model * sizeof(T),
-                              hipMemcpyDeviceToHost, h->stream));
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
+    CHK(copy_out_3d(h, dmod_out, dm, 8, mem));
   }
-  if (dev_unpadded) HIPCHK(hipStreamSynchronize(h->stream));  // before dmpad is freed
+  if (mem == ELA_MEM_HOST) HIPCHK(hipStreamSynchronize(h->stream));
   return ELA_OK;
 }
 
@@ -675,7 +689,7 @@ extern "C" {
 
 const char* ela_last_error(void) { return g_err.c_str(); }
 
-const char* ela_version(void) { return "elastic_iso_3d gfx950 0.1"; }
+const char* ela_version(void) { return "elastic_iso_3d gfx950 0.2"; }
 
 int ela_create(const ela_config* cfg, ela_handle** out) {
   if (!out) return fail(ELA_ERR_ARG, "null output handle");
@@ -683,22 +697,34 @@ int ela_create(const ela_config* cfg, ela_handle** out) {
   CHK(check_cfg(cfg));
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));
-  if (cfg->device < 0 || cfg->device >= ndev) return fail(ELA_ERR_ARG, "device %d not present (%d devices)", cfg->device, ndev);
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(ELA_ERR_ARG, "device %d not present (%d devices)", cfg->device, ndev);
   DeviceGuard dg(cfg->device);
   ela_handle* h = new ela_handle();
   h->cfg = *cfg;
   h->esize = cfg->precision;
   h->n_model = cfg->nz * cfg->nx * cfg->ny;
-  h->n_pad = round_up(h->n_model, 64);
-  auto bail = [&](int rc) { delete h; return rc; };
-  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(ELA_ERR_HIP, "stream create failed"));
+  const int64_t line = 128 / h->esize;  // elements per 128-byte cache line
+  h->zoff = line - ela::FAT;
+  h->nzi = round_up(h->zoff + cfg->nz, line);
+  h->n_int = h->nzi * cfg->nx * cfg->ny;
+  h->stride = round_up(h->n_int, 64);
+  auto bail = [&](int rc) {
+    delete h;
+    return rc;
+  };
+  if ((uint64_t)h->stride * (uint64_t)h->esize >= (1ull << 32))
+    return bail(fail(ELA_ERR_ARG, "grid of %lld cells exceeds 4 GiB per array (32-bit offsets)", (long long)h->n_model));
+  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(ELA_ERR_HIP, "stream create failed"));
   h->stream = h->own_stream;
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return bail(fail(ELA_ERR_HIP, "event create failed"));
-  int rc = h->wave.alloc(18 * h->n_pad * h->esize);
+  int rc = h->wave.alloc(18 * h->stride * h->esize);
   if (rc) return bail(rc);
-  rc = h->med.alloc(8 * h->n_pad * h->esize);
+  rc = h->med.alloc(8 * h->stride * h->esize);
   if (rc) return bail(rc);
+  if (hipMemset(h->med.p, 0, 8 * h->stride * h->esize) != hipSuccess) return bail(fail(ELA_ERR_HIP, "medium clear"));
   // time filter (nonlinearPropElasticGpuFunctions_3D.cu:224-241) and taper (:248-255)
   const int64_t sub = cfg->sub, hh = sub + 1, ntw = (cfg->nts - 1) * sub + 1;
   h->filt.resize(2 * hh);
@@ -715,14 +741,16 @@ int ela_create(const ela_config* cfg, ela_handle** out) {
       tap[d] = cfg->alpha_cos + (1.0 - cfg->alpha_cos) * std::cos(M_PI / double(mp) * double(mp - d));
     rc = h->taper.alloc(mp * h->esize);
     if (rc) return bail(rc);
+    hipError_t e;
     if (h->esize == 4) {
       std::vector<float> tf(tap.begin(), tap.end());
-      if (hipMemcpy(h->taper.p, tf.data(), mp * 4, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy"));
+      e = hipMemcpy(h->taper.p, tf.data(), mp * 4, hipMemcpyHostToDevice);
     } else {
-      if (hipMemcpy(h->taper.p, tap.data(), mp * 8, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy"));
+      e = hipMemcpy(h->taper.p, tap.data(), mp * 8, hipMemcpyHostToDevice);
     }
+    if (e != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy failed"));
   }
-  h->by_x = cfg->precision == ELA_FP64 ? 8 : 8;
+  h->by_x = cfg->precision == ELA_FP64 ? 8 : 16;
   *out = h;
   return ELA_OK;
 }
@@ -754,10 +782,7 @@ int ela_synchronize(ela_handle* h) {
 int ela_set_medium(ela_handle* h, const void* med8, int mem) {
   if (!h || !med8) return fail(ELA_ERR_ARG, "null argument");
   DeviceGuard dg(h->cfg.device);
-  const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  for (int k = 0; k < 8; ++k)
-    HIPCHK(hipMemcpyAsync((char*)h->med.p + k * h->n_pad * h->esize, (const char*)med8 + k * h->n_model * h->esize,
-                          h->n_model * h->esize, kind, h->stream));
+  CHK(copy_in_3d(h, h->med.p, med8, 8, mem));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->have_medium = true;
   return ELA_OK;
@@ -791,6 +816,28 @@ int ela_born_adjoint(ela_handle* h, const ela_points* sp, const ela_traces* st, 
   DeviceGuard dg(h->cfg.device);
   return h->esize == 4 ? born_run<float>(h, sp, st, nullptr, dmod8, rp, rt, true, mem)
                        : born_run<double>(h, sp, st, nullptr, dmod8, rp, rt, true, mem);
+}
+
+int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  if (tile_x != 4 && tile_x != 8 && tile_x != 16) return fail(ELA_ERR_ARG, "tile_x must be 4, 8 or 16");
+  if (y_chunk < 1) return fail(ELA_ERR_ARG, "y_chunk must be >= 1");
+  h->by_x = tile_x;
+  h->ychunk = y_chunk;
+  return ELA_OK;
+}
+
+int ela_set_tile_order(ela_handle* h, int xcd_contiguous) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  h->remap = xcd_contiguous ? 1 : 0;
+  return ELA_OK;
+}
+
+int ela_get_tiling(const ela_handle* h, int* tile_x, int* y_chunk) {
+  if (!h || !tile_x || !y_chunk) return fail(ELA_ERR_ARG, "null argument");
+  *tile_x = h->by_x;
+  *y_chunk = h->ychunk;
+  return ELA_OK;
 }
 
 int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* ms_stress) {

@@ -57,10 +57,12 @@ struct StepArgs {
   T cx[4], cy[4], cz[4];
   const T* taper;       // [min_pad]
   int min_pad;
-  int nz, nx, ny;
-  int64_t ys;           // nz * nx
+  int nz, nx, ny;       // grid sizes (reference layout)
+  int nzi, zoff;        // internal row pitch and z origin (line-aligned interior)
+  int64_t ys;           // nx * nzi
   int ychunk;
   int nbz, nbx, nby;    // tile counts along z, x, y-chunks
+  int remap;            // deal tiles to XCDs in contiguous ranges
 };
 
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
@@ -114,6 +116,45 @@ __device__ __forceinline__ T dmx(const Plane<T, BX>& p, int r, int q, const T* c
          c[2] * (p[r + 2][q] - p[r - 3][q]) + c[3] * (p[r + 3][q] - p[r - 4][q]);
 }
 
+// 32-bit element offsets from a uniform base pointer: the compiler can use
+// the scalar-base + 32-bit vector-offset addressing mode, which saves the
+// 64-bit per-lane address arithmetic. The host guarantees that every array
+// is smaller than 4 GiB (ela_create checks n * sizeof(T) < 2^32).
+template <typename T>
+__device__ __forceinline__ const T* at(const T* base, uint32_t i) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint64_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ T* at(T* base, uint32_t i) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint64_t)(i * (uint32_t)sizeof(T)));
+}
+// Streaming accesses (read once / written once per step). The non-temporal
+// cache policy is selectable at build time; measured on MI355X it slows the
+// stencil down when applied to the loads, so loads default to the normal
+// policy.
+#ifndef ELA_NT_LOADS
+#define ELA_NT_LOADS 0
+#endif
+#ifndef ELA_NT_STORES
+#define ELA_NT_STORES 1
+#endif
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* base, uint32_t i) {
+#if ELA_NT_LOADS
+  return __builtin_nontemporal_load(at(base, i));
+#else
+  return *at(base, i);
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(T* base, uint32_t i, T v) {
+#if ELA_NT_STORES
+  __builtin_nontemporal_store(v, at(base, i));
+#else
+  *at(base, i) = v;
+#endif
+}
+
 // Values that the derivative operators see. In MODE_BORN_ADJ they are the
 // medium-multiplied fields (stepAdjGpuBorn_3D multiplies at every loaded
 // position, kernelsElaGpu_3D.cu:864-1040).
@@ -121,35 +162,35 @@ template <typename T, int MODE>
 struct StressVal {
   const StepArgs<T>& a;
   bool ok;
-  __device__ __forceinline__ T sxx(int64_t i) const {
+  __device__ __forceinline__ T sxx(uint32_t i) const {
     if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
-      return a.c[SXX][i] * a.m[L2M][i] + (a.c[SYY][i] + a.c[SZZ][i]) * a.m[LAM][i];
-    return a.c[SXX][i];
+      return *at(a.c[SXX], i) * *at(a.m[L2M], i) + (*at(a.c[SYY], i) + *at(a.c[SZZ], i)) * *at(a.m[LAM], i);
+    return *at(a.c[SXX], i);
   }
-  __device__ __forceinline__ T syy(int64_t i) const {
+  __device__ __forceinline__ T syy(uint32_t i) const {
     if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
-      return a.c[SYY][i] * a.m[L2M][i] + (a.c[SXX][i] + a.c[SZZ][i]) * a.m[LAM][i];
-    return a.c[SYY][i];
+      return *at(a.c[SYY], i) * *at(a.m[L2M], i) + (*at(a.c[SXX], i) + *at(a.c[SZZ], i)) * *at(a.m[LAM], i);
+    return *at(a.c[SYY], i);
   }
-  __device__ __forceinline__ T szz(int64_t i) const {
+  __device__ __forceinline__ T szz(uint32_t i) const {
     if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
-      return a.c[SZZ][i] * a.m[L2M][i] + (a.c[SXX][i] + a.c[SYY][i]) * a.m[LAM][i];
-    return a.c[SZZ][i];
+      return *at(a.c[SZZ], i) * *at(a.m[L2M], i) + (*at(a.c[SXX], i) + *at(a.c[SYY], i)) * *at(a.m[LAM], i);
+    return *at(a.c[SZZ], i);
   }
-  __device__ __forceinline__ T sxz(int64_t i) const {
+  __device__ __forceinline__ T sxz(uint32_t i) const {
     if (!ok) return T(0);
-    return MODE == MODE_BORN_ADJ ? a.c[SXZ][i] * a.m[MXZ][i] : a.c[SXZ][i];
+    return MODE == MODE_BORN_ADJ ? *at(a.c[SXZ], i) * *at(a.m[MXZ], i) : *at(a.c[SXZ], i);
   }
-  __device__ __forceinline__ T sxy(int64_t i) const {
+  __device__ __forceinline__ T sxy(uint32_t i) const {
     if (!ok) return T(0);
-    return MODE == MODE_BORN_ADJ ? a.c[SXY][i] * a.m[MXY][i] : a.c[SXY][i];
+    return MODE == MODE_BORN_ADJ ? *at(a.c[SXY], i) * *at(a.m[MXY], i) : *at(a.c[SXY], i);
   }
-  __device__ __forceinline__ T syz(int64_t i) const {
+  __device__ __forceinline__ T syz(uint32_t i) const {
     if (!ok) return T(0);
-    return MODE == MODE_BORN_ADJ ? a.c[SYZ][i] * a.m[MYZ][i] : a.c[SYZ][i];
+    return MODE == MODE_BORN_ADJ ? *at(a.c[SYZ], i) * *at(a.m[MYZ], i) : *at(a.c[SYZ], i);
   }
 };
 
@@ -157,228 +198,290 @@ template <typename T, int MODE>
 struct VelVal {
   const StepArgs<T>& a;
   bool ok;
-  __device__ __forceinline__ T v(int f, int64_t i) const {
+  __device__ __forceinline__ T v(int f, uint32_t i) const {
     if (!ok) return T(0);
-    return MODE == MODE_BORN_ADJ ? a.c[f][i] * a.m[f][i] : a.c[f][i];  // BX_/BY_/BZ_ == VX/VY/VZ
+    return MODE == MODE_BORN_ADJ ? *at(a.c[f], i) * *at(a.m[f], i) : *at(a.c[f], i);  // BX_/BY_/BZ_ == VX/VY/VZ
   }
 };
 
+// Per-thread tile geometry, halo roles and 32-bit column offsets.
+//
+// Internal layout: rows of nzi elements (a multiple of one 128-byte cache
+// line) with the reference's z = 0 stored at zoff, chosen so that the first
+// interior cell (z = FAT) starts a cache line; a 64-wide tile then covers
+// exactly two (FP32) lines and its z halos are the adjacent lines.
 struct TileCoord {
-  int tz, ty, iz, ix, y0, y1, z0, x0;
+  int tz, ty, iz, izo, ix, y0, y1, z0, x0;   // iz internal, izo reference z
+  bool okc, active;        // own column inside the grid / inside the interior
+  bool xh, okxl, okxh;     // x-halo loader (ty < 4) and its low/high validity
+  bool zh, okzh;           // z-halo loader (lanes 0-7) and validity
+  int colh;                // LDS column of the z-halo value
+  uint32_t col, cxl, cxh, czh;
 };
 
 template <int BX>
-__device__ __forceinline__ TileCoord tile_coord(int nbz, int nbx, int nby, int ychunk, int ny) {
+__device__ __forceinline__ TileCoord tile_coord(int nbz, int nbx, int nby, int ychunk, int nz, int nzi, int zoff,
+                                                int nx, int ny, bool remap) {
   const int nb = nbz * nbx * nby;
-  const int lb = xcd_remap(blockIdx.x, nb);
+  const int lb = remap ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
   const int bz = lb % nbz;
   const int bx = (lb / nbz) % nbx;
   const int by = lb / (nbz * nbx);
   TileCoord t;
   t.tz = threadIdx.x;
   t.ty = threadIdx.y;
-  t.z0 = FAT + bz * TZ;
+  t.z0 = zoff + FAT + bz * TZ;
   t.x0 = FAT + bx * BX;
   t.iz = t.z0 + t.tz;
+  t.izo = t.iz - zoff;
   t.ix = t.x0 + t.ty;
   t.y0 = FAT + by * ychunk;
   t.y1 = min(t.y0 + ychunk, ny - FAT);
+  t.okc = t.izo < nz && t.ix < nx;
+  t.active = (t.izo < nz - FAT) && (t.ix < nx - FAT);
+  t.xh = t.ty < FAT;
+  const int ixlo = t.x0 - FAT + t.ty, ixhi = t.x0 + BX + t.ty;
+  t.okxl = t.xh && t.izo < nz;
+  t.okxh = t.okxl && ixhi < nx;
+  t.zh = t.tz < 2 * FAT;
+  const int izh = t.tz < FAT ? t.z0 - FAT + t.tz : t.z0 + TZ + (t.tz - FAT);
+  t.colh = t.tz < FAT ? t.tz : TZ + t.tz;
+  t.okzh = t.zh && (izh - zoff) < nz && t.ix < nx;
+  t.col = (uint32_t)(t.ix * nzi + t.iz);
+  t.cxl = (uint32_t)(ixlo * nzi + t.iz);
+  t.cxh = (uint32_t)(ixhi * nzi + t.iz);
+  t.czh = (uint32_t)(t.ix * nzi + izh);
   return t;
 }
 
 // ---------------------------------------------------------------------------
 // Velocity update: vx, vy, vz from the six stresses.
+//
+// Software pipeline per y plane: the global loads of plane iy+1 (new queue
+// values, LDS centres and halos) are issued before plane iy is computed and
+// land in the second LDS buffer, so one barrier per plane suffices and the
+// HBM latency overlaps the stencil arithmetic.
 // ---------------------------------------------------------------------------
+template <typename T>
+struct VelStage {
+  T xx, zz, xz;                 // LDS centres of the non-queue fields
+  T hxx0, hxx1, hxy0, hxy1, hxz0, hxz1;  // x-halo rows (low / high)
+  T hzz, hxz, hyz;              // z-halo column
+  T nxy, nyy, nyz;              // new queue values (plane + 4, + 3, + 4)
+};
+
 template <typename T, int BX, int MODE, bool SS, bool ACC>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
-  __shared__ Plane<T, BX> l_xx, l_zz, l_xy, l_xz, l_yz;
-  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.ny);
-  const int nz = a.nz, nx = a.nx;
-  const int64_t ys = a.ys;
-  const bool okz = t.iz < nz, okx = t.ix < nx;
-  const bool active = (t.iz < nz - FAT) && (t.ix < nx - FAT);
-  const StressVal<T, MODE> S{a, okz && okx};
-  // halo loaders: x-halo rows (ty < 4) and z-halo columns (tz < 8)
-  const bool xh = t.ty < FAT;
-  const int ixlo = t.x0 - FAT + t.ty, ixhi = t.x0 + BX + t.ty;
-  const bool okxhi = ixhi < nx;
-  const bool zh = t.tz < 2 * FAT;
-  const int izh = t.tz < FAT ? t.z0 - FAT + t.tz : t.z0 + TZ + (t.tz - FAT);
-  const int colh = t.tz < FAT ? t.tz : TZ + t.tz;
-  const bool okzh = izh < nz;
-  const StressVal<T, MODE> SXlo{a, okz && xh}, SXhi{a, okz && xh && okxhi};
-  const StressVal<T, MODE> SZh{a, zh && okzh && okx};
+  __shared__ Plane<T, BX> l_xx[2], l_zz[2], l_xy[2], l_xz[2], l_yz[2];
+  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
+  const uint32_t ys = (uint32_t)a.ys;
+  const StressVal<T, MODE> S{a, t.okc}, SXl{a, t.okxl}, SXh{a, t.okxh}, SZ{a, t.okzh};
 
-  const int64_t col = (int64_t)t.ix * nz + t.iz;
-  const int64_t colxlo = (int64_t)ixlo * nz + t.iz, colxhi = (int64_t)ixhi * nz + t.iz;
-  const int64_t colzh = (int64_t)t.ix * nz + izh;
+  auto fetch = [&](VelStage<T>& v, int p) {
+    const uint32_t pl = (uint32_t)p * ys;
+    v.xx = S.sxx(pl + t.col);
+    v.zz = S.szz(pl + t.col);
+    v.xz = S.sxz(pl + t.col);
+    v.nxy = S.sxy(pl + 4 * ys + t.col);
+    v.nyy = S.syy(pl + 3 * ys + t.col);
+    v.nyz = S.syz(pl + 4 * ys + t.col);
+    if (t.xh) {
+      v.hxx0 = SXl.sxx(pl + t.cxl); v.hxx1 = SXh.sxx(pl + t.cxh);
+      v.hxy0 = SXl.sxy(pl + t.cxl); v.hxy1 = SXh.sxy(pl + t.cxh);
+      v.hxz0 = SXl.sxz(pl + t.cxl); v.hxz1 = SXh.sxz(pl + t.cxh);
+    }
+    if (t.zh) {
+      v.hzz = SZ.szz(pl + t.czh); v.hxz = SZ.sxz(pl + t.czh); v.hyz = SZ.syz(pl + t.czh);
+    }
+  };
+  const int r = t.ty + FAT, q = t.tz + FAT;
+  auto stage = [&](int b, const VelStage<T>& v, T cxy, T cyz) {
+    l_xx[b][r][q] = v.xx; l_zz[b][r][q] = v.zz; l_xz[b][r][q] = v.xz; l_xy[b][r][q] = cxy; l_yz[b][r][q] = cyz;
+    if (t.xh) {
+      l_xx[b][t.ty][q] = v.hxx0; l_xx[b][r + BX][q] = v.hxx1;
+      l_xy[b][t.ty][q] = v.hxy0; l_xy[b][r + BX][q] = v.hxy1;
+      l_xz[b][t.ty][q] = v.hxz0; l_xz[b][r + BX][q] = v.hxz1;
+    }
+    if (t.zh) { l_zz[b][r][t.colh] = v.hzz; l_xz[b][r][t.colh] = v.hxz; l_yz[b][r][t.colh] = v.hyz; }
+  };
 
+  // queues: sxy (D+y, q[j] = f[iy-3+j]), syy (D-y, f[iy-4+j]), syz (D+y)
   T qxy[8], qyy[8], qyz[8];
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
-    qxy[j] = S.sxy((int64_t)(t.y0 - 3 + j) * ys + col);
-    qyy[j] = S.syy((int64_t)(t.y0 - 4 + j) * ys + col);
-    qyz[j] = S.syz((int64_t)(t.y0 - 3 + j) * ys + col);
+    qxy[j] = S.sxy((uint32_t)(t.y0 - 3 + j) * ys + t.col);
+    qyy[j] = S.syy((uint32_t)(t.y0 - 4 + j) * ys + t.col);
+    qyz[j] = S.syz((uint32_t)(t.y0 - 3 + j) * ys + t.col);
   }
-  const int r = t.ty + FAT, q = t.tz + FAT;
+  {
+    VelStage<T> v0;
+    fetch(v0, t.y0);
+    qxy[7] = v0.nxy; qyy[7] = v0.nyy; qyz[7] = v0.nyz;
+    stage(0, v0, qxy[3], qyz[3]);
+  }
+  __syncthreads();
+  int b = 0;
   for (int iy = t.y0; iy < t.y1; ++iy) {
-    const int64_t pl = (int64_t)iy * ys;
-    const int64_t i = pl + col;
-    qxy[7] = S.sxy(i + 4 * ys);
-    qyy[7] = S.syy(i + 3 * ys);
-    qyz[7] = S.syz(i + 4 * ys);
-    const T cxx = S.sxx(i), czz = S.szz(i), cxz = S.sxz(i);
-    T hxx_lo = 0, hxx_hi = 0, hxy_lo = 0, hxy_hi = 0, hxz_lo = 0, hxz_hi = 0;
-    if (xh) {
-      hxx_lo = SXlo.sxx(pl + colxlo); hxx_hi = SXhi.sxx(pl + colxhi);
-      hxy_lo = SXlo.sxy(pl + colxlo); hxy_hi = SXhi.sxy(pl + colxhi);
-      hxz_lo = SXlo.sxz(pl + colxlo); hxz_hi = SXhi.sxz(pl + colxhi);
-    }
-    T hzz = 0, hxzz = 0, hyzz = 0;
-    if (zh) {
-      hzz = SZh.szz(pl + colzh); hxzz = SZh.sxz(pl + colzh); hyzz = SZh.syz(pl + colzh);
-    }
-    __syncthreads();
-    l_xx[r][q] = cxx; l_zz[r][q] = czz; l_xz[r][q] = cxz; l_xy[r][q] = qxy[3]; l_yz[r][q] = qyz[3];
-    if (xh) {
-      l_xx[t.ty][q] = hxx_lo; l_xx[r + BX][q] = hxx_hi;
-      l_xy[t.ty][q] = hxy_lo; l_xy[r + BX][q] = hxy_hi;
-      l_xz[t.ty][q] = hxz_lo; l_xz[r + BX][q] = hxz_hi;
-    }
-    if (zh) {
-      l_zz[r][colh] = hzz; l_xz[r][colh] = hxzz; l_yz[r][colh] = hyzz;
-    }
-    __syncthreads();
-    if (active) {
+    const bool more = iy + 1 < t.y1;
+    VelStage<T> nx_;
+    if (more) fetch(nx_, iy + 1);
+    if (t.active) {
+      const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
-      const T sx = dmx<T, BX>(l_xx, r, q, cx) + dq(qxy, cy) + dpz<T, BX>(l_xz, r, q, cz);
-      const T sy = dpx<T, BX>(l_xy, r, q, cx) + dq(qyy, cy) + dpz<T, BX>(l_yz, r, q, cz);
-      const T sz = dpx<T, BX>(l_xz, r, q, cx) + dq(qyz, cy) + dmz<T, BX>(l_zz, r, q, cz);
-      const T w = damp_w(a, t.iz, t.ix, iy);
+      T o3[3], m3[3];
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        o3[f] = ld_nt(a.o[f], i);
+        if (MODE != MODE_BORN_ADJ) m3[f] = ld_nt(a.m[f], i);
+      }
+      const T sx = dmx<T, BX>(l_xx[b], r, q, cx) + dq(qxy, cy) + dpz<T, BX>(l_xz[b], r, q, cz);
+      const T sy = dpx<T, BX>(l_xy[b], r, q, cx) + dq(qyy, cy) + dpz<T, BX>(l_yz[b], r, q, cz);
+      const T sz = dpx<T, BX>(l_xz[b], r, q, cx) + dq(qyz, cy) + dmz<T, BX>(l_zz[b], r, q, cz);
+      const T w = damp_w(a, t.izo, t.ix, iy);
       const T s3[3] = {sx, sy, sz};
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
         T u;
-        if (MODE == MODE_FWD) u = w * a.o[f][i] + a.m[f][i] * s3[f];
-        else if (MODE == MODE_ADJ) u = w * a.o[f][i] - a.m[f][i] * s3[f];
-        else u = w * a.o[f][i] - s3[f];
-        if (SS) u += a.ssl[f][i] * a.ssfl + a.ssr[f][i] * a.ssfr;
+        if (MODE == MODE_FWD) u = w * o3[f] + m3[f] * s3[f];
+        else if (MODE == MODE_ADJ) u = w * o3[f] - m3[f] * s3[f];
+        else u = w * o3[f] - s3[f];
+        if (SS) u += *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr;
         u *= w;
-        a.o[f][i] = u;
+        st_nt(a.o[f], i, u);
         if (ACC) {
-          a.accl[f][i] += u * a.accfl;
-          a.accr[f][i] += u * a.accfr;
+          *at(a.accl[f], i) += u * a.accfl;
+          *at(a.accr[f], i) += u * a.accfr;
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < 7; ++j) { qxy[j] = qxy[j + 1]; qyy[j] = qyy[j + 1]; qyz[j] = qyz[j + 1]; }
+    if (more) {
+      qxy[7] = nx_.nxy; qyy[7] = nx_.nyy; qyz[7] = nx_.nyz;
+      stage(b ^ 1, nx_, qxy[3], qyz[3]);
+    }
+    __syncthreads();
+    b ^= 1;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Stress update: six stresses from vx, vy, vz.
+// Stress update: six stresses from vx, vy, vz (same pipeline).
 // ---------------------------------------------------------------------------
+template <typename T>
+struct StressStage {
+  T hlo[3], hhi[3], hz[3];   // x-halo rows and z-halo column of vx, vy, vz
+  T nx, ny, nz;              // new queue values (vx: plane+3, vy: +4, vz: +3)
+};
+
 template <typename T, int BX, int MODE, bool SS, bool ACC>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
-  __shared__ Plane<T, BX> l_vx, l_vy, l_vz;
-  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.ny);
-  const int nz = a.nz, nx = a.nx;
-  const int64_t ys = a.ys;
-  const bool okz = t.iz < nz, okx = t.ix < nx;
-  const bool active = (t.iz < nz - FAT) && (t.ix < nx - FAT);
-  const VelVal<T, MODE> V{a, okz && okx};
-  const bool xh = t.ty < FAT;
-  const int ixlo = t.x0 - FAT + t.ty, ixhi = t.x0 + BX + t.ty;
-  const bool okxhi = ixhi < nx;
-  const bool zh = t.tz < 2 * FAT;
-  const int izh = t.tz < FAT ? t.z0 - FAT + t.tz : t.z0 + TZ + (t.tz - FAT);
-  const int colh = t.tz < FAT ? t.tz : TZ + t.tz;
-  const bool okzh = izh < nz;
-  const VelVal<T, MODE> VXlo{a, okz && xh}, VXhi{a, okz && xh && okxhi};
-  const VelVal<T, MODE> VZh{a, zh && okzh && okx};
+  __shared__ Plane<T, BX> l_vx[2], l_vy[2], l_vz[2];
+  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
+  const uint32_t ys = (uint32_t)a.ys;
+  const VelVal<T, MODE> V{a, t.okc}, Vlo{a, t.okxl}, Vhi{a, t.okxh}, Vzh{a, t.okzh};
 
-  const int64_t col = (int64_t)t.ix * nz + t.iz;
-  const int64_t colxlo = (int64_t)ixlo * nz + t.iz, colxhi = (int64_t)ixhi * nz + t.iz;
-  const int64_t colzh = (int64_t)t.ix * nz + izh;
+  auto fetch = [&](StressStage<T>& v, int p) {
+    const uint32_t pl = (uint32_t)p * ys;
+    v.nx = V.v(VX, pl + 3 * ys + t.col);
+    v.ny = V.v(VY, pl + 4 * ys + t.col);
+    v.nz = V.v(VZ, pl + 3 * ys + t.col);
+    if (t.xh) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) { v.hlo[f] = Vlo.v(f, pl + t.cxl); v.hhi[f] = Vhi.v(f, pl + t.cxh); }
+    }
+    if (t.zh) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) v.hz[f] = Vzh.v(f, pl + t.czh);
+    }
+  };
+  const int r = t.ty + FAT, q = t.tz + FAT;
+  auto stage = [&](int b, const StressStage<T>& v, T cvx, T cvy, T cvz) {
+    l_vx[b][r][q] = cvx; l_vy[b][r][q] = cvy; l_vz[b][r][q] = cvz;
+    if (t.xh) {
+      l_vx[b][t.ty][q] = v.hlo[0]; l_vx[b][r + BX][q] = v.hhi[0];
+      l_vy[b][t.ty][q] = v.hlo[1]; l_vy[b][r + BX][q] = v.hhi[1];
+      l_vz[b][t.ty][q] = v.hlo[2]; l_vz[b][r + BX][q] = v.hhi[2];
+    }
+    if (t.zh) { l_vx[b][r][t.colh] = v.hz[0]; l_vy[b][r][t.colh] = v.hz[1]; l_vz[b][r][t.colh] = v.hz[2]; }
+  };
 
-  // queues: vx (D-y), vy (D+y), vz (D-y)
+  // queues: vx (D-y, q[j] = f[iy-4+j]), vy (D+y, f[iy-3+j]), vz (D-y)
   T qx[8], qy[8], qz[8];
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
-    qx[j] = V.v(VX, (int64_t)(t.y0 - 4 + j) * ys + col);
-    qy[j] = V.v(VY, (int64_t)(t.y0 - 3 + j) * ys + col);
-    qz[j] = V.v(VZ, (int64_t)(t.y0 - 4 + j) * ys + col);
+    qx[j] = V.v(VX, (uint32_t)(t.y0 - 4 + j) * ys + t.col);
+    qy[j] = V.v(VY, (uint32_t)(t.y0 - 3 + j) * ys + t.col);
+    qz[j] = V.v(VZ, (uint32_t)(t.y0 - 4 + j) * ys + t.col);
   }
-  const int r = t.ty + FAT, q = t.tz + FAT;
+  {
+    StressStage<T> v0;
+    fetch(v0, t.y0);
+    qx[7] = v0.nx; qy[7] = v0.ny; qz[7] = v0.nz;
+    stage(0, v0, qx[4], qy[3], qz[4]);
+  }
+  __syncthreads();
+  int b = 0;
   for (int iy = t.y0; iy < t.y1; ++iy) {
-    const int64_t pl = (int64_t)iy * ys;
-    const int64_t i = pl + col;
-    qx[7] = V.v(VX, i + 3 * ys);
-    qy[7] = V.v(VY, i + 4 * ys);
-    qz[7] = V.v(VZ, i + 3 * ys);
-    T hlo[3] = {0, 0, 0}, hhi[3] = {0, 0, 0}, hz[3] = {0, 0, 0};
-    if (xh) {
-#pragma unroll
-      for (int f = 0; f < 3; ++f) { hlo[f] = VXlo.v(f, pl + colxlo); hhi[f] = VXhi.v(f, pl + colxhi); }
-    }
-    if (zh) {
-#pragma unroll
-      for (int f = 0; f < 3; ++f) hz[f] = VZh.v(f, pl + colzh);
-    }
-    __syncthreads();
-    l_vx[r][q] = qx[4]; l_vy[r][q] = qy[3]; l_vz[r][q] = qz[4];
-    if (xh) {
-      l_vx[t.ty][q] = hlo[0]; l_vx[r + BX][q] = hhi[0];
-      l_vy[t.ty][q] = hlo[1]; l_vy[r + BX][q] = hhi[1];
-      l_vz[t.ty][q] = hlo[2]; l_vz[r + BX][q] = hhi[2];
-    }
-    if (zh) {
-      l_vx[r][colh] = hz[0]; l_vy[r][colh] = hz[1]; l_vz[r][colh] = hz[2];
-    }
-    __syncthreads();
-    if (active) {
+    const bool more = iy + 1 < t.y1;
+    StressStage<T> nx_;
+    if (more) fetch(nx_, iy + 1);
+    if (t.active) {
+      const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
-      const T exx = dpx<T, BX>(l_vx, r, q, cx);
+      T o6[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) o6[k] = ld_nt(a.o[SXX + k], i);
+      T l2m = 0, lam = 0, mxz = 0, mxy = 0, myz = 0;
+      if (MODE != MODE_BORN_ADJ) {
+        l2m = ld_nt(a.m[L2M], i); lam = ld_nt(a.m[LAM], i);
+        mxz = ld_nt(a.m[MXZ], i); mxy = ld_nt(a.m[MXY], i); myz = ld_nt(a.m[MYZ], i);
+      }
+      const T exx = dpx<T, BX>(l_vx[b], r, q, cx);
       const T eyy = dq(qy, cy);
-      const T ezz = dpz<T, BX>(l_vz, r, q, cz);
-      const T gxy = dq(qx, cy) + dmx<T, BX>(l_vy, r, q, cx);
-      const T gxz = dmz<T, BX>(l_vx, r, q, cz) + dmx<T, BX>(l_vz, r, q, cx);
-      const T gyz = dq(qz, cy) + dmz<T, BX>(l_vy, r, q, cz);
-      const T w = damp_w(a, t.iz, t.ix, iy);
+      const T ezz = dpz<T, BX>(l_vz[b], r, q, cz);
+      const T gxy = dq(qx, cy) + dmx<T, BX>(l_vy[b], r, q, cx);
+      const T gxz = dmz<T, BX>(l_vx[b], r, q, cz) + dmx<T, BX>(l_vz[b], r, q, cx);
+      const T gyz = dq(qz, cy) + dmz<T, BX>(l_vy[b], r, q, cz);
+      const T w = damp_w(a, t.izo, t.ix, iy);
       T u[6];
       if (MODE == MODE_BORN_ADJ) {
-        u[0] = w * a.o[SXX][i] - exx;
-        u[1] = w * a.o[SYY][i] - eyy;
-        u[2] = w * a.o[SZZ][i] - ezz;
-        u[3] = w * a.o[SXZ][i] - gxz;
-        u[4] = w * a.o[SXY][i] - gxy;
-        u[5] = w * a.o[SYZ][i] - gyz;
+        u[0] = w * o6[0] - exx;
+        u[1] = w * o6[1] - eyy;
+        u[2] = w * o6[2] - ezz;
+        u[3] = w * o6[3] - gxz;
+        u[4] = w * o6[4] - gxy;
+        u[5] = w * o6[5] - gyz;
       } else {
         const T sg = MODE == MODE_FWD ? T(1) : T(-1);
-        const T l2m = a.m[L2M][i], lam = a.m[LAM][i];
-        u[0] = w * a.o[SXX][i] + sg * (l2m * exx) + sg * (lam * (eyy + ezz));
-        u[1] = w * a.o[SYY][i] + sg * (l2m * eyy) + sg * (lam * (exx + ezz));
-        u[2] = w * a.o[SZZ][i] + sg * (l2m * ezz) + sg * (lam * (exx + eyy));
-        u[3] = w * a.o[SXZ][i] + sg * (a.m[MXZ][i] * gxz);
-        u[4] = w * a.o[SXY][i] + sg * (a.m[MXY][i] * gxy);
-        u[5] = w * a.o[SYZ][i] + sg * (a.m[MYZ][i] * gyz);
+        u[0] = w * o6[0] + sg * (l2m * exx) + sg * (lam * (eyy + ezz));
+        u[1] = w * o6[1] + sg * (l2m * eyy) + sg * (lam * (exx + ezz));
+        u[2] = w * o6[2] + sg * (l2m * ezz) + sg * (lam * (exx + eyy));
+        u[3] = w * o6[3] + sg * (mxz * gxz);
+        u[4] = w * o6[4] + sg * (mxy * gxy);
+        u[5] = w * o6[5] + sg * (myz * gyz);
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         const int f = SXX + k;
         T v = u[k];
-        if (SS) v += a.ssl[f][i] * a.ssfl + a.ssr[f][i] * a.ssfr;
+        if (SS) v += *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr;
         v *= w;
-        a.o[f][i] = v;
+        st_nt(a.o[f], i, v);
         if (ACC) {
-          a.accl[f][i] += v * a.accfl;
-          a.accr[f][i] += v * a.accfr;
+          *at(a.accl[f], i) += v * a.accfl;
+          *at(a.accr[f], i) += v * a.accfr;
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < 7; ++j) { qx[j] = qx[j + 1]; qy[j] = qy[j + 1]; qz[j] = qz[j + 1]; }
+    if (more) {
+      qx[7] = nx_.nx; qy[7] = nx_.ny; qz[7] = nx_.nz;
+      stage(b ^ 1, nx_, qx[4], qy[3], qz[4]);
+    }
+    __syncthreads();
+    b ^= 1;
   }
 }
 
@@ -387,7 +490,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PointArgs {
-  const int64_t* pos[9];   // positions of the component's grid
+  const int32_t* pos[9];   // internal-layout positions of the component's grid (-1: skip)
   int64_t n[9];
   int64_t off[10];         // prefix offsets over components
   T* field[9];             // wavefield component arrays
@@ -402,7 +505,7 @@ struct PointArgs {
   T accfl, accfr;
   // damping (injection only)
   const T* taper;
-  int min_pad, nz, nx, ny;
+  int min_pad, nz, nx, ny, nzi, zoff;
 };
 
 template <typename T>
@@ -414,12 +517,12 @@ __device__ __forceinline__ int comp_of(const PointArgs<T>& a, int64_t g) {
 }
 
 template <typename T>
-__device__ __forceinline__ T point_taper(const PointArgs<T>& a, int64_t p) {
-  const int64_t ys = (int64_t)a.nz * a.nx;
-  const int iy = (int)(p / ys);
-  const int64_t rem = p - (int64_t)iy * ys;
-  const int ix = (int)(rem / a.nz);
-  const int iz = (int)(rem - (int64_t)ix * a.nz);
+__device__ __forceinline__ T point_taper(const PointArgs<T>& a, int32_t p) {
+  const int ys = a.nzi * a.nx;
+  const int iy = p / ys;
+  const int rem = p - iy * ys;
+  const int ix = rem / a.nzi;
+  const int iz = rem - ix * a.nzi - a.zoff;
   int d = min(min(iz - FAT, ix - FAT), min(a.nz - iz - 1 - FAT, a.nx - ix - 1 - FAT));
   d = min(d, min(iy - FAT, a.ny - iy - 1 - FAT));
   return d < a.min_pad ? a.taper[d] : T(1);
@@ -434,7 +537,8 @@ __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   if (g >= a.off[9]) return;
   const int c = comp_of(a, g);
   const int64_t k = g - a.off[c];
-  const int64_t p = a.pos[c][k];
+  const int32_t p = a.pos[c][k];
+  if (p < 0) return;
   const T* tr = a.tr[c] + k * a.tr_stride_pt;
   const T s = tr[(int64_t)a.its * a.tr_stride_t] * a.fa + tr[(int64_t)(a.its + 1) * a.tr_stride_t] * a.fb;
   const T v = point_taper(a, p) * s;
@@ -453,7 +557,9 @@ __global__ void __launch_bounds__(256) k_record(PointArgs<T> a) {
   if (g >= a.off[9]) return;
   const int c = comp_of(a, g);
   const int64_t k = g - a.off[c];
-  const T v = a.field[c][a.pos[c][k]];
+  const int32_t p = a.pos[c][k];
+  if (p < 0) return;
+  const T v = a.field[c][p];
   T* tr = a.tr[c] + k * a.tr_stride_pt;
   tr[(int64_t)a.its * a.tr_stride_t] += v * a.fa;
   tr[(int64_t)(a.its + 1) * a.tr_stride_t] += v * a.fb;
@@ -492,29 +598,25 @@ struct ImagingArgs {
   T cx[4], cy[4], cz[4];
   T dts_inv;
   int its, nts;
-  int nz, nx, ny;
+  int nz, nx, ny, nzi, zoff;
   int64_t ys;
   int ychunk, nbz, nbx, nby;
+  int remap;
 };
 
 template <typename T, int BX, bool ADJ>
 __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
   __shared__ Plane<T, BX> l_vx, l_vy, l_vz;
-  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.ny);
-  const int nz = a.nz, nx = a.nx;
+  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
   const int64_t ys = a.ys;
-  const bool ok = t.iz < nz && t.ix < nx;
-  const bool active = (t.iz < nz - FAT) && (t.ix < nx - FAT);
-  const bool xh = t.ty < FAT;
-  const int ixlo = t.x0 - FAT + t.ty, ixhi = t.x0 + BX + t.ty;
-  const bool okxlo = t.iz < nz && xh, okxhi = okxlo && ixhi < nx;
-  const bool zh = t.tz < 2 * FAT;
-  const int izh = t.tz < FAT ? t.z0 - FAT + t.tz : t.z0 + TZ + (t.tz - FAT);
-  const int colh = t.tz < FAT ? t.tz : TZ + t.tz;
-  const bool okzh = zh && izh < nz && t.ix < nx;
-  const int64_t col = (int64_t)t.ix * nz + t.iz;
-  const int64_t colxlo = (int64_t)ixlo * nz + t.iz, colxhi = (int64_t)ixhi * nz + t.iz;
-  const int64_t colzh = (int64_t)t.ix * nz + izh;
+  const bool ok = t.okc;
+  const bool active = t.active;
+  const bool xh = t.xh;
+  const bool okxlo = t.okxl, okxhi = t.okxh;
+  const bool zh = t.zh;
+  const int colh = t.colh;
+  const bool okzh = t.okzh;
+  const int64_t col = t.col, colxlo = t.cxl, colxhi = t.cxh, colzh = t.czh;
   T qx[8], qy[8], qz[8];
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
@@ -586,6 +688,24 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
 #pragma unroll
     for (int j = 0; j < 7; ++j) { qx[j] = qx[j + 1]; qy[j] = qy[j + 1]; qz[j] = qz[j + 1]; }
   }
+}
+
+// Reference-layout flat indices -> internal-layout offsets. Positions outside
+// the FD interior are replaced by -1 (skipped by the point kernels) and
+// reported through *bad.
+__global__ void __launch_bounds__(256) k_convert_pos(const int64_t* __restrict__ in, int32_t* __restrict__ out,
+                                                     int64_t n, int nz, int nx, int ny, int nzi, int zoff,
+                                                     int* bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = in[i];
+  const int64_t ys = (int64_t)nz * nx;
+  const int64_t iy = p >= 0 ? p / ys : -1;
+  const int64_t r = p - iy * ys;
+  const int64_t ix = r / nz, iz = r - ix * nz;
+  const bool ok = p >= 0 && iy >= FAT && iy < ny - FAT && ix >= FAT && ix < nx - FAT && iz >= FAT && iz < nz - FAT;
+  out[i] = ok ? (int32_t)(iy * nx * nzi + ix * nzi + iz + zoff) : -1;
+  if (!ok) atomicOr(bad, 1);
 }
 
 template <typename T>

@@ -194,6 +194,17 @@ class Propagator:
         self._keep.clear()
         return dmod8
 
+    def set_tiling(self, tile_x: int, y_chunk: int) -> None:
+        check(self.lib.ela_set_tiling(self.h, int(tile_x), int(y_chunk)), "ela_set_tiling")
+
+    def set_tile_order(self, xcd_contiguous: bool) -> None:
+        check(self.lib.ela_set_tile_order(self.h, int(bool(xcd_contiguous))), "ela_set_tile_order")
+
+    def get_tiling(self):
+        tx, yc = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.ela_get_tiling(self.h, ctypes.byref(tx), ctypes.byref(yc)), "ela_get_tiling")
+        return tx.value, yc.value
+
     def time_step_kernels(self, reps: int = 50):
         """Mean launch duration [ms] of the velocity and stress stencil kernels
         (HIP events on the handle's stream; overwrites the wavefields)."""

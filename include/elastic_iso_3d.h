@@ -41,6 +41,7 @@
  *     vx->X vy->Y vz->Z sxx,syy,szz->centre sxz->XZ sxy->XY syz->YZ;
  *   - positions must be unique within a grid (as produced by the reference's
  *     spaceInterpGpu_3D::convertIrregToReg) and inside the FD interior;
+ *   - every array must be smaller than 4 GiB (32-bit element offsets);
  *   - `sub` is the number of fine steps per coarse sample, i.e. the doubled
  *     value the reference propagates with (fdParamElastic_3D.cpp:29-36).
  */
@@ -162,6 +163,16 @@ int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const double* z,
  * HIP events on the handle's stream [ms], and the number of fine steps and
  * interior cells it advanced. */
 int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_t* cells);
+
+/* Tile shape of the stencil kernels: x extent of a 64-wide z tile (4, 8 or
+ * 16 rows) and the number of y planes one block marches through. Any
+ * setting gives the same result; it only changes speed (the autotuner in
+ * the Python package picks it per grid). */
+int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk);
+int ela_get_tiling(const ela_handle* h, int* tile_x, int* y_chunk);
+/* Block-to-tile order: 1 (default) deals each XCD a contiguous range of
+ * tiles, 0 keeps the hardware's round-robin order. Speed only. */
+int ela_set_tile_order(ela_handle* h, int xcd_contiguous);
 
 /* Diagnostic timing: launch the velocity and the stress stencil kernels of a
  * forward fine step `reps` times each on the handle's current wavefields and

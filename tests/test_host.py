@@ -1,0 +1,115 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports
+every symbol of include/elastic_iso_3d.h, and its host-only routines (medium
+staggering/scaling, trilinear interpolation) agree with the oracle. No GPU
+compute call is made here."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from elasticisolib_3d_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from elasticisolib_3d_amd import build
+        build.build()
+    return _lib.load()
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "elastic_iso_3d.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ela_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol(lib):
+    from elasticisolib_3d_amd import _lib
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTS)
+    assert lib.ela_version().decode().startswith("elastic_iso_3d")
+
+
+def test_scale_medium_matches_oracle(lib):
+    from elasticisolib_3d_amd.propagator import scale_medium
+    rng = np.random.default_rng(0)
+    shape = (9, 11, 13)
+    rho = 1800 + 400 * rng.random(shape)
+    lam = 1e9 * (3 + rng.random(shape))
+    mu = 1e9 * (2 + rng.random(shape))
+    a = scale_medium(rho, lam, mu, 4e-4)
+    b = O.lib_scale_medium(rho, lam, mu, 4e-4)
+    assert np.max(np.abs(a - b) / np.abs(b)) < 1e-14
+
+
+def test_interp_linear_matches_oracle(lib):
+    from elasticisolib_3d_amd.acquisition import PaddedAxes, ShotGeometry, SpaceInterp
+    rng = np.random.default_rng(1)
+    d = 12.5
+    ax = PaddedAxes(nz=40, nx=44, ny=38, oz=-(4 + 5) * d, ox=-(4 + 6) * d, oy=-(4 + 5) * d, dz=d, dx=d, dy=d,
+                    fat=4, z_pad_minus=5, z_pad_plus=7, x_pad_minus=6, x_pad_plus=6, y_pad=5)
+    n = 40
+    z = rng.uniform(0, 18 * d, n)     # devices inside the unpadded model
+    x = rng.uniform(0, 22 * d, n)
+    y = rng.uniform(0, 18 * d, n)
+    z[:5] = np.arange(5) * d          # on grid nodes
+    for g in range(7):
+        gax = ax.grid(g)
+        a = SpaceInterp(gax, z, x, y)
+        b = O.linear_interp(z, x, y, gax.nz, gax.nx, gax.ny, gax.oz, gax.ox, gax.oy, d, d, d, 4,
+                            gax.z_pad_minus, gax.z_pad_plus, gax.x_pad_minus, gax.x_pad_plus, gax.y_pad)
+        assert np.array_equal(a.pos_unique, b.pos_unique)
+        assert np.array_equal(a.nbr, b.idx)
+        assert np.allclose(a.weight, b.weight, rtol=0, atol=1e-7)
+    # trace mapping: forward and adjoint are transposes
+    geo = ShotGeometry(ax, z, x, y)
+    reg = [rng.standard_normal((geo.grids[gi].n_reg, 5)) for gi in (1, 2, 3, 0, 0, 0, 4, 5, 6)]
+    irr = [rng.standard_normal((n, 5)) for _ in range(9)]
+    lhs = sum(np.sum(a * b) for a, b in zip(geo.to_irregular(reg), irr))
+    rhs = sum(np.sum(a * b) for a, b in zip(reg, geo.to_regular(irr)))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_interp_rejects_out_of_bounds(lib):
+    from elasticisolib_3d_amd.acquisition import PaddedAxes, SpaceInterp
+    ax = PaddedAxes(nz=30, nx=30, ny=30, oz=-90.0, ox=-90.0, oy=-90.0, dz=10, dx=10, dy=10, fat=4,
+                    z_pad_minus=5, z_pad_plus=5, x_pad_minus=5, x_pad_plus=5, y_pad=5)
+    with pytest.raises(ValueError):
+        SpaceInterp(ax, [-50.0], [10.0], [10.0])
+
+
+def test_padding_rule_matches_reference_sizes():
+    """padElasticFileGpuMain_3D.cpp:47-86: zPadPlus/xPadPlus complete the block multiple."""
+    from elasticisolib_3d_amd.models import pad_sizes
+    assert pad_sizes(250, 30) == (30, 40)
+    assert pad_sizes(400, 30) == (30, 34)
+    assert pad_sizes(100, 20) == (20, 24)
+    for n, p in ((250, 30), (400, 30), (100, 20), (160, 30), (256, 30)):
+        lo, hi = pad_sizes(n, p)
+        assert (lo + n + hi) % 16 == 0 and hi >= p
+
+
+def test_bench_refuses_several_gpus():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    from elasticisolib_3d_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(_lib.ElaError):
+        _lib.load()

@@ -1,0 +1,359 @@
+"""Pins of the CPU oracle against mathematics (no GPU).
+
+The reference ships no golden vectors or known-answer tests (its only check
+is the dot-product test, elastic_iso_float/include/operator.h:50-92), so each
+part of the oracle is checked against what the mathematics fixes:
+
+- FD coefficients: closed forms of the 8th-order staggered first derivative;
+- every stencil coupling (which field, which axis, D+ or D-, which medium
+  factor): exact derivatives of degree-7 polynomials placed at the physical
+  staggered positions of the fields (grid shifts of
+  python_float/Elastic_iso_float_3D.py:97-110) and the elastic equations;
+- transposed stepper: <A x, y> = <x, A^T y>;
+- staggering: exact small arrays and adjointness;
+- time filter, taper: closed forms;
+- trilinear weights: partition of unity, exact on linear fields;
+- whole operators: dot-product tests (nonlinear, Born), linearity, a first-
+  order Taylor check of the Born operator against the nonlinear one, and the
+  analytic far/near-field Green's function of a point force (Stokes solution)
+  for the absolute amplitude;
+- two independent restatements (C++ and numpy) agree.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ela_numpy as N
+from oracle import oracle as O
+from tests.problems import interior_points, random_problem, rel
+
+FAT = 4
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build():
+    O.build()
+
+
+def test_fd_coefficients_closed_form():
+    exact = [1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168]
+    for c, e in zip(O.FD_COEFFS, exact):
+        assert abs(c - e) < 1e-10   # the reference literals differ from the closed forms by <= 4.2e-11
+    # first-derivative consistency: sum (2k+1) c_k = 1; higher odd moments vanish
+    c = np.array(exact)
+    k = np.arange(4)
+    for p in (1, 3, 5, 7):
+        m = np.sum(c * (2 * k + 1) ** p) / 2 ** (p - 1)
+        assert abs(m - (1.0 if p == 1 else 0.0)) < 1e-12
+
+
+# Physical staggered offsets (z, x, y) in cells of each field.
+OFFS = {
+    "vx": (0, -0.5, 0), "vy": (0, 0, -0.5), "vz": (-0.5, 0, 0),
+    "sxx": (0, 0, 0), "syy": (0, 0, 0), "szz": (0, 0, 0),
+    "sxz": (-0.5, -0.5, 0), "sxy": (0, -0.5, -0.5), "syz": (-0.5, 0, -0.5),
+}
+FIELDS = ["vx", "vy", "vz", "sxx", "syy", "szz", "sxz", "sxy", "syz"]
+AX = {"z": 0, "x": 1, "y": 2}
+# Elastic equations: output field -> list of (input field, axis, medium key)
+L2M_, L_, MU_ = 3.0, 5.0, 7.0
+COUPLING = {
+    "vx": [("sxx", "x", "B"), ("sxy", "y", "B"), ("sxz", "z", "B")],
+    "vy": [("sxy", "x", "B"), ("syy", "y", "B"), ("syz", "z", "B")],
+    "vz": [("sxz", "x", "B"), ("syz", "y", "B"), ("szz", "z", "B")],
+    "sxx": [("vx", "x", L2M_), ("vy", "y", L_), ("vz", "z", L_)],
+    "syy": [("vy", "y", L2M_), ("vx", "x", L_), ("vz", "z", L_)],
+    "szz": [("vz", "z", L2M_), ("vx", "x", L_), ("vy", "y", L_)],
+    "sxy": [("vx", "y", MU_), ("vy", "x", MU_)],
+    "sxz": [("vx", "z", MU_), ("vz", "x", MU_)],
+    "syz": [("vz", "y", MU_), ("vy", "z", MU_)],
+}
+
+
+def _poly(u):
+    return 0.002 * u ** 7 - 0.03 * u ** 5 + u ** 3 - 2.0 * u ** 2 + 0.5 * u + 1.0
+
+
+def _dpoly(u):
+    return 0.014 * u ** 6 - 0.15 * u ** 4 + 3 * u ** 2 - 4.0 * u + 0.5
+
+
+@pytest.mark.parametrize("src", FIELDS)
+@pytest.mark.parametrize("axis", ["z", "x", "y"])
+def test_stencil_couplings_exact_on_polynomials(src, axis):
+    """Feed a degree-7 polynomial along one axis into one field; every
+    output field must receive medium x exact derivative at its own staggered
+    position, and nothing else."""
+    g = O.Grid(nz=18, nx=18, ny=18, nts=2, sub=2, dz=2.0, dx=2.0, dy=2.0, dts=0.001, minPad=0)
+    shape = g.shape
+    med = np.zeros((8,) + shape)
+    med[0:3] = 1.0          # B = 1
+    med[3], med[4] = L2M_, L_
+    med[5:8] = MU_
+    p1 = np.zeros((9,) + shape)
+    iy, ix, iz = np.meshgrid(np.arange(g.ny), np.arange(g.nx), np.arange(g.nz), indexing="ij")
+    idx = {"z": iz, "x": ix, "y": iy}[axis]
+    h = {"z": g.dz, "x": g.dx, "y": g.dx}[axis]   # y derivatives use dx (reference)
+    scale = 0.2  # keep the polynomial O(1)
+    u_src = (idx + OFFS[src][AX[axis]]) * scale
+    p1[FIELDS.index(src)] = _poly(u_src)
+    out = O.lib_step(g, med, np.zeros_like(p1), p1, 0)
+    sl = (slice(FAT, -FAT),) * 3
+    for o in FIELDS:
+        expect = np.zeros(shape)
+        for (f, ax, m) in COUPLING[o]:
+            if f == src and ax == axis:
+                u_o = (idx + OFFS[o][AX[axis]]) * scale
+                mm = 1.0 if m == "B" else m
+                expect = expect + mm * _dpoly(u_o) * scale / h
+        got = out[FIELDS.index(o)]
+        err = np.max(np.abs(got[sl] - expect[sl]))
+        ref = max(np.max(np.abs(expect[sl])), 1.0)
+        assert err / ref < 1e-9, (o, src, axis, err, ref)
+        # FAT layer untouched
+        assert np.all(got[:FAT] == 0) and np.all(got[-FAT:] == 0)
+
+
+def test_adjoint_stepper_is_transpose():
+    """<M D x, y> = <x, -D M y> on fields that vanish in the FAT layer."""
+    g = O.Grid(nz=20, nx=17, ny=19, nts=2, sub=2, dz=3.0, dx=2.0, dy=2.0, dts=0.001, minPad=0)
+    rng = np.random.default_rng(3)
+    med = 1 + rng.random((8,) + g.shape)
+    x = rng.standard_normal((9,) + g.shape)
+    y = rng.standard_normal((9,) + g.shape)
+    for a in (x, y):
+        a[:, :FAT] = 0; a[:, -FAT:] = 0; a[:, :, :FAT] = 0; a[:, :, -FAT:] = 0; a[:, :, :, :FAT] = 0; a[:, :, :, -FAT:] = 0
+    zero = np.zeros_like(x)
+    Ax = O.lib_step(g, med, zero, x, 0)
+    ATy = O.lib_step(g, med, zero, y, 2)
+    lhs, rhs = np.sum(Ax * y), np.sum(x * ATy)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-13
+    # the nonlinear adjoint stepper is the forward one with the sign flipped
+    assert np.allclose(O.lib_step(g, med, zero, x, 1), -Ax, rtol=0, atol=1e-12 * np.abs(Ax).max())
+
+
+def test_stagger_exact_and_adjoint():
+    a = np.arange(1.0, 6.0).reshape(1, 1, 5) * np.ones((3, 4, 1))   # varies along z
+    fz = O.lib_stagger(0, False, a)
+    az = O.lib_stagger(0, True, a)
+    assert np.allclose(fz[0, 0], [1.5, 2.5, 3.5, 4.5, 5.0])
+    assert np.allclose(az[0, 0], [0.5, 1.5, 2.5, 3.5, 5.0 + 2.0])
+    rng = np.random.default_rng(4)
+    for axis in (0, 1, 2):
+        x = rng.standard_normal((5, 6, 7))
+        y = rng.standard_normal((5, 6, 7))
+        lhs = np.sum(O.lib_stagger(axis, False, x) * y)
+        rhs = np.sum(x * O.lib_stagger(axis, True, y))
+        assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_scale_medium_homogeneous():
+    shape = (6, 7, 8)
+    rho = np.full(shape, 2000.0)
+    lam = np.full(shape, 4e9)
+    mu = np.full(shape, 3e9)
+    dtw = 5e-4
+    m = O.lib_scale_medium(rho, lam, mu, dtw)
+    c = (slice(1, -1),) * 3
+    assert np.allclose(m[0][c], 2 * dtw / 2000.0)
+    assert np.allclose(m[3], 2 * dtw * (4e9 + 6e9))
+    assert np.allclose(m[4], 2 * dtw * 4e9)
+    assert np.allclose(m[5][c], 2 * dtw * 3e9)
+    # edge rules of the adjoint half-cell average: first x sample halved, last 1.5x
+    assert np.allclose(m[0][:, 0, :], 2 * dtw / 1000.0)
+    assert np.allclose(m[0][:, -1, :], 2 * dtw / 3000.0)
+
+
+def test_time_filter_and_taper_closed_form():
+    g = O.Grid(nz=20, nx=20, ny=20, nts=11, sub=8, dz=1, dx=1, dy=1, dts=0.004, minPad=6, alphaCos=0.97)
+    f, w = O.lib_tables(g)
+    h = g.sub + 1
+    ntw = (g.nts - 1) * g.sub + 1
+    s = 1 / math.sqrt(ntw / g.nts)
+    assert np.allclose(f[:h] + f[h:], s)
+    assert np.allclose(f[:h], (1 - np.arange(h) / g.sub) * s)
+    assert abs(w[0] - (2 * 0.97 - 1)) < 1e-15
+    assert np.all(np.diff(w) > 0) and w[-1] < 1
+    assert np.allclose(N.time_filter(g.sub, g.nts), f)
+
+
+def test_trilinear_weights():
+    rng = np.random.default_rng(5)
+    nz, nx, ny, d = 30, 32, 28, 10.0
+    o = -(FAT + 3) * d
+    z = rng.uniform(0, 100, 50); x = rng.uniform(0, 120, 50); y = rng.uniform(0, 90, 50)
+    it = O.linear_interp(z, x, y, nz, nx, ny, o, o, o, d, d, d, FAT, 3, 3, 3, 3, 3)
+    assert np.allclose(it.weight.sum(1), 1.0, atol=1e-6)
+    # exact on a linear field
+    iy, ix, iz = np.unravel_index(it.pos_unique, (ny, nx, nz))
+    field = 2.0 + 0.3 * (o + iz * d) - 0.7 * (o + ix * d) + 0.2 * (o + iy * d)
+    vals = it.forward(field[:, None])[:, 0]
+    assert np.allclose(vals, 2.0 + 0.3 * z - 0.7 * x + 0.2 * y, rtol=1e-5)
+    # a device on a grid node puts weight 1 on that node
+    it2 = O.linear_interp([20.0], [30.0], [40.0], nz, nx, ny, o, o, o, d, d, d, FAT, 3, 3, 3, 3, 3)
+    assert it2.weight.max() == pytest.approx(1.0) and np.sum(it2.weight > 0) == 1
+
+
+def test_numpy_and_cpp_restatements_agree():
+    g = O.Grid(nz=22, nx=24, ny=20, nts=5, sub=4, dz=10., dx=10., dy=10., dts=0.004, minPad=4, alphaCos=0.9)
+    P = random_problem(g, seed=7)
+    a = O.lib_nl_forward(g, P["med"], P["src"], P["spos"], P["rpos"])
+    b = N.nl_forward(g, P["med"], P["src"], P["spos"], P["rpos"])
+    assert rel(a, b) < 1e-13
+
+
+class _Pos:
+    def __init__(self, p):
+        self.pos_unique = p
+
+
+def _dot_problem(seed, minPad=5, alpha=0.95):
+    g = O.Grid(nz=26, nx=25, ny=24, nts=7, sub=4, dz=10., dx=10., dy=10., dts=0.004, minPad=minPad, alphaCos=alpha)
+    return g, random_problem(g, seed=seed)
+
+
+@pytest.mark.parametrize("minPad,alpha", [(0, 1.0), (5, 0.95)])
+def test_nonlinear_dot_product(minPad, alpha):
+    """The reference's operator pair: medium-scaled injection, forward
+    propagation, recording; adjoint: medium-scaled receiver traces, adjoint
+    propagation, extraction (nonlinearPropElasticGpu_3D.cpp:200-277, 481-558)."""
+    g, P = _dot_problem(8, minPad, alpha)
+    rng = np.random.default_rng(9)
+    S = [_Pos(p) for p in P["spos"]]
+    R = [_Pos(p) for p in P["rpos"]]
+    m = [rng.standard_normal(s.shape) for s in P["src"]]
+    d = [rng.standard_normal(r.shape) for r in P["rec"]]
+    dv = g.dx * g.dy * g.dz
+    Fm = O.lib_nl_forward(g, P["med"], O.scale_sources(P["med"], S, m, dv), P["spos"], P["rpos"])
+    Ftd = O.lib_nl_adjoint(g, P["med"], O.scale_sources(P["med"], R, d, dv), P["spos"], P["rpos"])
+    lhs = sum(np.sum(a * b) for a, b in zip(Fm, d))
+    rhs = sum(np.sum(a * b) for a, b in zip(m, Ftd))
+    assert abs(lhs - rhs) / abs(rhs) < 1e-12
+
+
+def test_born_dot_product():
+    g, P = _dot_problem(10)
+    rng = np.random.default_rng(11)
+    dm = P["dm"]
+    d = [rng.standard_normal(r.shape) for r in P["rec"]]
+    Bm = O.lib_born_forward(g, P["med"], P["src"], P["spos"], dm, P["rpos"])
+    Btd = O.lib_born_adjoint(g, P["med"], P["src"], P["spos"], d, P["rpos"])
+    lhs = sum(np.sum(a * b) for a, b in zip(Bm, d))
+    rhs = np.sum(dm * Btd)
+    assert abs(lhs - rhs) / abs(rhs) < 1e-12
+
+
+def test_born_host_staggering_adjoint():
+    rng = np.random.default_rng(12)
+    shape = (7, 8, 9)
+    med = 1 + rng.random((8,) + shape)
+    x = rng.standard_normal((3,) + shape)
+    y = rng.standard_normal((8,) + shape)
+    Sx = O.born_stagger_perturbation(x, med, 1e-3, O.lib_stagger)
+    STy = O.born_unstagger_image(y, med, 1e-3, O.lib_stagger)
+    assert abs(np.sum(Sx * y) - np.sum(x * STy)) < 1e-12 * abs(np.sum(Sx * y))
+
+
+def test_linearity():
+    g, P = _dot_problem(13)
+    a = O.lib_nl_forward(g, P["med"], P["src"], P["spos"], P["rpos"])
+    b = O.lib_nl_forward(g, P["med"], [2.5 * s for s in P["src"]], P["spos"], P["rpos"])
+    assert rel(b, [2.5 * x for x in a]) < 1e-13
+
+
+@pytest.mark.parametrize("param", [0, 1])
+def test_born_first_order_taylor(param):
+    """(F(m + eps dm) - F(m - eps dm)) / (2 eps) ~ B dm for density and
+    lambda perturbations away from the source (the reference's Born operator
+    linearises the scheme up to the coarse-time derivative of the source
+    wavefield, so agreement is to a few percent, not to rounding)."""
+    g = O.Grid(nz=40, nx=40, ny=40, nts=60, sub=8, dz=10., dx=10., dy=10., dts=0.004, minPad=6, alphaCos=0.99)
+    shape = g.shape
+    rho = np.full(shape, 2000.0)
+    vp, vs = 2500.0, 1440.0
+    mu = rho * vs ** 2
+    lam = rho * vp ** 2 - 2 * mu
+    # explosive source at the centre, receivers on a line
+    nz, nx = g.nz, g.nx
+    c = 20
+    flat = lambda iz, ix, iy: iy * nz * nx + ix * nz + iz  # noqa: E731
+    spos = [np.array([flat(c, c, c)])] + [np.zeros(0, np.int64)] * 6
+    rpos = [np.array([flat(12, x, 20) for x in range(10, 31, 2)])] * 7
+    w = np.zeros(g.nts)
+    t = np.arange(g.nts) * g.dts - 0.09
+    a2 = (math.pi * 12.0) ** 2
+    w = (1 - 2 * a2 * t * t) * np.exp(-a2 * t * t)
+    src = [np.zeros((len(spos[O.COMP_GRID[k]]), g.nts)) for k in range(9)]
+    # a smooth perturbation blob between source and receivers, zero at the source
+    iy, ix, iz = np.meshgrid(np.arange(g.ny), np.arange(g.nx), np.arange(g.nz), indexing="ij")
+    blob = np.exp(-((iz - 14) ** 2 + (ix - 20) ** 2 + (iy - 20) ** 2) / 8.0)
+    d3 = np.zeros((3,) + shape)
+    d3[param] = blob * (rho if param == 0 else lam) * 1.0
+
+    def fwd(scale):
+        r = rho + scale * d3[0]
+        lm = lam + scale * d3[1]
+        med = O.lib_scale_medium(r, lm, mu, g.dtw)
+        m8 = med.reshape(8, -1)
+        s = [np.zeros_like(x) for x in src]
+        pc = spos[0][0]
+        for k in (3, 4, 5):   # explosive: mxx = myy = mzz
+            s[k][0] = w * (m8[3][pc] + 2 * m8[4][pc]) / (g.dx * g.dy * g.dz)
+        return O.lib_nl_forward(g, med, s, spos, rpos), med, s
+
+    eps = 1e-4
+    dp, _, _ = fwd(eps)
+    dn, _, _ = fwd(-eps)
+    fd = [(a - b) / (2 * eps) for a, b in zip(dp, dn)]
+    _, med0, s0 = fwd(0.0)
+    dm8 = O.born_stagger_perturbation(d3, med0, g.dtw, O.lib_stagger)
+    born = O.lib_born_forward(g, med0, s0, spos, dm8, rpos)
+    err = rel(born, fd)
+    print(f"Born vs central difference, parameter {param}: relative L2 {err:.4f}")
+    assert err < 0.1, err
+
+
+def _stokes_vz_on_axis(t, r, rho, alpha, beta, force):
+    """Particle velocity vz on the axis of a vertical point force F(t) in a
+    homogeneous solid (Aki & Richards eq. 4.23 with gamma_z = 1):
+    u_z = 1/(4 pi rho) [2/r^3 int_{r/a}^{r/b} tau F(t - tau) dtau + F(t - r/a)/(a^2 r)],
+    v_z = du_z/dt."""
+    dt = t[1] - t[0]
+    taus = np.arange(r / alpha, r / beta, dt / 20)
+    Fi = lambda tt: np.interp(tt, t, force, left=0.0, right=0.0)  # noqa: E731
+    near = np.array([np.trapezoid(taus * Fi(tt - taus), taus) for tt in t])
+    u = (2.0 / r ** 3 * near + Fi(t - r / alpha) / (alpha ** 2 * r)) / (4 * math.pi * rho)
+    return np.gradient(u, dt)
+
+
+def test_point_force_matches_stokes_solution():
+    """Absolute amplitude, sign and timing of the whole chain (source scaling
+    by 2dtw/rho/(dx dy dz), time filter normalisation, stencil) against the
+    analytic Green's function, within the dispersion/sampling error."""
+    n, d = 72, 10.0
+    g = O.Grid(nz=n, nx=n, ny=n, nts=51, sub=8, dz=d, dx=d, dy=d, dts=0.004, minPad=0, alphaCos=1.0)
+    rho0, vp, vs = 2000.0, 2500.0, 1440.0
+    shape = g.shape
+    rho = np.full(shape, rho0)
+    mu = rho * vs ** 2
+    lam = rho * vp ** 2 - 2 * mu
+    med = O.lib_scale_medium(rho, lam, mu, g.dtw)
+    f0 = 15.0
+    t = np.arange(g.nts) * g.dts
+    a2 = (math.pi * f0) ** 2
+    F = (1 - 2 * a2 * (t - 1 / f0) ** 2) * np.exp(-a2 * (t - 1 / f0) ** 2) * 1e9   # N
+    c = n // 2
+    flat = lambda iz, ix, iy: iy * n * n + ix * n + iz  # noqa: E731
+    nrz = 15   # receiver 15 cells below, same staggered (Z) grid
+    spos = [np.zeros(0, np.int64)] * 3 + [np.array([flat(c, c, c)])] + [np.zeros(0, np.int64)] * 3
+    rpos = [np.zeros(0, np.int64)] * 3 + [np.array([flat(c + nrz, c, c)])] + [np.zeros(0, np.int64)] * 3
+    src = [np.zeros((len(spos[O.COMP_GRID[k]]), g.nts)) for k in range(9)]
+    src[2][0] = F * med[2].ravel()[spos[3][0]] / (d ** 3)
+    rec = O.lib_nl_forward(g, med, src, spos, rpos)
+    vz = rec[2][0]
+    ana = _stokes_vz_on_axis(t, nrz * d, rho0, vp, vs, F)
+    win = t < 0.2   # before any boundary reflection returns
+    err = np.linalg.norm(vz[win] - ana[win]) / np.linalg.norm(ana[win])
+    print(f"Stokes on-axis vz relative L2 error {err:.4f}")
+    assert err < 0.05, err
