@@ -50,7 +50,7 @@ def measured_traffic():
     return None
 
 
-def gpu_setup(workload):
+def gpu_setup(workload, precision="f32"):
     import torch
     from elasticisolib_3d_amd import propagator as P
     from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
@@ -64,14 +64,15 @@ def gpu_setup(workload):
     rec_geom = ShotGeometry(a, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
     src_reg = scale_regular_traces(med, src_geom, src_geom.to_regular(source_traces(w)), a.dx * a.dy * a.dz)
     log(f"[bench] host set-up {time.time() - t0:.1f}s; receiver points per grid {rec_geom.counts()}")
-    prop = P.Propagator(a.nz, a.nx, a.ny, w.nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos, "f32")
+    prop = P.Propagator(a.nz, a.nx, a.ny, w.nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos, precision)
     dev = "cuda"
-    prop.set_medium(torch.tensor(med, dtype=torch.float32, device=dev))
+    dt = torch.float32 if precision == "f32" else torch.float64
+    prop.set_medium(torch.tensor(med, dtype=dt, device=dev))
     del med
     spos = [torch.tensor(p, device=dev) for p in src_geom.positions()]
     rpos = [torch.tensor(p, device=dev) for p in rec_geom.positions()]
-    src = [torch.tensor(s, dtype=torch.float32, device=dev) for s in src_reg]
-    rec = [torch.zeros((n, w.nts), dtype=torch.float32, device=dev) for n in
+    src = [torch.tensor(s, dtype=dt, device=dev) for s in src_reg]
+    rec = [torch.zeros((n, w.nts), dtype=dt, device=dev) for n in
            [rec_geom.counts()[g] for g in (1, 2, 3, 0, 0, 0, 4, 5, 6)]]
     torch.cuda.synchronize()
     return prop, spos, src, rpos, rec
@@ -118,6 +119,130 @@ def cpu_baseline(target_s: float = 15.0):
     }
 
 
+def bench_born(args):
+    """Born forward and adjoint (FP32) on the 400x400x250 model smoothed with a
+    15-cell Gaussian, 1 % sparse reflectivity, 501 coarse samples, 150x150
+    receivers; dot-product test of the shot-level operator (perturbation
+    staggering + propagation) with FP64 inner products."""
+    import torch
+    from elasticisolib_3d_amd import propagator as P
+    from elasticisolib_3d_amd.models import born_layered_400
+
+    t0 = time.time()
+    w = born_layered_400(nts=args.nts if args.nts != 626 else 501)
+    log(f"[bench] model {w.name} padded {w.axes.nz}x{w.axes.nx}x{w.axes.ny} built in {time.time() - t0:.1f}s")
+    prop, spos, src, rpos, rec = gpu_setup(w)
+    med = P.scale_medium(w.rho, w.lame, w.mu, w.dtw)
+    dm3 = w.extra["dm3"]
+    dm8 = P.born_stagger(dm3, med, w.dtw)
+    dm8_d = torch.tensor(dm8, dtype=torch.float32, device="cuda")
+    del dm8
+    prop.set_born_storage(args.born_segment)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop.born_forward(spos, src, dm8_d, rpos, rec_out=rec)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    ms_f, steps_f, cells = prop.last_timing()
+    log(f"[bench] Born forward {t_fwd:.1f} s ({steps_f} fine steps)")
+
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    data = [torch.randn(r.shape, generator=gen, device="cuda", dtype=torch.float32) for r in rec]
+    img = torch.zeros((8,) + tuple(w.rho.shape), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop.born_adjoint(spos, src, data, rpos, img)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    ms_a, steps_a, _ = prop.last_timing()
+    seg = prop.born_storage_used()
+    log(f"[bench] Born adjoint {t_adj:.1f} s ({steps_a} fine steps, storage plan {seg})")
+
+    lhs = float(sum(float((r.double() * d.double()).sum()) for r, d in zip(rec, data)))
+    img_h = img.cpu().numpy().astype(np.float64)
+    del img
+    m_adj = P.born_unstagger(img_h, med, w.dtw)
+    rhs = float(np.sum(dm3 * m_adj))
+    dot_rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    log(f"[bench] dot test <B m, d> = {lhs:.9e}  <m, B* d> = {rhs:.9e}  rel {dot_rel:.2e}")
+    gf = float(cells) * steps_f / t_fwd / 1e9
+    ga = float(cells) * steps_a / t_adj / 1e9
+    out = {
+        "metric": "Born Gcell-updates/s (forward; adjoint in config)",
+        "value": round(gf, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(t_fwd * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "FP32 Born forward + adjoint, one shot: 400x400x250 layered model smoothed sigma=15 cells "
+                        "(padded 328x472x468), 1% sparse drho/dlambda/dmu N(0, 0.02 background), explosive 10 Hz "
+                        f"Ricker, {w.nts} coarse samples x {w.sub} fine steps, 150x150 receivers",
+            "interior_cells": cells,
+            "forward_fine_steps": steps_f,
+            "forward_s": round(t_fwd, 2),
+            "adjoint_fine_steps": steps_a,
+            "adjoint_s": round(t_adj, 2),
+            "adjoint_gcell_updates_per_s": round(ga, 3),
+            "adjoint_storage": "all coarse samples in HBM" if seg < 0 else f"exact checkpoints every {seg} coarse samples",
+            "dot_product": {"lhs": lhs, "rhs": rhs, "rel_err": dot_rel, "reductions": "fp64"},
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_fp64_check(args):
+    """FP32 vs FP64 GPU nonlinear forward on the 400x400x250 workload: relative
+    L2 difference of all receiver traces (the FP64 run is the reference)."""
+    import torch
+    from elasticisolib_3d_amd import propagator as P
+    from elasticisolib_3d_amd.models import layered_400
+
+    w = layered_400(nts=args.nts)
+    res = {}
+    for prec in ("f64", "f32"):
+        prop, spos, src, rpos, rec = gpu_setup(w, prec)
+        t0 = time.perf_counter()
+        prop.nl_forward(spos, src, rpos, rec_out=rec)
+        torch.cuda.synchronize()
+        res[prec] = ([r.double().cpu() for r in rec], time.perf_counter() - t0)
+        log(f"[bench] {prec} forward {res[prec][1]:.1f} s")
+        del prop, spos, src, rpos, rec
+        torch.cuda.empty_cache()
+    num = sum(float(((a - b) ** 2).sum()) for a, b in zip(res["f32"][0], res["f64"][0]))
+    den = sum(float((b ** 2).sum()) for b in res["f64"][0])
+    err = (num / den) ** 0.5
+    per = []
+    for a, b in zip(res["f32"][0], res["f64"][0]):
+        nb = float((b ** 2).sum()) ** 0.5
+        per.append(float(((a - b) ** 2).sum()) ** 0.5 / nb if nb > 0 else 0.0)
+    cells = float(w.interior_cells) * w.fine_steps
+    out = {
+        "metric": "FP32 vs FP64 receiver-trace relative L2 error",
+        "value": err,
+        "unit": "relative L2",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(res["f32"][1] * 1e3, 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 vs f64", "data": "synthetic",
+        "config": {
+            "workload": "nonlinear forward, 400x400x250 layered model (padded 328x472x468), "
+                        f"{w.nts} coarse samples x {w.sub} fine steps, 200x200 receivers, 9 components",
+            "per_component_rel_err": [round(e, 9) for e in per],
+            "f64_gcell_updates_per_s": round(cells / res["f64"][1] / 1e9, 3),
+            "f32_gcell_updates_per_s": round(cells / res["f32"][1] / 1e9, 3),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +253,11 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for counter calibration)")
+    ap.add_argument("--workload", choices=("nl_forward", "born", "fp64_check"), default="nl_forward",
+                    help="nl_forward (default): the headline line; born: Born forward/adjoint and dot test; "
+                         "fp64_check: FP32 vs FP64 traces on the forward workload")
+    ap.add_argument("--born-segment", type=int, default=0,
+                    help="Born adjoint storage: 0 automatic, -1 all samples, K checkpoint segment")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -143,6 +273,11 @@ def main():
               "shots would be distributed one process per GPU, which this build does not launch.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "born":
+        return bench_born(args)
+    if args.workload == "fp64_check":
+        return bench_fp64_check(args)
 
     import torch
     from elasticisolib_3d_amd.models import layered_400

@@ -24,7 +24,8 @@ EXPORTS = (
     "ela_synchronize", "ela_set_medium", "ela_scale_medium", "ela_nl_forward",
     "ela_nl_adjoint", "ela_born_forward", "ela_born_adjoint", "ela_last_timing",
     "ela_interp_linear", "ela_time_step_kernels", "ela_set_tiling", "ela_get_tiling",
-    "ela_set_tile_order",
+    "ela_set_tile_order", "ela_set_born_storage", "ela_born_storage_used",
+    "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
 )
 
 
@@ -95,6 +96,8 @@ def load() -> ctypes.CDLL:
     lib.ela_synchronize.argtypes = [vp]
     lib.ela_set_medium.argtypes = [vp, vp, i32]
     lib.ela_scale_medium.argtypes = [i64, i64, i64, ctypes.c_double, vp, vp, vp, vp]
+    for name in ("ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient"):
+        getattr(lib, name).argtypes = [i64, i64, i64, ctypes.c_double, vp, vp, vp]
     P, Tr = ctypes.POINTER(ElaPoints), ctypes.POINTER(ElaTraces)
     lib.ela_nl_forward.argtypes = [vp, P, Tr, P, Tr, i32]
     lib.ela_nl_adjoint.argtypes = [vp, P, Tr, P, Tr, i32]
@@ -103,6 +106,8 @@ def load() -> ctypes.CDLL:
     lib.ela_interp_linear.argtypes = [ctypes.POINTER(ElaInterpAxes), i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)]
     lib.ela_set_tiling.argtypes = [vp, i32, i32]
     lib.ela_set_tile_order.argtypes = [vp, i32]
+    lib.ela_set_born_storage.argtypes = [vp, i64, i64]
+    lib.ela_born_storage_used.argtypes = [vp, ctypes.POINTER(i64)]
     lib.ela_get_tiling.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.ela_time_step_kernels.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.ela_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64), ctypes.POINTER(i64)]

@@ -108,6 +108,20 @@ class ShotGeometry:
         self.grids: List[SpaceInterp] = [SpaceInterp(axes.grid(g), z, x, y) for g in range(7)]
         self.n_dev = len(np.atleast_1d(z))
 
+    @classmethod
+    def from_grids(cls, grids: Sequence[SpaceInterp]) -> "ShotGeometry":
+        """From 7 interpolators already built on the centre, x, y, z, xz, xy
+        and yz grids (the per-grid lists of the geometry builders)."""
+        if len(grids) != 7:
+            raise ValueError("need 7 grid interpolators")
+        n = {g.n_dev for g in grids}
+        if len(n) != 1:
+            raise ValueError("the 7 interpolators must hold the same devices")
+        obj = cls.__new__(cls)
+        obj.grids = list(grids)
+        obj.n_dev = n.pop()
+        return obj
+
     def positions(self) -> List[np.ndarray]:
         return [g.pos_unique for g in self.grids]
 
@@ -143,3 +157,32 @@ def scale_regular_traces(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.
     out[8] = reg[8] * m[7][pos[6]][:, None]
     inv = 1.0 / dxyz
     return [o * inv for o in out]
+
+
+def source_scaling_gradient(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.ndarray],
+                            gscaled: Sequence[np.ndarray], dxyz: float) -> np.ndarray:
+    """Gradient, with respect to the 8 scaled medium arrays, of a function of
+    the scaled source traces, through scale_regular_traces: `reg` are the
+    unscaled regular traces, `gscaled` the gradient with respect to the scaled
+    ones. Returns an [8, ny, nx, nz] array that is zero away from the source
+    points (np.add.at accumulates points shared by components)."""
+    shape = med8.shape
+    g = np.zeros((8, int(np.prod(shape[1:]))), np.float64)
+    pos = geom.positions()
+    inv = 1.0 / dxyz
+    r = [np.asarray(x, np.float64) for x in reg]
+    s = [np.asarray(x, np.float64) for x in gscaled]
+
+    def add(k, p, v):
+        if len(p):
+            np.add.at(g[k], p, v * inv)
+
+    add(0, pos[1], np.sum(s[0] * r[0], axis=1))
+    add(1, pos[2], np.sum(s[1] * r[1], axis=1))
+    add(2, pos[3], np.sum(s[2] * r[2], axis=1))
+    add(3, pos[0], np.sum(s[3] * r[3] + s[4] * r[4] + s[5] * r[5], axis=1))
+    add(4, pos[0], np.sum(s[3] * (r[4] + r[5]) + s[4] * (r[3] + r[5]) + s[5] * (r[3] + r[4]), axis=1))
+    add(5, pos[4], np.sum(s[6] * r[6], axis=1))
+    add(6, pos[5], np.sum(s[7] * r[7], axis=1))
+    add(7, pos[6], np.sum(s[8] * r[8], axis=1))
+    return g.reshape(shape)

@@ -78,9 +78,12 @@ struct DevBuf {
     bytes = b;
     return ELA_OK;
   }
-  ~DevBuf() {
+  void release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
   }
+  ~DevBuf() { release(); }
 };
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
@@ -102,7 +105,9 @@ struct ela_handle {
   DevBuf med;               // 8 arrays
   DevBuf taper;             // min_pad values
   DevBuf scratch_tm;        // time-major trace buffer
-  DevBuf coarse;            // Born coarse wavefield [nts][3][stride]
+  DevBuf wave2;             // Born scattered / adjoint wavefield, 2 x 9 arrays
+  DevBuf coarse;            // Born coarse-time vx, vy, vz slices [slot][3][stride]
+  DevBuf ckpt;              // Born adjoint checkpoints [segment][21][stride]
   DevBuf ss;                // Born secondary sources 2 x 9 arrays
   DevBuf dmod;              // Born perturbations / images, 8 arrays
   DevBuf pos;               // converted positions (int32) and an error flag
@@ -113,6 +118,9 @@ struct ela_handle {
   int by_x = 16;            // x extent of a tile
   int ychunk = 8;           // y planes per block
   int remap = 0;            // 1: contiguous tile ranges per XCD
+  int64_t born_segment = 0; // Born adjoint storage: 0 auto, -1 all samples, K > 0 checkpoints every K
+  int64_t born_budget = 0;  // device-memory budget of the automatic plan [bytes], 0: free memory
+  int64_t born_last_segment = 0;  // plan of the last Born adjoint (-1 full storage)
 };
 
 namespace {
@@ -386,14 +394,24 @@ struct Wave {
 };
 
 template <typename T>
-Wave<T> wave_ptrs(ela_handle* h) {
+Wave<T> wave_ptrs(DevBuf& b, int64_t stride) {
   Wave<T> w;
   for (int k = 0; k < 9; ++k) {
-    w.p0[k] = arr<T>(h->wave, h->stride, k);
-    w.p1[k] = arr<T>(h->wave, h->stride, 9 + k);
+    w.p0[k] = arr<T>(b, stride, k);
+    w.p1[k] = arr<T>(b, stride, 9 + k);
   }
   return w;
 }
+
+// The 9 components of a time-major trace buffer share one base pointer
+// (make_points offsets each component).
+template <typename T>
+struct TmViews {
+  T* v[9];
+  explicit TmViews(T* tm) {
+    for (int c = 0; c < 9; ++c) v[c] = tm;
+  }
+};
 
 int64_t interior_cells(const ela_handle* h) {
   return (h->cfg.nz - 2 * ela::FAT) * (h->cfg.nx - 2 * ela::FAT) * (h->cfg.ny - 2 * ela::FAT);
@@ -428,7 +446,7 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
   T* tmv[9];
   for (int c = 0; c < 9; ++c) tmv[c] = tm;
 
-  Wave<T> w = wave_ptrs<T>(h);
+  Wave<T> w = wave_ptrs<T>(h->wave, h->stride);
   CHK(fill<T>(h, h->wave.p, 18 * h->stride));
   if (!adjoint) {
     CHK(fill<T>(h, tm, nrec_tot * nts));
@@ -486,42 +504,59 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
   return ELA_OK;
 }
 
-// Pass 1 of the Born operators: propagate the background source and keep the
-// coarse-time vx, vy, vz (interpWavefieldVxVyVz_3D accumulation,
-// BornElasticGpuFunctions_3D.cu:1268-1316) resident in HBM as V[nts][3][stride].
+// ---- Born operators ----------------------------------------------------------
+//
+// Pass 1 propagates the background source and accumulates the coarse-time
+// vx, vy, vz (interpWavefieldVxVyVz_3D, BornElasticGpuFunctions_3D.cu:1268-1316):
+// fine step it2 of coarse step k adds its velocities to V[k] and V[k+1] with
+// the hat weights, so V[k] is final once coarse step k has run. Pass 2 needs
+// the secondary sources of coarse samples its and its+1, i.e. V[its-1..its+2].
+//
+// Forward: the two passes are interleaved, the background running two coarse
+// samples ahead of the scattered field, so only four coarse slices are ever
+// live (the reference stores all nts of them in pinned host memory; at
+// 72 M cells and 501 samples that is 436 GB).
+// Adjoint: the scattered field runs backwards in time and needs V in reverse
+// order. V is kept for all samples when it fits in the memory budget;
+// otherwise the forward sweep stores exact (FP32/FP64) checkpoints of the
+// background state plus the partial V of the segment start every K coarse
+// samples, and each segment is recomputed before its adjoint sweep. The
+// kernels and their order are the same in all cases, so the results are
+// bit-identical.
+
+// One coarse step k of pass 1 (sub fine steps): VL = V[k], VR = V[k+1].
 template <typename T>
-int born_pass1(ela_handle* h, const DevPoints& sp, T* const src_tr[9], T* V) {
-  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1;
-  const int64_t np = h->stride;
-  Wave<T> w = wave_ptrs<T>(h);
-  CHK(fill<T>(h, h->wave.p, 18 * np));
-  CHK(fill<T>(h, V, (int64_t)nts * 3 * np));
+int bg_coarse_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k, T* VL, T* VR) {
+  const int64_t sub = h->cfg.sub, hh = sub + 1, np = h->stride;
   const double* f = h->filt.data();
-  for (int64_t its = 0; its < nts - 1; ++its) {
-    T* L = V + its * 3 * np;
-    T* R = V + (its + 1) * 3 * np;
-    for (int64_t it2 = 1; it2 <= sub; ++it2) {
-      ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-      for (int k = 0; k < 3; ++k) { a.accl[k] = L + k * np; a.accr[k] = R + k * np; }
-      a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-      CHK((launch_step<T, ela::MODE_FWD, false, true, false>(h, a)));
-      ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
-      for (int k = 0; k < 3; ++k) { S.accl[k] = a.accl[k]; S.accr[k] = a.accr[k]; }
-      S.accfl = a.accfl; S.accfr = a.accfr;
-      launch_inject<T, true>(h, S, (int)its, f[it2 - 1], f[hh + it2 - 1]);
-      w.swap();
-    }
+  for (int64_t it2 = 1; it2 <= sub; ++it2) {
+    ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+    for (int c = 0; c < 3; ++c) { a.accl[c] = VL + c * np; a.accr[c] = VR + c * np; }
+    a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
+    CHK((launch_step<T, ela::MODE_FWD, false, true, false>(h, a)));
+    ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
+    for (int c = 0; c < 3; ++c) { S.accl[c] = a.accl[c]; S.accr[c] = a.accr[c]; }
+    S.accfl = a.accfl; S.accfr = a.accfr;
+    launch_inject<T, true>(h, S, (int)k, f[it2 - 1], f[hh + it2 - 1]);
+    w.swap();
   }
   return ELA_OK;
 }
 
+// Secondary sources (forward, ss = out) or imaging (adjoint, dmo += ...) of
+// coarse sample its; Vo, Vc, Vn are the 3-field slices V[its-1], V[its],
+// V[its+1] (null outside 0..nts-1).
 template <typename T>
-ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* V, int64_t its, const T* dm, T* dmo, T* const ss[9]) {
+ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* Vo, const T* Vc, const T* Vn, int64_t its, const T* dm,
+                                 T* dmo, T* const ss[9]) {
   ela::ImagingArgs<T> a;
   std::memset(&a, 0, sizeof(a));
-  const int64_t np = h->stride, nts = h->cfg.nts;
-  auto slice = [&](int64_t t, int k) -> const T* { return (t < 0 || t >= nts) ? nullptr : V + t * 3 * np + k * np; };
-  for (int k = 0; k < 3; ++k) { a.o[k] = slice(its - 1, k); a.c[k] = slice(its, k); a.n[k] = slice(its + 1, k); }
+  const int64_t np = h->stride;
+  for (int k = 0; k < 3; ++k) {
+    a.o[k] = Vo ? Vo + k * np : nullptr;
+    a.c[k] = Vc + k * np;
+    a.n[k] = Vn ? Vn + k * np : nullptr;
+  }
   for (int k = 0; k < 8; ++k) {
     if (dm) a.dm[k] = dm + k * np;
     if (dmo) a.dmo[k] = dmo + k * np;
@@ -529,7 +564,7 @@ ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* V, int64_t its, const T
   for (int k = 0; k < 9; ++k) a.ss[k] = ss[k];
   set_coefs<T>(h, a.cx, a.cy, a.cz);
   a.dts_inv = T(1.0 / (2.0 * h->cfg.dts));
-  a.its = (int)its; a.nts = (int)nts;
+  a.its = (int)its; a.nts = (int)h->cfg.nts;
   a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
   a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
   a.ys = h->cfg.nx * h->nzi;
@@ -546,6 +581,206 @@ void launch_imaging(ela_handle* h, const ela::ImagingArgs<T>& a) {
   hipLaunchKernelGGL((ela::k_imaging<T, 8, ADJ>), dim3(t.nblocks), dim3(ela::TZ, 8), 0, h->stream, a);
 }
 
+// Storage plan of the Born adjoint's coarse wavefield.
+struct BornPlan {
+  bool full = true;      // V for all nts samples
+  int64_t K = 0;         // checkpoint segment length [coarse samples]
+  int64_t S = 0;         // number of segments
+  int64_t slots = 0;     // V slices held (full: nts; checkpointed: K + 3)
+};
+
+// Arrays (of h->stride elements) the Born adjoint needs besides V and the
+// checkpoints: medium 8, two wavefields 36, secondary sources 18, images 8.
+constexpr int64_t kBornFixedArrays = 8 + 36 + 18 + 8;
+
+int born_plan(ela_handle* h, BornPlan& p) {
+  const int64_t nts = h->cfg.nts, ncoarse = nts - 1;
+  const double arr_b = double(h->stride) * h->esize;
+  if (h->born_segment < 0) {
+    p.full = true; p.slots = nts; return ELA_OK;
+  }
+  if (h->born_segment > 0) {
+    p.full = false;
+    p.K = std::min<int64_t>(h->born_segment, ncoarse);
+    p.S = (ncoarse + p.K - 1) / p.K;
+    p.slots = p.K + 3;
+    return ELA_OK;
+  }
+  // automatic: full storage if it fits, else the longest segment that fits
+  double budget;
+  if (h->born_budget > 0) {
+    budget = double(h->born_budget);
+  } else {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    budget = double(fr) + double(h->coarse.bytes) + double(h->ckpt.bytes) + double(h->wave2.bytes) +
+             double(h->ss.bytes) + double(h->dmod.bytes) + double(h->wave.bytes) + double(h->med.bytes) -
+             double(2ull << 30);  // 2 GiB margin for traces and the runtime
+  }
+  const double avail = budget / arr_b - double(kBornFixedArrays);
+  if (double(3 * nts) <= avail) { p.full = true; p.slots = nts; return ELA_OK; }
+  for (int64_t K = ncoarse; K >= 1; --K) {
+    const int64_t S = (ncoarse + K - 1) / K;
+    if (double(21 * S + 3 * (K + 3)) <= avail) {
+      p.full = false; p.K = K; p.S = S; p.slots = K + 3;
+      return ELA_OK;
+    }
+  }
+  return fail(ELA_ERR_MEM, "Born adjoint: %.1f GB of device memory cannot hold the checkpoints of %lld coarse samples",
+              budget / 1e9, (long long)nts);
+}
+
+template <typename T>
+int born_forward_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], const DevPoints& rp, T* tm,
+                     int64_t nrec_tot, const T* dm) {
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, np = h->stride;
+  CHK(h->coarse.alloc(4 * 3 * np * sizeof(T)));
+  CHK(h->wave2.alloc(18 * np * sizeof(T)));
+  CHK(h->ss.alloc(18 * np * sizeof(T)));
+  T* ring = (T*)h->coarse.p;
+  auto V = [&](int64_t t) -> T* { return (t < 0 || t >= nts) ? nullptr : ring + (t % 4) * 3 * np; };
+  T* ssL[9];
+  T* ssR[9];
+  for (int k = 0; k < 9; ++k) { ssL[k] = arr<T>(h->ss, np, k); ssR[k] = arr<T>(h->ss, np, 9 + k); }
+  Wave<T> bg = wave_ptrs<T>(h->wave, np), sc = wave_ptrs<T>(h->wave2, np);
+  CHK(fill<T>(h, h->wave.p, 18 * np));
+  CHK(fill<T>(h, h->wave2.p, 18 * np));
+  CHK(fill<T>(h, h->ss.p, 18 * np));
+  CHK(fill<T>(h, ring, 4 * 3 * np));
+  CHK(fill<T>(h, tm, nrec_tot * nts));
+  const double* f = h->filt.data();
+
+  auto scattered_step = [&](int64_t its) -> int {
+    if (its == 0) launch_imaging<T, false>(h, imaging_args<T>(h, V(-1), V(0), V(1), 0, dm, nullptr, ssL));
+    launch_imaging<T, false>(h, imaging_args<T>(h, V(its), V(its + 1), V(its + 2), its + 1, dm, nullptr, ssR));
+    for (int64_t it2 = 1; it2 <= sub; ++it2) {
+      ela::StepArgs<T> a = base_args<T>(h, sc.p0, sc.p1);
+      for (int k = 0; k < 9; ++k) { a.ssl[k] = ssL[k]; a.ssr[k] = ssR[k]; }
+      a.ssfl = T(f[it2 - 1]); a.ssfr = T(f[hh + it2 - 1]);
+      CHK((launch_step<T, ela::MODE_FWD, true, false, false>(h, a)));
+      launch_record<T>(h, make_points<T>(h, rp, sc.p0, TmViews<T>(tm).v, true, nrec_tot), (int)its, f[it2],
+                       f[hh + it2]);
+      sc.swap();
+    }
+    for (int k = 0; k < 9; ++k) std::swap(ssL[k], ssR[k]);
+    return ELA_OK;
+  };
+
+  int64_t next = 0;  // next scattered coarse step
+  for (int64_t k = 0; k <= nts - 2; ++k) {
+    if (k >= 3) CHK(fill<T>(h, V(k + 1), 3 * np));  // reuse the slice of V[k-3]
+    CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, V(k), V(k + 1)));
+    // V[0..k] are final (all of V once k == nts-2)
+    while (next <= nts - 2 && (next + 2 <= k || k == nts - 2)) CHK(scattered_step(next++));
+  }
+  h->last_steps = 2 * (nts - 1) * sub;
+  return ELA_OK;
+}
+
+template <typename T>
+int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], const DevPoints& rp, T* tm,
+                     int64_t nrec_tot, T* dm) {
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, np = h->stride;
+  BornPlan plan;
+  CHK(born_plan(h, plan));
+  h->born_last_segment = plan.full ? -1 : plan.K;
+  // release what the new plan does not reuse before allocating (peak memory)
+  const size_t ck_bytes = plan.full ? 0 : (size_t)plan.S * 21 * np * sizeof(T);
+  const size_t v_bytes = (size_t)plan.slots * 3 * np * sizeof(T);
+  if (h->ckpt.bytes != ck_bytes) h->ckpt.release();
+  if (h->coarse.bytes != v_bytes) h->coarse.release();
+  CHK(h->ckpt.alloc(ck_bytes));
+  CHK(h->coarse.alloc(v_bytes));
+  CHK(h->wave2.alloc(18 * np * sizeof(T)));
+  CHK(h->ss.alloc(18 * np * sizeof(T)));
+  T* pool = (T*)h->coarse.p;
+  T* ck = (T*)h->ckpt.p;
+  T* ssL[9];
+  T* ssR[9];
+  for (int k = 0; k < 9; ++k) { ssL[k] = arr<T>(h->ss, np, k); ssR[k] = arr<T>(h->ss, np, 9 + k); }
+  Wave<T> bg = wave_ptrs<T>(h->wave, np), ad = wave_ptrs<T>(h->wave2, np);
+  const double* f = h->filt.data();
+  int64_t steps = 0;
+
+  // V slice of coarse sample t in the current storage window [base, base + slots)
+  int64_t base = 0;
+  auto V = [&](int64_t t) -> T* {
+    return (t < 0 || t >= nts) ? nullptr : pool + (t - base) * 3 * np;
+  };
+
+  CHK(fill<T>(h, h->wave.p, 18 * np));
+  if (plan.full) {
+    CHK(fill<T>(h, pool, nts * 3 * np));
+    for (int64_t k = 0; k <= nts - 2; ++k) CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, V(k), V(k + 1)));
+    steps += (nts - 1) * sub;
+  } else {
+    // forward sweep: checkpoint s holds (p0, p1) and the partial V[s K] at the
+    // start of coarse step s K; two rotating slices carry the accumulation
+    T* rot[2] = {pool, pool + 3 * np};
+    CHK(fill<T>(h, pool, 6 * np));
+    for (int64_t k = 0; k <= nts - 2; ++k) {
+      if (k % plan.K == 0) {
+        T* c = ck + (k / plan.K) * 21 * np;
+        for (int f9 = 0; f9 < 9; ++f9) {
+          HIPCHK(hipMemcpyAsync(c + f9 * np, bg.p0[f9], np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+          HIPCHK(hipMemcpyAsync(c + (9 + f9) * np, bg.p1[f9], np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+        }
+        HIPCHK(hipMemcpyAsync(c + 18 * np, rot[k % 2], 3 * np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+      }
+      CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, rot[k % 2], rot[(k + 1) % 2]));
+      CHK(fill<T>(h, rot[k % 2], 3 * np));
+    }
+    steps += (nts - 1) * sub;
+  }
+
+  // adjoint sweep
+  CHK(fill<T>(h, h->wave2.p, 18 * np));
+  CHK(fill<T>(h, h->ss.p, 18 * np));
+  auto adjoint_step = [&](int64_t its) -> int {
+    for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
+      ela::StepArgs<T> a = base_args<T>(h, ad.p0, ad.p1);
+      for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
+      a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
+      CHK((launch_step<T, ela::MODE_BORN_ADJ, false, true, true>(h, a)));
+      ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm).v, true, nrec_tot);
+      for (int k = 0; k < 9; ++k) { R.accl[k] = ssL[k]; R.accr[k] = ssR[k]; }
+      R.accfl = a.accfl; R.accfr = a.accfr;
+      launch_inject<T, true>(h, R, (int)its, f[it2 + 1], f[hh + it2 + 1]);
+      ad.swap();
+    }
+    launch_imaging<T, true>(h, imaging_args<T>(h, V(its), V(its + 1), V(its + 2), its + 1, nullptr, dm, ssR));
+    for (int k = 0; k < 9; ++k) std::swap(ssL[k], ssR[k]);
+    for (int k = 0; k < 9; ++k) CHK(fill<T>(h, ssL[k], np));
+    steps += sub;
+    return ELA_OK;
+  };
+
+  if (plan.full) {
+    for (int64_t its = nts - 2; its >= 0; --its) CHK(adjoint_step(its));
+  } else {
+    for (int64_t s = plan.S - 1; s >= 0; --s) {
+      const int64_t k0 = s * plan.K, k1 = std::min(k0 + plan.K, nts - 1);
+      const int64_t kend = std::min(k1 + 1, nts - 2);  // last recomputed coarse step
+      base = k0;
+      // restore the background state and the partial V[k0]
+      const T* c = ck + s * 21 * np;
+      for (int f9 = 0; f9 < 9; ++f9) {
+        HIPCHK(hipMemcpyAsync(bg.p0[f9], c + f9 * np, np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(bg.p1[f9], c + (9 + f9) * np, np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+      }
+      HIPCHK(hipMemcpyAsync(V(k0), c + 18 * np, 3 * np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+      CHK(fill<T>(h, V(k0 + 1), (kend + 1 - k0) * 3 * np));
+      for (int64_t k = k0; k <= kend; ++k) CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, V(k), V(k + 1)));
+      steps += (kend + 1 - k0) * sub;
+      // V[k0 .. min(k1 + 1, nts - 1)] are final
+      for (int64_t its = k1 - 1; its >= k0; --its) CHK(adjoint_step(its));
+    }
+  }
+  launch_imaging<T, true>(h, imaging_args<T>(h, V(-1), V(0), V(1), 0, nullptr, dm, ssR));
+  h->last_steps = steps;
+  return ELA_OK;
+}
+
 template <typename T>
 int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, const void* dmod_in,
              void* dmod_out, const ela_points* rp_in, const ela_traces* rt_in, bool adjoint, int mem) {
@@ -553,7 +788,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
   if (h->cfg.nts < 3) return fail(ELA_ERR_ARG, "Born operators need nts >= 3");
   CHK(check_points(sp_in, "source"));
   CHK(check_points(rp_in, "receiver"));
-  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, np = h->stride;
+  const int64_t nts = h->cfg.nts, np = h->stride;
   DevPoints sp, rp;
   CHK(convert_points(h, sp_in, rp_in, mem, sp, rp));
   void* ds[9];
@@ -569,69 +804,23 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
   CHK(fill<T>(h, dm, 8 * np));
   CHK(copy_in_3d(h, dm, adjoint ? dmod_out : dmod_in, 8, mem));
 
-  CHK(h->coarse.alloc((size_t)nts * 3 * np * sizeof(T)));
-  T* V = (T*)h->coarse.p;
-  CHK(h->ss.alloc(18 * np * sizeof(T)));
-  T* ssL[9];
-  T* ssR[9];
-  for (int k = 0; k < 9; ++k) { ssL[k] = arr<T>(h->ss, np, k); ssR[k] = arr<T>(h->ss, np, 9 + k); }
-
   const int64_t nrec_tot = total_points(rp);
   CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
   T* tm = (T*)h->scratch_tm.p;
-  T* tmv[9];
-  for (int c = 0; c < 9; ++c) tmv[c] = tm;
 
-  HIPCHK(hipEventRecord(h->ev0, h->stream));
-  CHK(born_pass1<T>(h, sp, src_tr, V));
-  Wave<T> w = wave_ptrs<T>(h);
-  CHK(fill<T>(h, h->wave.p, 18 * np));
-  CHK(fill<T>(h, h->ss.p, 18 * np));
-  const double* f = h->filt.data();
-
-  if (!adjoint) {
-    CHK(fill<T>(h, tm, nrec_tot * nts));
-    launch_imaging<T, false>(h, imaging_args<T>(h, V, 0, dm, nullptr, ssL));
-    for (int64_t its = 0; its < nts - 1; ++its) {
-      launch_imaging<T, false>(h, imaging_args<T>(h, V, its + 1, dm, nullptr, ssR));
-      for (int64_t it2 = 1; it2 <= sub; ++it2) {
-        ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-        for (int k = 0; k < 9; ++k) { a.ssl[k] = ssL[k]; a.ssr[k] = ssR[k]; }
-        a.ssfl = T(f[it2 - 1]); a.ssfr = T(f[hh + it2 - 1]);
-        CHK((launch_step<T, ela::MODE_FWD, true, false, false>(h, a)));
-        launch_record<T>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2], f[hh + it2]);
-        w.swap();
-      }
-      for (int k = 0; k < 9; ++k) std::swap(ssL[k], ssR[k]);
-    }
-  } else {
+  if (adjoint) {
     int64_t off = 0;
     for (int c = 0; c < 9; ++c) {
       const int64_t n = rp.n[kCompGrid[c]];
       launch_transpose<T>(h, rec_tr[c], tm + off, n, nts, nts, nrec_tot);
       off += n;
     }
-    for (int64_t its = nts - 2; its >= 0; --its) {
-      for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
-        ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-        for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
-        a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-        CHK((launch_step<T, ela::MODE_BORN_ADJ, false, true, true>(h, a)));
-        ela::PointArgs<T> R = make_points<T>(h, rp, w.p0, tmv, true, nrec_tot);
-        for (int k = 0; k < 9; ++k) { R.accl[k] = ssL[k]; R.accr[k] = ssR[k]; }
-        R.accfl = a.accfl; R.accfr = a.accfr;
-        launch_inject<T, true>(h, R, (int)its, f[it2 + 1], f[hh + it2 + 1]);
-        w.swap();
-      }
-      launch_imaging<T, true>(h, imaging_args<T>(h, V, its + 1, nullptr, dm, ssR));
-      for (int k = 0; k < 9; ++k) std::swap(ssL[k], ssR[k]);
-      for (int k = 0; k < 9; ++k) CHK(fill<T>(h, ssL[k], np));
-    }
-    launch_imaging<T, true>(h, imaging_args<T>(h, V, 0, nullptr, dm, ssR));
   }
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  if (!adjoint) CHK(born_forward_run<T>(h, sp, src_tr, rp, tm, nrec_tot, dm));
+  else CHK(born_adjoint_run<T>(h, sp, src_tr, rp, tm, nrec_tot, dm));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   HIPCHK(hipGetLastError());
-  h->last_steps = 2 * (nts - 1) * sub;
 
   if (!adjoint) {
     int64_t off = 0;
@@ -651,7 +840,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
 
 template <typename T>
 int time_kernels(ela_handle* h, int reps, double* ms_v, double* ms_s) {
-  Wave<T> w = wave_ptrs<T>(h);
+  Wave<T> w = wave_ptrs<T>(h->wave, h->stride);
   ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
   float tv = 0.f, ts = 0.f;
   CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a, 3)));  // warm caches and code
@@ -830,6 +1019,21 @@ int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {
 int ela_set_tile_order(ela_handle* h, int xcd_contiguous) {
   if (!h) return fail(ELA_ERR_ARG, "null handle");
   h->remap = xcd_contiguous ? 1 : 0;
+  return ELA_OK;
+}
+
+int ela_set_born_storage(ela_handle* h, int64_t segment, int64_t budget_bytes) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  if (segment < -1) return fail(ELA_ERR_ARG, "segment must be -1, 0 or a positive number of coarse samples");
+  if (budget_bytes < 0) return fail(ELA_ERR_ARG, "negative memory budget");
+  h->born_segment = segment;
+  h->born_budget = budget_bytes;
+  return ELA_OK;
+}
+
+int ela_born_storage_used(const ela_handle* h, int64_t* segment) {
+  if (!h || !segment) return fail(ELA_ERR_ARG, "null argument");
+  *segment = h->born_last_segment;
   return ELA_OK;
 }
 

@@ -30,6 +30,17 @@ void half_cell_adjoint(const double* in, double* out, int64_t n, int64_t s, int6
   }
 }
 
+// Forward half-cell average (the transpose of half_cell_adjoint):
+//   out[i] = (in[i] + in[i+1])/2 for i < n-1,  out[n-1] = in[n-1].
+void half_cell_forward(const double* in, double* out, int64_t n, int64_t s, int64_t na) {
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t a = (i / s) % na;
+    out[i] = a < na - 1 ? 0.5 * (in[i] + in[i + s]) : in[i];
+  }
+}
+
+bool dims_ok(int64_t nz, int64_t nx, int64_t ny) { return nz >= 2 && nx >= 2 && ny >= 2; }
+
 }  // namespace
 
 extern "C" int ela_scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* rho,
@@ -64,6 +75,111 @@ extern "C" int ela_scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, 
     Mxz[i] *= tdt;
     Mxy[i] *= tdt;
     Myz[i] *= tdt;
+  }
+  return ELA_OK;
+}
+
+// Born perturbations (BornElasticShotsGpu_3D.cpp:167-235): (drho, dlambda, dmu)
+// -> drhox = rhoxDtw * Sx'(drho) (likewise y, z), dlame = 2 dtw dlambda,
+// dmu = 2 dtw dmu, dmuxz = 2 dtw Sz'(Sx'(dmu)), dmuxy = 2 dtw Sy'(Sx'(dmu)),
+// dmuyz = 2 dtw Sz'(Sy'(dmu)), with S' the half-cell average used for the
+// medium (half_cell_adjoint).
+extern "C" int ela_born_stagger(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
+                                const double* dm3, double* out8) {
+  if (!dims_ok(nz, nx, ny) || !med8 || !dm3 || !out8 || !(dtw > 0)) return ELA_ERR_ARG;
+  const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
+  const double* drho = dm3;
+  const double* dlam = dm3 + n;
+  const double* dmu = dm3 + 2 * n;
+  const double tdt = 2.0 * dtw;
+  std::vector<double> t(n);
+  half_cell_adjoint(drho, out8, n, sx, nx);
+  half_cell_adjoint(drho, out8 + n, n, sy, ny);
+  half_cell_adjoint(drho, out8 + 2 * n, n, sz, nz);
+  half_cell_adjoint(dmu, t.data(), n, sx, nx);
+  half_cell_adjoint(t.data(), out8 + 5 * n, n, sz, nz);
+  half_cell_adjoint(t.data(), out8 + 6 * n, n, sy, ny);
+  half_cell_adjoint(dmu, t.data(), n, sy, ny);
+  half_cell_adjoint(t.data(), out8 + 7 * n, n, sz, nz);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) out8[k * n + i] *= med8[k * n + i];
+    out8[3 * n + i] = tdt * dlam[i];
+    out8[4 * n + i] = tdt * dmu[i];
+    for (int k = 5; k < 8; ++k) out8[k * n + i] *= tdt;
+  }
+  return ELA_OK;
+}
+
+// Adjoint of ela_born_stagger (BornElasticShotsGpu_3D.cpp:541-607): the 8
+// images -> (drho, dlambda, dmu), overwritten.
+extern "C" int ela_born_unstagger(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
+                                  const double* img8, double* out3) {
+  if (!dims_ok(nz, nx, ny) || !med8 || !img8 || !out3 || !(dtw > 0)) return ELA_ERR_ARG;
+  const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
+  const double tdt = 2.0 * dtw;
+  std::vector<double> a(n), t(n), u(n);
+  double* drho = out3;
+  double* dlam = out3 + n;
+  double* dmu = out3 + 2 * n;
+  std::memset(drho, 0, n * sizeof(double));
+  const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
+  for (int k = 0; k < 3; ++k) {
+    for (int64_t i = 0; i < n; ++i) a[i] = img8[k * n + i] * med8[k * n + i];
+    half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
+    for (int64_t i = 0; i < n; ++i) drho[i] += t[i];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    dlam[i] = tdt * img8[3 * n + i];
+    dmu[i] = tdt * img8[4 * n + i];
+  }
+  // (outer, inner) forward averages of muxz, muxy, muyz
+  const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
+  const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
+  for (int k = 0; k < 3; ++k) {
+    for (int64_t i = 0; i < n; ++i) a[i] = tdt * img8[(5 + k) * n + i];
+    half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
+    half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
+    for (int64_t i = 0; i < n; ++i) dmu[i] += u[i];
+  }
+  return ELA_OK;
+}
+
+// Gradient of a function of the 8 scaled medium arrays with respect to
+// (rho, lambda, mu): the transposed Jacobian of ela_scale_medium applied to
+// g8 = dJ/d(rhoxDtw ... muyzDtw). rho?Dtw = 2 dtw / S'(rho) gives
+// d/drho = S(-(rho?Dtw)^2 / (2 dtw) g); lamb2MuDtw and lambDtw give
+// 2 dtw (g_l2m + g_l) for lambda and 4 dtw g_l2m for mu; mu??Dtw give
+// 2 dtw S_outer(S_inner(g)) for mu. out3 is overwritten.
+extern "C" int ela_medium_gradient(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
+                                   const double* g8, double* out3) {
+  if (!dims_ok(nz, nx, ny) || !med8 || !g8 || !out3 || !(dtw > 0)) return ELA_ERR_ARG;
+  const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
+  const double tdt = 2.0 * dtw;
+  std::vector<double> a(n), t(n), u(n);
+  double* grho = out3;
+  double* glam = out3 + n;
+  double* gmu = out3 + 2 * n;
+  std::memset(grho, 0, n * sizeof(double));
+  const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
+  for (int k = 0; k < 3; ++k) {
+    for (int64_t i = 0; i < n; ++i) {
+      const double b = med8[k * n + i];
+      a[i] = -b * b / tdt * g8[k * n + i];
+    }
+    half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
+    for (int64_t i = 0; i < n; ++i) grho[i] += t[i];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    glam[i] = tdt * (g8[3 * n + i] + g8[4 * n + i]);
+    gmu[i] = 2.0 * tdt * g8[3 * n + i];
+  }
+  const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
+  const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
+  for (int k = 0; k < 3; ++k) {
+    for (int64_t i = 0; i < n; ++i) a[i] = tdt * g8[(5 + k) * n + i];
+    half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
+    half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
+    for (int64_t i = 0; i < n; ++i) gmu[i] += u[i];
   }
   return ELA_OK;
 }

@@ -19,9 +19,9 @@
 //   - fields needing x or z derivatives are staged per plane in LDS with a
 //     4-cell halo; y derivatives come from an 8-deep register queue per
 //     thread, so each plane of each field is read from HBM once per tile;
-//   - block ids are remapped so that each XCD (blocks are dealt round-robin to
-//     the 8 XCDs) owns a contiguous range of tiles: x/z neighbours that share
-//     halo rows are resident on the same L2 at the same time.
+//   - blocks are dealt round-robin to the 8 XCDs by the hardware; an optional
+//     remap gives each XCD a contiguous range of tiles instead (measured
+//     slower on the 328 x 472 x 468 grid, so off by default).
 // No MFMA: the stencil is not a dense contraction; the kernels are HBM-bound.
 #pragma once
 

@@ -160,6 +160,48 @@ def layered_400(nts: int = 626, seed: int = 2024, n_rec_axis: int = 200, sigma: 
                     ricker(nts, dts, 10.0, 0.1), "explosive")
 
 
+def born_layered_400(nts: int = 501, seed: int = 2024, n_rec_axis: int = 150, sigma_bg: float = 15.0,
+                     density: float = 0.01, rel_amp: float = 0.02, seed_dm: int = 7) -> Workload:
+    """Born workload on the 400 x 400 x 250 layered model: background = the
+    model of layered_400 smoothed with a Gaussian of sigma_bg cells (Vp and
+    Vs smoothed, Gardner density from the smoothed Vp); sparse reflectivity
+    drho, dlambda, dmu nonzero on `density` of the model cells, N(0, rel_amp *
+    background). One explosive shot at the centre (z = 10 cells), 2.0 s
+    (501 samples of 4 ms, 8 fine steps each), 150 x 150 surface receivers
+    every 2 cells. The perturbations are in extra["dm3"] on the padded grid
+    (zero in the padding)."""
+    from scipy.ndimage import gaussian_filter
+    nxm, nym, nzm, d, pad = 400, 400, 250, 12.5, 30
+    vp, vs, _ = layered_model(nzm, nxm, nym, seed, sigma=8.0)
+    vp = gaussian_filter(vp.astype(np.float32), sigma_bg, mode="nearest").astype(np.float64)
+    vs = gaussian_filter(vs.astype(np.float32), sigma_bg, mode="nearest").astype(np.float64)
+    rho0 = 310.0 * vp ** 0.25
+    rho, lam, mu = vpvsrho_to_rholamemu(vp, vs, rho0)
+    del vp, vs, rho0
+    rng = np.random.default_rng(seed_dm)
+    ncell = rho.size
+    k = max(1, int(round(density * ncell)))
+    idx = rng.choice(ncell, size=k, replace=False)
+    dm = np.zeros((3, ncell))
+    for i, base in enumerate((rho, lam, mu)):
+        dm[i, idx] = rng.standard_normal(k) * rel_amp * base.ravel()[idx]
+    dm = dm.reshape((3,) + rho.shape)
+    m, pads = pad_model(np.stack([rho, lam, mu]), pad, pad, pad)
+    del rho, lam, mu
+    zm, zp, xm, xp, yp = pads
+    dmp = np.zeros_like(m)
+    dmp[:, FAT + yp:FAT + yp + nym, FAT + xm:FAT + xm + nxm, FAT + zm:FAT + zm + nzm] = dm
+    ny, nx, nz = m.shape[1:]
+    ax = _axes(nz, nx, ny, d, pads)
+    dts = 0.004
+    src = np.array([[200 * d, 200 * d, 10 * d]])
+    xs = np.arange(n_rec_axis) * 2 * d + (200 - n_rec_axis) * d
+    gx, gy = np.meshgrid(xs, xs, indexing="ij")
+    rec = np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], axis=1)
+    return Workload("born_layered_400x400x250", m[0], m[1], m[2], ax, nts, dts, 8, 0.99, src, rec,
+                    ricker(nts, dts, 10.0, 0.1), "explosive", extra={"dm3": dmp})
+
+
 def source_traces(w: Workload) -> List[np.ndarray]:
     """9 irregular source traces [n_src, nts] for the workload's source type."""
     n = w.src_xyz.shape[0]

@@ -200,6 +200,18 @@ class Propagator:
     def set_tile_order(self, xcd_contiguous: bool) -> None:
         check(self.lib.ela_set_tile_order(self.h, int(bool(xcd_contiguous))), "ela_set_tile_order")
 
+    def set_born_storage(self, segment: int = 0, budget_bytes: int = 0) -> None:
+        """Coarse-wavefield storage of the Born adjoint: 0 automatic (within
+        budget_bytes, 0 = free device memory), -1 every coarse sample kept,
+        K > 0 exact checkpoints every K coarse samples with recomputation."""
+        check(self.lib.ela_set_born_storage(self.h, int(segment), int(budget_bytes)), "ela_set_born_storage")
+
+    def born_storage_used(self) -> int:
+        """Plan of the last Born adjoint: -1 all samples kept, else the segment length."""
+        k = ctypes.c_int64()
+        check(self.lib.ela_born_storage_used(self.h, ctypes.byref(k)), "ela_born_storage_used")
+        return k.value
+
     def get_tiling(self):
         tx, yc = ctypes.c_int(), ctypes.c_int()
         check(self.lib.ela_get_tiling(self.h, ctypes.byref(tx), ctypes.byref(yc)), "ela_get_tiling")
@@ -238,3 +250,29 @@ def scale_medium(rho, lame, mu, dtw):
     check(lib.ela_scale_medium(nz, nx, ny, float(dtw), rho.ctypes.data, lame.ctypes.data, mu.ctypes.data,
                                out.ctypes.data), "ela_scale_medium")
     return out
+
+
+def _host3(name, dtw, med8, a, n_out):
+    lib = _lib.load()
+    med8 = np.ascontiguousarray(med8, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    ny, nx, nz = med8.shape[1:]
+    out = np.empty((n_out, ny, nx, nz), np.float64)
+    check(getattr(lib, name)(nz, nx, ny, float(dtw), med8.ctypes.data, a.ctypes.data, out.ctypes.data), name)
+    return out
+
+
+def born_stagger(dm3, med8, dtw):
+    """(drho, dlambda, dmu) [3, ny, nx, nz] -> the 8 staggered, scaled
+    perturbations of born_forward (BornElasticShotsGpu_3D.cpp:167-235)."""
+    return _host3("ela_born_stagger", dtw, med8, dm3, 8)
+
+
+def born_unstagger(img8, med8, dtw):
+    """Adjoint of born_stagger (BornElasticShotsGpu_3D.cpp:541-607)."""
+    return _host3("ela_born_unstagger", dtw, med8, img8, 3)
+
+
+def medium_gradient(g8, med8, dtw):
+    """dJ/d(8 scaled arrays) -> dJ/d(rho, lambda, mu) through scale_medium."""
+    return _host3("ela_medium_gradient", dtw, med8, g8, 3)

@@ -20,6 +20,10 @@
  *                                 (elastic_iso_float/src/fdParamElastic_3D.cpp:130-183)
  *   ela_interp_linear             spaceInterpGpu_3D's constructor, linear method
  *                                 (elastic_iso_float/include/spaceInterpGpu_3D.h:41)
+ *   ela_born_stagger              perturbation staggering/scaling of BornElasticShotsGpu_3D::forward
+ *                                 (elastic_iso_float/src/BornElasticShotsGpu_3D.cpp:167-235)
+ *   ela_born_unstagger            its adjoint in BornElasticShotsGpu_3D::adjoint
+ *                                 (elastic_iso_float/src/BornElasticShotsGpu_3D.cpp:541-607)
  *
  * Differences from the reference interface, all deliberate:
  *   - state lives in an opaque handle (no file-scope globals indexed by iGpu);
@@ -117,6 +121,24 @@ int ela_set_medium(ela_handle* h, const void* med8, int mem);
 int ela_scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* rho,
                      const double* lame, const double* mu, double* med8);
 
+/* Born model perturbations (drho, dlambda, dmu) [3][ny][nx][nz] -> the 8
+ * staggered, scaled arrays of ela_born_forward (BornElasticShotsGpu_3D.cpp:167-235).
+ * med8 is the output of ela_scale_medium for the background. Double precision. */
+int ela_born_stagger(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
+                     const double* dm3, double* out8);
+
+/* Adjoint of ela_born_stagger: the 8 images of ela_born_adjoint -> (drho,
+ * dlambda, dmu), overwritten (BornElasticShotsGpu_3D.cpp:541-607). */
+int ela_born_unstagger(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
+                       const double* img8, double* out3);
+
+/* Chain rule through ela_scale_medium: g8 = dJ/d(the 8 scaled arrays) ->
+ * dJ/d(rho, lambda, mu), overwritten. No counterpart in the reference (its
+ * inversion uses the Born adjoint as the gradient); used by the adjoint-state
+ * gradient. */
+int ela_medium_gradient(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
+                        const double* g8, double* out3);
+
 /* Nonlinear forward: source traces -> receiver traces. `rec` is overwritten. */
 int ela_nl_forward(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
                    const ela_points* rec_pos, ela_traces* rec, int mem);
@@ -137,6 +159,25 @@ int ela_born_forward(ela_handle* h, const ela_points* src_pos, const ela_traces*
  * device and adds to them). */
 int ela_born_adjoint(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
                      void* dmod8, const ela_points* rec_pos, const ela_traces* rec, int mem);
+
+/* Storage of the background coarse wavefield (vx, vy, vz at every coarse
+ * sample) that the Born adjoint images against. The reference keeps all of it
+ * in pinned host memory (BornElasticGpuFunctions_3D.cu:2147-2200), 3*nts
+ * arrays; here it stays in HBM when it fits, otherwise the forward sweep
+ * stores exact checkpoints of the background state every `segment` coarse
+ * samples and each segment is recomputed before its adjoint sweep (one extra
+ * background propagation; results are bit-identical).
+ *   segment = 0   automatic: all samples if they fit, else the longest
+ *                 segment that fits, within budget_bytes of device memory
+ *                 (0: the free device memory at call time);
+ *   segment = -1  always keep all samples;
+ *   segment = K   checkpoint every K coarse samples.
+ * The Born forward never stores the coarse wavefield: it propagates the
+ * background two coarse samples ahead of the scattered field. */
+int ela_set_born_storage(ela_handle* h, int64_t segment, int64_t budget_bytes);
+/* Plan used by the last Born adjoint: -1 all samples kept, K > 0 checkpoint
+ * segment length. */
+int ela_born_storage_used(const ela_handle* h, int64_t* segment);
 
 /* Axes of one acquisition grid (its origin already shifted by half a cell on
  * the staggered axes, as python_float/Elastic_iso_float_3D.py:97-110 does),
@@ -166,12 +207,12 @@ int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_
 
 /* Tile shape of the stencil kernels: x extent of a 64-wide z tile (4, 8 or
  * 16 rows) and the number of y planes one block marches through. Any
- * setting gives the same result; it only changes speed (the autotuner in
- * the Python package picks it per grid). */
+ * setting gives the same result; it only changes speed. */
 int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk);
 int ela_get_tiling(const ela_handle* h, int* tile_x, int* y_chunk);
-/* Block-to-tile order: 1 (default) deals each XCD a contiguous range of
- * tiles, 0 keeps the hardware's round-robin order. Speed only. */
+/* Block-to-tile order: 0 (default) keeps the hardware's round-robin
+ * dealing of blocks to the 8 XCDs, 1 gives each XCD a contiguous range of
+ * tiles. Speed only. */
 int ela_set_tile_order(ela_handle* h, int xcd_contiguous);
 
 /* Diagnostic timing: launch the velocity and the stress stencil kernels of a

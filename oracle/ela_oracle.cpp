@@ -17,6 +17,8 @@
 //   - nonlinear time loops:                elastic_iso_float/src/nonlinearPropElasticGpuFunctions_3D.cu:909-925, 1330-1345
 //   - Born time loops:                     elastic_iso_float/src/BornElasticGpuFunctions_3D.cu:1268-1407, 2147-2281
 //   - trilinear weights, unique points:    elastic_iso_float/src/spaceInterpGpu_3D.cpp:113-175, 214-279
+//   - adjoint-state gradient (no reference counterpart): the transpose of the
+//     linearised nl_forward, built from the pieces above (gradient()).
 //
 // Every loop is unfused and in the reference order (step, inject, damp,
 // record/extract, swap), so that this file is a literal statement of the
@@ -533,6 +535,74 @@ void scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, const T* rho, 
   }
 }
 
+// Adjoint-state gradient of a data functional with respect to the 8 scaled
+// medium arrays, for the nonlinear forward operator with fixed (already
+// medium-scaled) source traces. No counterpart in the reference (its
+// inversion images with the Born adjoint); this is the exact transpose of the
+// linearisation of nl_forward above, in the same unfused order:
+//   forward step j:  a' = W (a + M D b + s_j),   b' = W b,   record a'
+//   dJ/dM           = sum_j  mu_j (.) D(b_j)   (per medium array, the
+//                     derivative combination it multiplies in step_md)
+//   mu_j            = the transposed recursion of born_adjoint's loop
+//                     (step_dm, inject dJ/d(data), damp), whose p0 after
+//                     damping is W lambda_j = mu_j
+//   dJ/d(source traces) = mu_j extracted at the source points with the
+//                     injection's filter weights (record, as nl_adjoint).
+// `adj` holds dJ/d(receiver traces) [nReg][nts]; g8 and gsrc are accumulated.
+template <typename T>
+void gradient(const Grid& g, const T* med, const T* src, const int64_t* spos, const int64_t* nsrc, const T* adj,
+              const int64_t* rpos, const int64_t* nrec, T* g8, T* gsrc) {
+  Ctx<T> C(g);
+  const int64_t n = C.nModel, xs = C.xs, ys = C.ys;
+  const T* m[8];
+  for (int i = 0; i < 8; ++i) m[i] = med + i * n;
+  Points<T> S(spos, nsrc, const_cast<T*>(src), g.nts);
+  Points<T> R(rpos, nrec, const_cast<T*>(adj), g.nts);
+  Points<T> G(spos, nsrc, gsrc, g.nts);
+  const int64_t nsteps = (g.nts - 1) * g.sub;
+  std::vector<T> B(size_t(nsteps) * 9 * n);  // p1 before every forward step
+  Fields<T> p0(n), p1(n);
+  int64_t k = 0;
+  for (int64_t its = 0; its < g.nts - 1; ++its)
+    for (int64_t it2 = 1; it2 < g.sub + 1; ++it2, ++k) {
+      for (int f = 0; f < 9; ++f) std::copy(p1.f[f].begin(), p1.f[f].end(), B.begin() + (k * 9 + f) * n);
+      step_md(C, m, p0, p1, T(1));
+      inject(C, p0, S, its, it2 - 1);
+      damp(C, p0, p1);
+      p0.swap(p1);
+    }
+  p0.zero();
+  p1.zero();
+  const T *cx = C.cx, *cy = C.cy, *cz = C.cz;
+  for (int64_t its = g.nts - 2; its > -1; --its)
+    for (int64_t it2 = g.sub - 1; it2 > -1; --it2) {
+      step_dm(C, m, p0, p1);
+      inject(C, p0, R, its, it2 + 1);
+      damp(C, p0, p1);
+      const T* b = B.data() + size_t(its * g.sub + it2) * 9 * n;
+      const T *vx = b, *vy = b + n, *vz = b + 2 * n, *sxx = b + 3 * n, *syy = b + 4 * n, *szz = b + 5 * n;
+      const T *sxz = b + 6 * n, *sxy = b + 7 * n, *syz = b + 8 * n;
+      const T* u[9];
+      for (int f = 0; f < 9; ++f) u[f] = p0.f[f].data();
+      for_interior(C, [&](int64_t i) {
+        const T sx = dminus(sxx, i, xs, cx) + dplus(sxy, i, ys, cy) + dplus(sxz, i, 1, cz);
+        const T sy = dplus(sxy, i, xs, cx) + dminus(syy, i, ys, cy) + dplus(syz, i, 1, cz);
+        const T sz = dplus(sxz, i, xs, cx) + dplus(syz, i, ys, cy) + dminus(szz, i, 1, cz);
+        const T exx = dplus(vx, i, xs, cx), eyy = dplus(vy, i, ys, cy), ezz = dplus(vz, i, 1, cz);
+        g8[BX * n + i] += u[VX][i] * sx;
+        g8[BY * n + i] += u[VY][i] * sy;
+        g8[BZ * n + i] += u[VZ][i] * sz;
+        g8[L2M * n + i] += u[SXX][i] * exx + u[SYY][i] * eyy + u[SZZ][i] * ezz;
+        g8[LAM * n + i] += u[SXX][i] * (eyy + ezz) + u[SYY][i] * (exx + ezz) + u[SZZ][i] * (exx + eyy);
+        g8[MXY * n + i] += u[SXY][i] * (dminus(vx, i, ys, cy) + dminus(vy, i, xs, cx));
+        g8[MXZ * n + i] += u[SXZ][i] * (dminus(vx, i, 1, cz) + dminus(vz, i, xs, cx));
+        g8[MYZ * n + i] += u[SYZ][i] * (dminus(vz, i, ys, cy) + dminus(vy, i, 1, cz));
+      });
+      record(C, p0, G, its, it2);
+      p0.swap(p1);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -571,6 +641,11 @@ static Grid to_grid(const orc_grid* p) {
                               const int64_t* nsrc, T* dmod, const T* rec, const int64_t* rpos,    \
                               const int64_t* nrec) {                                              \
     born_adjoint<T>(to_grid(g), med, src, spos, nsrc, dmod, rec, rpos, nrec);                     \
+  }                                                                                               \
+  void orc_gradient_##SFX(const orc_grid* g, const T* med, const T* src, const int64_t* spos,       \
+                          const int64_t* nsrc, const T* adj, const int64_t* rpos, const int64_t* nrec, \
+                          T* g8, T* gsrc) {                                                       \
+    gradient<T>(to_grid(g), med, src, spos, nsrc, adj, rpos, nrec, g8, gsrc);                     \
   }                                                                                               \
   void orc_stagger_##SFX(int axis, int adjoint, int64_t nz, int64_t nx, int64_t ny, const T* in,  \
                          T* out) {                                                                \

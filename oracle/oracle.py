@@ -209,6 +209,27 @@ def lib_born_adjoint(g: Grid, med8, src_traces, src_pos, rec_traces, rec_pos, dt
     return dm
 
 
+def lib_gradient(g: Grid, med8, src_traces, src_pos, adj_traces, rec_pos, dtype=np.float64):
+    """Adjoint-state gradient of <adj, nl_forward(med8)> with respect to the 8
+    scaled medium arrays (fixed scaled source traces), and its gradient with
+    respect to the scaled source traces. Returns (g8 [8, ny, nx, nz], gsrc
+    [9 arrays [nReg, nts]])."""
+    L = lib()
+    sp, sn = _pack_pos(src_pos)
+    rp, rn = _pack_pos(rec_pos)
+    med = np.ascontiguousarray(med8, dtype=dtype)
+    src = _pack_traces(src_traces, sn, g.nts, dtype)
+    adj = _pack_traces(adj_traces, rn, g.nts, dtype)
+    g8 = np.zeros((8,) + g.shape, dtype)
+    gsrc = np.zeros_like(src)
+    ct = _ct(dtype)
+    gc = g.c()
+    getattr(L, "orc_gradient_" + _sfx(dtype))(
+        ctypes.byref(gc), _ptr(med, ct), _ptr(src, ct), _ptr(sp, ctypes.c_int64), _ptr(sn, ctypes.c_int64),
+        _ptr(adj, ct), _ptr(rp, ctypes.c_int64), _ptr(rn, ctypes.c_int64), _ptr(g8, ct), _ptr(gsrc, ct))
+    return g8, _unpack_traces(gsrc, sn, g.nts)
+
+
 def lib_stagger(axis: int, adjoint: bool, a, dtype=np.float64):
     """axis 0=z 1=x 2=y on a [ny, nx, nz] array."""
     a = np.ascontiguousarray(a, dtype=dtype)

@@ -113,3 +113,55 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(_lib.ElaError):
         _lib.load()
+
+
+def _rand_model(shape, rng):
+    rho = 1800 + 400 * rng.random(shape)
+    lam = 1e9 * (3 + rng.random(shape))
+    mu = 1e9 * (2 + rng.random(shape))
+    return rho, lam, mu
+
+
+def test_born_stagger_matches_oracle(lib):
+    from elasticisolib_3d_amd.propagator import scale_medium, born_stagger, born_unstagger
+    rng = np.random.default_rng(3)
+    shape = (7, 9, 11)
+    rho, lam, mu = _rand_model(shape, rng)
+    dtw = 5e-4
+    med = scale_medium(rho, lam, mu, dtw)
+    dm3 = rng.standard_normal((3,) + shape)
+    a = born_stagger(dm3, med, dtw)
+    b = O.born_stagger_perturbation(dm3, med, dtw, O.lib_stagger)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    img8 = rng.standard_normal((8,) + shape)
+    a = born_unstagger(img8, med, dtw)
+    b = O.born_unstagger_image(img8, med, dtw, O.lib_stagger)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    # and they are transposes of each other
+    lhs = np.sum(born_stagger(dm3, med, dtw) * img8)
+    rhs = np.sum(dm3 * born_unstagger(img8, med, dtw))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_medium_gradient_is_transposed_jacobian(lib):
+    """<J dm, g8> = <dm, medium_gradient(g8)> with J dm from a central
+    difference of scale_medium (quadratic error ~ h^2, h = 1e-5 relative)."""
+    from elasticisolib_3d_amd.propagator import scale_medium, medium_gradient
+    rng = np.random.default_rng(4)
+    shape = (6, 8, 7)
+    rho, lam, mu = _rand_model(shape, rng)
+    dtw = 5e-4
+    med = scale_medium(rho, lam, mu, dtw)
+    g8 = rng.standard_normal((8,) + shape)
+    g3 = medium_gradient(g8, med, dtw)
+    for k, base in enumerate((rho, lam, mu)):
+        d = rng.standard_normal(shape) * base
+        h = 1e-5
+        args_p = [rho, lam, mu]
+        args_m = [rho, lam, mu]
+        args_p[k] = base + h * d
+        args_m[k] = base - h * d
+        jd = (scale_medium(*args_p, dtw) - scale_medium(*args_m, dtw)) / (2 * h)
+        lhs = np.sum(jd * g8)
+        rhs = np.sum(d * g3[k])
+        assert abs(lhs - rhs) <= 1e-7 * abs(rhs), (k, lhs, rhs)

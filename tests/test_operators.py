@@ -1,0 +1,165 @@
+"""CPU tests of the pySolver operator surface: parameter and SEP files, the
+fdParam QC, geometry builders, parameter conversion, and the shot operators'
+layouts and adjointness (propagation through the oracle-backed stand-in)."""
+import os
+
+import numpy as np
+import pytest
+
+from elasticisolib_3d_amd import operators as OP
+from elasticisolib_3d_amd.sep import Axis, Hypercube, ParObject, Vector, getSepVector, read_vector, write_vector
+from tests import operator_problem as PB
+
+
+def test_par_object(tmp_path):
+    f = tmp_path / "p.par"
+    f.write_text("nts=10 dts=0.004 # comment\nname=\"a b\"\niGpu=0,1\n")
+    p = ParObject(["prog", f"par={f}", "nts=12", "sub=3.0"])
+    assert p.getInt("nts") == 12 and p.getFloat("dts") == 0.004 and p.getString("name") == "a b"
+    assert p.getInts("iGpu") == [0, 1] and p.getInt("sub") == 3 and p.getInt("absent", 7) == 7
+    with pytest.raises(KeyError):
+        p.getInt("absent")
+
+
+def test_sep_roundtrip(tmp_path):
+    hyp = Hypercube([Axis(5, 0.5, 2.0), Axis(3), Axis(2, 1.0, 0.1)])
+    v = Vector(hyp, np.arange(30, dtype=np.float32).reshape(2, 3, 5))
+    write_vector(str(tmp_path / "v.H"), v)
+    w = read_vector(str(tmp_path / "v.H"), ndims=4)
+    assert w.getNdArray().shape == (1, 2, 3, 5)
+    assert np.array_equal(w.getNdArray()[0], v.getNdArray())
+    assert w.getHyper().getAxis(1).o == 0.5 and w.getHyper().getAxis(3).d == pytest.approx(0.1)
+    # big-endian (xdr) input
+    (tmp_path / "b.bin").write_bytes(np.arange(4, dtype=">f4").tobytes())
+    (tmp_path / "b.H").write_text(f'n1=4 esize=4 data_format="xdr_float" in="{tmp_path / "b.bin"}"\n')
+    assert np.array_equal(read_vector(str(tmp_path / "b.H")).getNdArray(), np.arange(4, dtype=np.float32))
+
+
+def test_fdparam_qc_and_staggering():
+    par = PB.make_par()
+    ep = PB.elastic_vector()
+    fd = OP.FdParam(ep, par)
+    assert fd.sub == 8 and fd.min_pad == 5 and fd.courant < 0.45
+    from elasticisolib_3d_amd.propagator import scale_medium
+    e = ep.getNdArray().astype(np.float64)
+    assert np.allclose(fd.med8, scale_medium(e[0], e[1], e[2], fd.dtw))
+    with pytest.raises(ValueError, match="Courant"):
+        OP.FdParam(ep, PB.make_par(sub=1))
+    with pytest.raises(ValueError, match="Dispersion"):
+        OP.FdParam(ep, PB.make_par(fMax=200))
+    with pytest.raises(ValueError, match="block size"):
+        OP.FdParam(ep, PB.make_par(blockSize=16))
+    with pytest.raises(ValueError, match="nz not consistent"):
+        OP.FdParam(ep, PB.make_par(nz=PB.NZ + 1))
+    with pytest.raises(ValueError, match="SUB_MAX"):
+        OP.FdParam(ep, PB.make_par(sub=51))
+    # without mod_par the C++ default (2) multiplies spacings by 1000: the
+    # metre-valued hypercube then fails the consistency check
+    par2 = PB.make_par()
+    par2._p.pop("mod_par")
+    with pytest.raises(ValueError, match="dz not consistent"):
+        OP.FdParam(ep, par2)
+
+
+def test_geometry_builders_and_layouts():
+    par = PB.make_par()
+    ep = PB.elastic_vector()
+    sgeo, rgeo = PB.geometries()
+    src = OP.buildSourceGeometry_3D(par, ep, sgeo)
+    rec = OP.buildReceiversGeometry_3D(par, ep, rgeo)
+    assert len(src) == 8 and len(src[0]) == 2 and src[7].n == 2
+    assert len(rec) == 8 and len(rec[0]) == 1 and rec[7].n == 5
+    # the x grid is shifted by half a cell in x only
+    cx = src[0][0].getRegPosUnique()
+    assert len(cx) == 8 and src[1][0].getNDeviceIrreg() == 1
+    with pytest.raises(ValueError):
+        OP.buildReceiversGeometry_3D(PB.make_par(nReceiverPerShot=4), ep, rgeo)
+    with pytest.raises(NotImplementedError):
+        OP.buildSourceGeometry_3D(PB.make_par(spaceInterpMethod="sinc", hFilter1d=2), ep, sgeo)
+
+
+def test_elastic_conversion():
+    rng = np.random.default_rng(2)
+    hyp = Hypercube([Axis(4), Axis(3), Axis(2), Axis(3)])
+    m = Vector(hyp, np.stack([2500 + rng.random((2, 3, 4)), 1400 + rng.random((2, 3, 4)),
+                              2000 + rng.random((2, 3, 4))]).astype(np.float64))
+    d = m.cloneSpace()
+    OP.ElasticConv_3D(m, 1).forward(False, m, d)
+    back = m.cloneSpace()
+    OP.ElasticConv_3D(m, -1).forward(False, d, back)
+    assert np.allclose(back.getNdArray(), m.getNdArray())
+    jab = OP.ElasticConvJab_3D(m, None, 1)
+    jab.setBackground(m)
+    assert jab.dotTestCpp(maxError=1e-12, seed=3)
+    # linearisation: central difference of the conversion
+    dm = m.clone().rand(4)
+    h = 1e-3
+    plus, minus = m.cloneSpace(), m.cloneSpace()
+    OP.ElasticConv_3D(m, 1).forward(False, Vector(hyp, m.getNdArray() + h * dm.getNdArray()), plus)
+    OP.ElasticConv_3D(m, 1).forward(False, Vector(hyp, m.getNdArray() - h * dm.getNdArray()), minus)
+    lin = m.cloneSpace()
+    jab.forward(False, dm, lin)
+    fd = (plus.getNdArray() - minus.getNdArray()) / (2 * h)
+    assert np.allclose(lin.getNdArray(), fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
+
+
+def _nl_op(monkeypatch, const_src=True):
+    PB.use_oracle_backend(monkeypatch)
+    par = PB.make_par()
+    ep = PB.elastic_vector()
+    sgeo, rgeo = PB.geometries()
+    src = OP.buildSourceGeometry_3D(par, ep, sgeo)
+    rec = OP.buildReceiversGeometry_3D(par, ep, rgeo)
+    t = Axis(par.getInt("nts"), 0.0, par.getFloat("dts"))
+    model = getSepVector(Hypercube([t, Axis(1), Axis(9), Axis(1 if const_src else 2)]), storage="dataDouble")
+    data = getSepVector(Hypercube([t, rec[7], Axis(9), src[7]]), storage="dataDouble")
+    op = OP.nonlinearPropElasticShotsGpu_3D(model, data, ep, par, *src[:7], *rec[:7])
+    return op, model, data, ep, par, src, rec
+
+
+@pytest.mark.parametrize("const_src", [True, False])
+def test_nonlinear_shots_operator_dot_product(monkeypatch, const_src):
+    op, model, data, *_ = _nl_op(monkeypatch, const_src)
+    assert op.dotTestCpp(maxError=1e-10, seed=5)
+    op.dotTest(maxError=1e-10, seed=6)
+
+
+def test_nonlinear_shots_operator_matches_single_shot(monkeypatch):
+    """Shot 1 of the operator's data equals a direct single-shot propagation
+    built from the same geometry (layout [nExp, 9, nRec, nts])."""
+    from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
+    op, model, data, ep, par, src, rec = _nl_op(monkeypatch, False)
+    model.rand(7)
+    op.forward(False, model, data)
+    sg = ShotGeometry.from_grids([src[g][1] for g in range(7)])
+    rg = ShotGeometry.from_grids([rec[g][0] for g in range(7)])
+    m = model.getNdArray()[1]
+    reg = scale_regular_traces(op.fd.med8, sg, sg.to_regular(list(m)), PB.D ** 3)
+    out = rg.to_irregular(op.prop.nl_forward(sg.positions(), reg, rg.positions()))
+    assert np.allclose(np.stack(out), data.getNdArray()[1], rtol=1e-12, atol=1e-14 * np.abs(out).max())
+    # add=True accumulates
+    d2 = data.clone()
+    op.forward(True, model, d2)
+    assert np.allclose(d2.getNdArray(), 2 * data.getNdArray())
+
+
+def test_born_shots_operator_dot_product(monkeypatch):
+    PB.use_oracle_backend(monkeypatch)
+    par = PB.make_par()
+    ep = PB.elastic_vector()
+    sgeo, rgeo = PB.geometries()
+    src = OP.buildSourceGeometry_3D(par, ep, sgeo)
+    rec = OP.buildReceiversGeometry_3D(par, ep, rgeo)
+    t = Axis(par.getInt("nts"), 0.0, par.getFloat("dts"))
+    sig = getSepVector(Hypercube([t, Axis(1), Axis(9)]), storage="dataDouble")
+    sig.getNdArray()[5] = np.sin(np.arange(par.getInt("nts")))  # mzz
+    model = getSepVector(ep.getHyper(), storage="dataDouble")
+    data = getSepVector(Hypercube([t, rec[7], Axis(9), src[7]]), storage="dataDouble")
+    op = OP.BornElasticShotsGpu_3D(model, data, ep, par, [sig], *src[:7], *rec[:7])
+    assert op.dotTestCpp(maxError=1e-10, seed=8)
+
+
+def test_shots_operator_refuses_domain_decomposition(monkeypatch):
+    PB.use_oracle_backend(monkeypatch)
+    with pytest.raises(NotImplementedError):
+        OP._device_of(PB.make_par(domDec=1))

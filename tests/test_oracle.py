@@ -357,3 +357,108 @@ def test_point_force_matches_stokes_solution():
     err = np.linalg.norm(vz[win] - ana[win]) / np.linalg.norm(ana[win])
     print(f"Stokes on-axis vz relative L2 error {err:.4f}")
     assert err < 0.05, err
+
+
+# ---------------------------------------------------------------------------
+# Adjoint-state gradient (no reference counterpart): pinned by central finite
+# differences of the FP64 forward operator.
+# ---------------------------------------------------------------------------
+
+def _grad_problem(seed=21):
+    from tests.problems import random_problem
+    g = O.Grid(nz=20, nx=22, ny=18, nts=5, sub=4, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=4, alphaCos=0.9)
+    P = random_problem(g, seed=seed, nsrc=2, nrec=6)
+    return g, P
+
+
+def test_gradient_matches_central_differences_medium():
+    """<g8, dM> equals (J(M + h dM) - J(M - h dM)) / 2h for J = <adj, F(M)>,
+    along random dense directions and along single cells of each array."""
+    g, P = _grad_problem()
+    med, adj = P["med"], P["rec"]
+    g8, _ = O.lib_gradient(g, med, P["src"], P["spos"], adj, P["rpos"])
+
+    def J(m):
+        out = O.lib_nl_forward(g, m, P["src"], P["spos"], P["rpos"])
+        return sum(float(np.sum(a * b)) for a, b in zip(out, adj))
+
+    rng = np.random.default_rng(3)
+    h = 1e-4
+    for trial in range(2):
+        dm = rng.standard_normal(med.shape) * med
+        fd = (J(med + h * dm) - J(med - h * dm)) / (2 * h)
+        an = float(np.sum(g8 * dm))
+        assert abs(fd - an) <= 1e-6 * abs(an), (trial, fd, an)
+    # single cells: the largest-gradient cell of each array
+    for k in range(8):
+        i = np.unravel_index(np.argmax(np.abs(g8[k])), g8[k].shape)
+        dm = np.zeros_like(med)
+        dm[(k,) + i] = med[(k,) + i]
+        fd = (J(med + h * dm) - J(med - h * dm)) / (2 * h)
+        an = float(np.sum(g8 * dm))
+        assert abs(fd - an) <= 1e-5 * abs(an), (k, fd, an)
+
+
+def test_gradient_source_traces_is_adjoint():
+    """J is linear in the scaled source traces: <gsrc, ds> = <adj, F(ds)>."""
+    g, P = _grad_problem(seed=22)
+    _, gsrc = O.lib_gradient(g, P["med"], P["src"], P["spos"], P["rec"], P["rpos"])
+    rng = np.random.default_rng(4)
+    ds = [rng.standard_normal(s.shape) for s in P["src"]]
+    out = O.lib_nl_forward(g, P["med"], ds, P["spos"], P["rpos"])
+    lhs = sum(float(np.sum(a * b)) for a, b in zip(gsrc, ds))
+    rhs = sum(float(np.sum(a * b)) for a, b in zip(out, P["rec"]))
+    assert abs(lhs - rhs) <= 1e-12 * abs(rhs)
+
+
+def test_gradient_through_elastic_parameters():
+    """Full chain (rho, lambda, mu) -> scaled medium and medium-scaled source
+    traces -> data: the product's host chain rule (medium_gradient,
+    source_scaling_gradient) on top of the oracle gradient matches central
+    differences in (rho, lambda, mu)."""
+    from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces, source_scaling_gradient
+    from elasticisolib_3d_amd.propagator import scale_medium, medium_gradient
+    from elasticisolib_3d_amd.acquisition import PaddedAxes
+    g, P = _grad_problem(seed=23)
+    d = 10.0
+    ax = PaddedAxes(nz=g.nz, nx=g.nx, ny=g.ny, oz=-(4 + 2) * d, ox=-(4 + 2) * d, oy=-(4 + 2) * d, dz=d, dx=d,
+                    dy=d, fat=4, z_pad_minus=2, z_pad_plus=2, x_pad_minus=2, x_pad_plus=2, y_pad=2)
+    sg = ShotGeometry(ax, [33.0], [41.0], [27.0])
+    rg = ShotGeometry(ax, [5.0, 11.0, 52.0], [12.0, 60.0, 37.0], [8.0, 44.0, 20.0])
+    rng = np.random.default_rng(5)
+    raw = [rng.standard_normal((n, g.nts)) for n in [sg.counts()[c] for c in (1, 2, 3, 0, 0, 0, 4, 5, 6)]]
+    adj = [rng.standard_normal((n, g.nts)) for n in [rg.counts()[c] for c in (1, 2, 3, 0, 0, 0, 4, 5, 6)]]
+    rho, lam, mu = P["rho"], P["lam"], P["mu"]
+    dV = d ** 3
+
+    def J(r, l, m):
+        med = scale_medium(r, l, m, g.dtw)
+        out = O.lib_nl_forward(g, med, scale_regular_traces(med, sg, raw, dV), sg.positions(), rg.positions())
+        return sum(float(np.sum(a * b)) for a, b in zip(out, adj))
+
+    med = scale_medium(rho, lam, mu, g.dtw)
+    g8, gsrc = O.lib_gradient(g, med, scale_regular_traces(med, sg, raw, dV), sg.positions(), adj, rg.positions())
+    g8 = g8 + source_scaling_gradient(med, sg, raw, gsrc, dV)
+    g3 = medium_gradient(g8, med, g.dtw)
+    h = 1e-5
+    base = (rho, lam, mu)
+    for k in range(3):
+        dm = rng.standard_normal(rho.shape) * base[k]
+        plus = list(base); minus = list(base)
+        plus[k] = base[k] + h * dm
+        minus[k] = base[k] - h * dm
+        fd = (J(*plus) - J(*minus)) / (2 * h)
+        an = float(np.sum(g3[k] * dm))
+        assert abs(fd - an) <= 1e-6 * abs(an), (k, fd, an)
+    # a cell at the source: the source-scaling term matters there
+    p = sg.positions()[0][0]
+    iy, ix, iz = np.unravel_index(p, rho.shape)
+    for k in (1, 2):
+        dm = np.zeros_like(rho)
+        dm[iy, ix, iz] = base[k][iy, ix, iz]
+        plus = list(base); minus = list(base)
+        plus[k] = base[k] + h * dm
+        minus[k] = base[k] - h * dm
+        fd = (J(*plus) - J(*minus)) / (2 * h)
+        an = float(np.sum(g3[k] * dm))
+        assert abs(fd - an) <= 1e-5 * abs(an), (k, fd, an)
