@@ -26,6 +26,7 @@ EXPORTS = (
     "ela_interp_linear", "ela_time_step_kernels", "ela_set_tiling", "ela_get_tiling",
     "ela_set_tile_order", "ela_set_born_storage", "ela_born_storage_used",
     "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
+    "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
 )
 
 
@@ -107,6 +108,11 @@ def load() -> ctypes.CDLL:
     lib.ela_set_tiling.argtypes = [vp, i32, i32]
     lib.ela_set_tile_order.argtypes = [vp, i32]
     lib.ela_set_born_storage.argtypes = [vp, i64, i64]
+    lib.ela_gradient_forward.argtypes = [vp, P, Tr, P, Tr, i32]
+    lib.ela_gradient_backward.argtypes = [vp, P, Tr, P, Tr, vp, Tr, i32]
+    lib.ela_set_gradient_checkpoints.argtypes = [vp, i64, i64, i32, i64]
+    lib.ela_gradient_plan_used.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
+                                           ctypes.POINTER(i64)]
     lib.ela_born_storage_used.argtypes = [vp, ctypes.POINTER(i64)]
     lib.ela_get_tiling.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.ela_time_step_kernels.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]

@@ -121,6 +121,21 @@ struct ela_handle {
   int64_t born_segment = 0; // Born adjoint storage: 0 auto, -1 all samples, K > 0 checkpoints every K
   int64_t born_budget = 0;  // device-memory budget of the automatic plan [bytes], 0: free memory
   int64_t born_last_segment = 0;  // plan of the last Born adjoint (-1 full storage)
+  // adjoint-state gradient
+  DevBuf ck1;               // outer checkpoints [n1][18][interior] (bfloat16 or native)
+  DevBuf ck2;               // inner checkpoints of one outer segment [K1/K][18][interior] (native)
+  DevBuf ckscale;           // per-checkpoint array scales [n1][18] + amax scratch [18]
+  DevBuf drv;               // derivative combinations of one inner segment [K][9][stride]
+  DevBuf g32;               // per-segment gradient partial sums [8][stride]
+  DevBuf g64;               // FP64 gradient [8][stride]
+  DevBuf stage2;            // host-call staging of the source-trace gradient
+  int64_t grad_interval = 20;   // inner interval K (fine steps)
+  int64_t grad_outer = 0;       // outer interval K1 (0: automatic)
+  int grad_bf16 = 1;            // outer checkpoints in bfloat16
+  int64_t grad_budget = 0;      // device-memory budget of the automatic plan (0: free memory)
+  bool grad_ready = false;      // a gradient forward has stored its checkpoints
+  int64_t plan_K = 0, plan_K1 = 0, plan_n1 = 0;
+  int plan_bf16 = 0;
 };
 
 namespace {
@@ -195,23 +210,24 @@ ela::StepArgs<T> base_args(ela_handle* h, T* o[9], T* c[9]) {
   return a;
 }
 
-// which: bit 0 = velocity kernel, bit 1 = stress kernel
-template <typename T, int BX, int MODE, bool SS, bool ACCV, bool ACCS>
+// which: bit 0 = velocity kernel, bit 1 = stress kernel; FV / FS: the
+// ela::F_* work fused into the velocity / stress launch.
+template <typename T, int BX, int MODE, int FV, int FS>
 int launch_step_bx(ela_handle* h, ela::StepArgs<T> a, int which) {
   Tiles t = tiles_for(h, BX, a.ychunk);
   a.nbz = t.nbz; a.nbx = t.nbx; a.nby = t.nby;
   dim3 block(ela::TZ, BX);
-  if (which & 1) hipLaunchKernelGGL((ela::k_velocity<T, BX, MODE, SS, ACCV>), dim3(t.nblocks), block, 0, h->stream, a);
-  if (which & 2) hipLaunchKernelGGL((ela::k_stress<T, BX, MODE, SS, ACCS>), dim3(t.nblocks), block, 0, h->stream, a);
+  if (which & 1) hipLaunchKernelGGL((ela::k_velocity<T, BX, MODE, FV>), dim3(t.nblocks), block, 0, h->stream, a);
+  if (which & 2) hipLaunchKernelGGL((ela::k_stress<T, BX, MODE, FS>), dim3(t.nblocks), block, 0, h->stream, a);
   return ELA_OK;
 }
 
-template <typename T, int MODE, bool SS, bool ACCV, bool ACCS>
+template <typename T, int MODE, int FV, int FS>
 int launch_step(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
   switch (h->by_x) {
-    case 4: return launch_step_bx<T, 4, MODE, SS, ACCV, ACCS>(h, a, which);
-    case 8: return launch_step_bx<T, 8, MODE, SS, ACCV, ACCS>(h, a, which);
-    default: return launch_step_bx<T, 16, MODE, SS, ACCV, ACCS>(h, a, which);
+    case 4: return launch_step_bx<T, 4, MODE, FV, FS>(h, a, which);
+    case 8: return launch_step_bx<T, 8, MODE, FV, FS>(h, a, which);
+    default: return launch_step_bx<T, 16, MODE, FV, FS>(h, a, which);
   }
 }
 
@@ -223,9 +239,9 @@ int fill(ela_handle* h, void* p, int64_t n) {
   return ELA_OK;
 }
 
-// 3-D array [k][ny][nx][nz] (caller) <-> library layout, k = 0..n_arrays-1.
-int copy_in_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem) {
-  const size_t es = h->esize;
+// 3-D array [k][ny][nx][nz] (caller) <-> library layout, k = 0..n_arrays-1,
+// elements of es bytes.
+int copy_in_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem, size_t es) {
   const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   for (int k = 0; k < n_arrays; ++k)
     HIPCHK(hipMemcpy2DAsync((char*)dst + (k * h->stride + h->zoff) * es, h->nzi * es,
@@ -234,14 +250,21 @@ int copy_in_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem)
   return ELA_OK;
 }
 
-int copy_out_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem) {
-  const size_t es = h->esize;
+int copy_in_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem) {
+  return copy_in_3d(h, dst, src, n_arrays, mem, (size_t)h->esize);
+}
+
+int copy_out_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem, size_t es) {
   const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   for (int k = 0; k < n_arrays; ++k)
     HIPCHK(hipMemcpy2DAsync((char*)dst + k * h->n_model * es, h->cfg.nz * es,
                             (const char*)src + (k * h->stride + h->zoff) * es, h->nzi * es, h->cfg.nz * es,
                             h->cfg.nx * h->cfg.ny, kind, h->stream));
   return ELA_OK;
+}
+
+int copy_out_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem) {
+  return copy_out_3d(h, dst, src, n_arrays, mem, (size_t)h->esize);
 }
 
 // ---- point lists ------------------------------------------------------------
@@ -318,12 +341,12 @@ ela::PointArgs<T> make_points(ela_handle* h, const DevPoints& pts, T* const fiel
   return a;
 }
 
-template <typename T, bool ACC>
+template <typename T, bool ACC, bool GRAD = false>
 void launch_inject(ela_handle* h, ela::PointArgs<T> a, int its, double fa, double fb) {
   if (a.off[9] == 0) return;
   a.its = its; a.fa = T(fa); a.fb = T(fb);
   const unsigned nb = (unsigned)((a.off[9] + 255) / 256);
-  hipLaunchKernelGGL((ela::k_inject<T, ACC>), dim3(nb), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL((ela::k_inject<T, ACC, GRAD>), dim3(nb), dim3(256), 0, h->stream, a);
 }
 
 template <typename T>
@@ -466,7 +489,7 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
     for (int64_t its = 0; its < nts - 1; ++its)
       for (int64_t it2 = 1; it2 <= sub; ++it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-        CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a)));
+        CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a)));
         launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), (int)its, f[it2 - 1],
                                 f[hh + it2 - 1]);
         launch_record<T>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2], f[hh + it2]);
@@ -476,7 +499,7 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
     for (int64_t its = nts - 2; its >= 0; --its)
       for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-        CHK((launch_step<T, ela::MODE_ADJ, false, false, false>(h, a)));
+        CHK((launch_step<T, ela::MODE_ADJ, 0, 0>(h, a)));
         launch_inject<T, false>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2 + 1],
                                 f[hh + it2 + 1]);
         launch_record<T>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), (int)its, f[it2], f[hh + it2]);
@@ -533,7 +556,7 @@ int bg_coarse_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_
     ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
     for (int c = 0; c < 3; ++c) { a.accl[c] = VL + c * np; a.accr[c] = VR + c * np; }
     a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-    CHK((launch_step<T, ela::MODE_FWD, false, true, false>(h, a)));
+    CHK((launch_step<T, ela::MODE_FWD, ela::F_ACC, 0>(h, a)));
     ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
     for (int c = 0; c < 3; ++c) { S.accl[c] = a.accl[c]; S.accr[c] = a.accr[c]; }
     S.accfl = a.accfl; S.accfr = a.accfr;
@@ -657,7 +680,7 @@ int born_forward_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
       ela::StepArgs<T> a = base_args<T>(h, sc.p0, sc.p1);
       for (int k = 0; k < 9; ++k) { a.ssl[k] = ssL[k]; a.ssr[k] = ssR[k]; }
       a.ssfl = T(f[it2 - 1]); a.ssfr = T(f[hh + it2 - 1]);
-      CHK((launch_step<T, ela::MODE_FWD, true, false, false>(h, a)));
+      CHK((launch_step<T, ela::MODE_FWD, ela::F_SS, ela::F_SS>(h, a)));
       launch_record<T>(h, make_points<T>(h, rp, sc.p0, TmViews<T>(tm).v, true, nrec_tot), (int)its, f[it2],
                        f[hh + it2]);
       sc.swap();
@@ -741,7 +764,7 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
       ela::StepArgs<T> a = base_args<T>(h, ad.p0, ad.p1);
       for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
       a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-      CHK((launch_step<T, ela::MODE_BORN_ADJ, false, true, true>(h, a)));
+      CHK((launch_step<T, ela::MODE_BORN_ADJ, ela::F_ACC, ela::F_ACC>(h, a)));
       ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm).v, true, nrec_tot);
       for (int k = 0; k < 9; ++k) { R.accl[k] = ssL[k]; R.accr[k] = ssR[k]; }
       R.accfl = a.accfl; R.accfr = a.accfr;
@@ -838,19 +861,338 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
   return ELA_OK;
 }
 
+// ---- adjoint-state gradient --------------------------------------------------
+//
+// Gradient of a data functional J with respect to the 8 scaled medium arrays
+// for the nonlinear forward operator: the transpose of the linearised fused
+// step  n = W (W o + M D c + s),
+//   dJ/dM = sum_j mu_j (.) D(c_j),
+// mu_j being the transposed recursion (the Born-adjoint stepper driven by
+// dJ/d(data) injected at the receivers). The forward pass stores checkpoints
+// of its state every K1 fine steps (bfloat16 or native precision). The
+// backward pass walks the checkpoint segments from last to first: it
+// recomputes each inner segment of K steps from its checkpoint while writing
+// the step's derivative combinations D(c_j) to HBM (F_STORE), then runs the
+// adjoint steps of the segment backwards with the imaging fused into the
+// stepper (F_GRAD) and the injection (GRAD), accumulating FP32/native partial
+// sums that are added into the FP64 gradient once per segment. With K1 > K
+// the outer segment is first recomputed to store native inner checkpoints
+// every K steps (two levels). The derivative of J with respect to the scaled
+// source traces is extracted at the source points (as the nonlinear adjoint
+// does) so that the caller can add the source-scaling term.
+
+struct GradPlan {
+  int64_t K = 20, K1 = 20, n1 = 0, n2 = 1;
+  bool bf16 = true;
+  size_t ck1_bytes = 0, ck2_bytes = 0;  // one checkpoint
+};
+
+int64_t interior_count(const ela_handle* h) { return interior_cells(h); }
+
+int grad_plan(ela_handle* h, GradPlan& p) {
+  const int64_t N = (h->cfg.nts - 1) * h->cfg.sub;
+  const int64_t nI = interior_count(h);
+  p.K = std::max<int64_t>(1, std::min<int64_t>(h->grad_interval, N));
+  p.bf16 = h->grad_bf16 != 0;
+  p.ck1_bytes = (size_t)18 * nI * (p.bf16 ? 2 : h->esize);
+  p.ck2_bytes = (size_t)18 * nI * h->esize;
+  const double arr = double(h->stride) * h->esize;
+  const double fixed = arr * (9.0 * p.K + 8 + 18) + double(h->stride) * 8.0 * 8;  // drv, g32, wave2, g64
+  auto mem_for = [&](int64_t K1) {
+    const int64_t n1 = (N + K1 - 1) / K1;
+    return fixed + double(n1) * p.ck1_bytes + (K1 > p.K ? double(K1 / p.K) * p.ck2_bytes : 0.0);
+  };
+  if (h->grad_outer > 0) {
+    if (h->grad_outer % p.K) return fail(ELA_ERR_ARG, "outer checkpoint interval must be a multiple of the inner one");
+    p.K1 = h->grad_outer;
+  } else {
+    double budget;
+    if (h->grad_budget > 0) {
+      budget = double(h->grad_budget);
+    } else {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      budget = double(fr) + double(h->ck1.bytes + h->ck2.bytes + h->drv.bytes + h->g32.bytes + h->g64.bytes +
+                                   h->wave2.bytes + h->coarse.bytes + h->ckpt.bytes + h->ss.bytes) -
+               double(2ull << 30);
+    }
+    p.K1 = 0;
+    if (mem_for(p.K) <= budget) {
+      p.K1 = p.K;
+    } else {
+      double best = 0;
+      for (int64_t m = 2; m * p.K <= N + p.K; ++m) {
+        const double need = mem_for(m * p.K);
+        if (need <= budget && (p.K1 == 0 || need < best)) { p.K1 = m * p.K; best = need; }
+      }
+    }
+    if (p.K1 == 0)
+      return fail(ELA_ERR_MEM, "gradient: %.1f GB of device memory cannot hold the checkpoints", budget / 1e9);
+  }
+  p.n1 = (N + p.K1 - 1) / p.K1;
+  p.n2 = p.K1 / p.K;
+  return ELA_OK;
+}
+
+template <typename T>
+ela::CkArgs<T> ck_args(ela_handle* h, const Wave<T>& w) {
+  ela::CkArgs<T> a;
+  for (int k = 0; k < 9; ++k) { a.f[k] = w.p0[k]; a.f[9 + k] = w.p1[k]; }
+  a.nzI = (int)(h->cfg.nz - 2 * ela::FAT);
+  a.nxI = (int)(h->cfg.nx - 2 * ela::FAT);
+  a.nyI = (int)(h->cfg.ny - 2 * ela::FAT);
+  a.nzi = (int)h->nzi;
+  a.zoff = (int)h->zoff;
+  a.ys = h->cfg.nx * h->nzi;
+  a.nI = interior_count(h);
+  return a;
+}
+
+dim3 ck_grid(int64_t nI) { return dim3((unsigned)std::min<int64_t>((nI + 255) / 256, 4096), 18); }
+
+template <typename T>
+int save_checkpoint(ela_handle* h, const Wave<T>& w, void* out, bool bf16, float* scale) {
+  ela::CkArgs<T> a = ck_args<T>(h, w);
+  const dim3 grid = ck_grid(a.nI);
+  uint32_t* amax = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(h->ckscale.p) + h->ckscale.bytes - 18 * 4);
+  if (bf16) {
+    hipLaunchKernelGGL((ela::k_fill<uint32_t>), dim3(1), dim3(64), 0, h->stream, amax, (int64_t)18, 0u);
+    hipLaunchKernelGGL((ela::k_ck_amax<T>), grid, dim3(256), 0, h->stream, a, amax);
+    hipLaunchKernelGGL((ela::k_ck_pack<T, true>), grid, dim3(256), 0, h->stream, a, out, amax, scale);
+  } else {
+    hipLaunchKernelGGL((ela::k_ck_pack<T, false>), grid, dim3(256), 0, h->stream, a, out, amax, scale);
+  }
+  return ELA_OK;
+}
+
+template <typename T>
+int load_checkpoint(ela_handle* h, const Wave<T>& w, const void* in, bool bf16, const float* scale) {
+  ela::CkArgs<T> a = ck_args<T>(h, w);
+  if (bf16) hipLaunchKernelGGL((ela::k_ck_unpack<T, true>), ck_grid(a.nI), dim3(256), 0, h->stream, a, in, scale);
+  else hipLaunchKernelGGL((ela::k_ck_unpack<T, false>), ck_grid(a.nI), dim3(256), 0, h->stream, a, in, scale);
+  return ELA_OK;
+}
+
+// One forward fine step k (source injection, optional store of D(c_k)).
+template <typename T, bool STORE>
+int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k, T* dst) {
+  const int64_t sub = h->cfg.sub, hh = sub + 1, np = h->stride;
+  const double* f = h->filt.data();
+  const int its = (int)(k / sub), i2 = (int)(k % sub);  // forward it2 = i2 + 1
+  ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+  if (STORE) {
+    for (int c = 0; c < 9; ++c) a.dst[c] = dst + c * np;
+    CHK((launch_step<T, ela::MODE_FWD, ela::F_STORE, ela::F_STORE>(h, a)));
+  } else {
+    CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a)));
+  }
+  launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), its, f[i2], f[hh + i2]);
+  w.swap();
+  return ELA_OK;
+}
+
+template <typename T>
+int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, const ela_points* rp_in,
+                 const ela_traces* rt_in, int mem) {
+  if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
+  CHK(check_points(sp_in, "source"));
+  CHK(check_points(rp_in, "receiver"));
+  h->grad_ready = false;
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, N = (nts - 1) * sub;
+  GradPlan plan;
+  CHK(grad_plan(h, plan));
+  DevPoints sp, rp;
+  CHK(convert_points(h, sp_in, rp_in, mem, sp, rp));
+  void* ds[9];
+  void* dr[9];
+  CHK(stage_traces(h, sp_in, st_in, rp_in, rt_in, true, false, mem, ds, dr));
+  T* src_tr[9];
+  T* rec_tr[9];
+  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)ds[c]; rec_tr[c] = (T*)dr[c]; }
+  const int64_t nrec_tot = total_points(rp);
+  CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
+  T* tm = (T*)h->scratch_tm.p;
+
+  if (h->ck1.bytes != plan.n1 * plan.ck1_bytes) h->ck1.release();
+  CHK(h->ck1.alloc(plan.n1 * plan.ck1_bytes));
+  CHK(h->ckscale.alloc((size_t)(plan.n1 + 1) * 18 * 4));
+  Wave<T> w = wave_ptrs<T>(h->wave, h->stride);
+  CHK(fill<T>(h, h->wave.p, 18 * h->stride));
+  CHK(fill<T>(h, tm, nrec_tot * nts));
+  const double* f = h->filt.data();
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int64_t k = 0; k < N; ++k) {
+    if (k % plan.K1 == 0) {
+      const int64_t s1 = k / plan.K1;
+      CHK(save_checkpoint<T>(h, w, (char*)h->ck1.p + s1 * plan.ck1_bytes, plan.bf16,
+                             (float*)h->ckscale.p + s1 * 18));
+    }
+    const int its = (int)(k / sub), i2 = (int)(k % sub);
+    ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
+    CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a)));
+    launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), its, f[i2], f[hh + i2]);
+    launch_record<T>(h, make_points<T>(h, rp, w.p0, TmViews<T>(tm).v, true, nrec_tot), its, f[i2 + 1],
+                     f[hh + i2 + 1]);
+    w.swap();
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  h->last_steps = N;
+  int64_t off = 0;
+  for (int c = 0; c < 9; ++c) {
+    const int64_t n = rp.n[kCompGrid[c]];
+    launch_transpose<T>(h, tm + off, rec_tr[c], nts, n, nrec_tot, nts);
+    off += n;
+  }
+  HIPCHK(hipGetLastError());
+  if (mem == ELA_MEM_HOST) {
+    CHK(stage_out_traces(h, rp_in, dr, rt_in));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  h->plan_K = plan.K; h->plan_K1 = plan.K1; h->plan_n1 = plan.n1; h->plan_bf16 = plan.bf16;
+  h->grad_ready = true;
+  return ELA_OK;
+}
+
+template <typename T>
+int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, const ela_points* rp_in,
+                  const ela_traces* at_in, double* grad8, ela_traces* gs_out, int mem) {
+  if (!h->grad_ready) return fail(ELA_ERR_STATE, "no gradient forward has stored checkpoints for this medium");
+  CHK(check_points(sp_in, "source"));
+  CHK(check_points(rp_in, "receiver"));
+  const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, N = (nts - 1) * sub, np = h->stride;
+  GradPlan plan;
+  plan.K = h->plan_K; plan.K1 = h->plan_K1; plan.n1 = h->plan_n1; plan.n2 = plan.K1 / plan.K;
+  plan.bf16 = h->plan_bf16 != 0;
+  const int64_t nI = interior_count(h);
+  plan.ck1_bytes = (size_t)18 * nI * (plan.bf16 ? 2 : h->esize);
+  plan.ck2_bytes = (size_t)18 * nI * h->esize;
+
+  DevPoints sp, rp;
+  CHK(convert_points(h, sp_in, rp_in, mem, sp, rp));
+  void* ds[9];
+  void* dr[9];
+  CHK(stage_traces(h, sp_in, st_in, rp_in, at_in, true, true, mem, ds, dr));
+  T* src_tr[9];
+  T* adj_tr[9];
+  for (int c = 0; c < 9; ++c) { src_tr[c] = (T*)ds[c]; adj_tr[c] = (T*)dr[c]; }
+  // source-trace gradient outputs: the caller's arrays or host-call staging
+  T* gs_tr[9];
+  if (mem == ELA_MEM_DEVICE) {
+    for (int c = 0; c < 9; ++c) gs_tr[c] = (T*)gs_out->tr[c];
+  } else {
+    CHK(h->stage2.alloc(trace_elems(sp_in, nts) * h->esize + 9 * 256));
+    size_t off = 0;
+    for (int c = 0; c < 9; ++c) {
+      gs_tr[c] = (T*)((char*)h->stage2.p + off);
+      off += round_up(sp_in->n[kCompGrid[c]] * nts * h->esize, 256);
+    }
+  }
+  const int64_t nrec_tot = total_points(rp), nsrc_tot = total_points(sp);
+  CHK(h->scratch_tm.alloc(std::max<int64_t>(1, (nrec_tot + nsrc_tot) * nts) * sizeof(T)));
+  T* tm_adj = (T*)h->scratch_tm.p;
+  T* tm_src = tm_adj + nrec_tot * nts;
+  int64_t off = 0;
+  for (int c = 0; c < 9; ++c) {
+    const int64_t n = rp.n[kCompGrid[c]];
+    launch_transpose<T>(h, adj_tr[c], tm_adj + off, n, nts, nts, nrec_tot);
+    off += n;
+  }
+  CHK(fill<T>(h, tm_src, nsrc_tot * nts));
+
+  if (plan.n2 > 1) {
+    if (h->ck2.bytes != plan.n2 * plan.ck2_bytes) h->ck2.release();
+    CHK(h->ck2.alloc(plan.n2 * plan.ck2_bytes));
+  }
+  CHK(h->drv.alloc((size_t)plan.K * 9 * np * sizeof(T)));
+  CHK(h->g32.alloc((size_t)8 * np * sizeof(T)));
+  CHK(h->g64.alloc((size_t)8 * np * sizeof(double)));
+  CHK(h->wave2.alloc((size_t)18 * np * sizeof(T)));
+  T* drv = (T*)h->drv.p;
+  T* g32 = (T*)h->g32.p;
+  double* g64 = (double*)h->g64.p;
+  CHK(fill<T>(h, g32, 8 * np));
+  CHK(fill<double>(h, g64, 8 * np));
+  CHK(copy_in_3d(h, g64, grad8, 8, mem, sizeof(double)));
+  CHK(fill<T>(h, h->wave2.p, 18 * np));
+  Wave<T> bg = wave_ptrs<T>(h->wave, np), ad = wave_ptrs<T>(h->wave2, np);
+  const double* f = h->filt.data();
+  const float* scales = (const float*)h->ckscale.p;
+  int64_t steps = 0;
+
+  auto adj_step = [&](int64_t k, const T* d) -> int {
+    const int its = (int)(k / sub), i2 = (int)(k % sub);
+    ela::StepArgs<T> a = base_args<T>(h, ad.p0, ad.p1);
+    for (int c = 0; c < 9; ++c) a.drv[c] = d + c * np;
+    for (int m = 0; m < 8; ++m) a.grad[m] = g32 + m * np;
+    CHK((launch_step<T, ela::MODE_BORN_ADJ, ela::F_GRAD, ela::F_GRAD>(h, a)));
+    ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm_adj).v, true, nrec_tot);
+    for (int c = 0; c < 9; ++c) R.drv[c] = a.drv[c];
+    for (int m = 0; m < 8; ++m) R.grad[m] = a.grad[m];
+    launch_inject<T, false, true>(h, R, its, f[i2 + 1], f[hh + i2 + 1]);
+    launch_record<T>(h, make_points<T>(h, sp, ad.p0, TmViews<T>(tm_src).v, true, nsrc_tot), its, f[i2], f[hh + i2]);
+    ad.swap();
+    return ELA_OK;
+  };
+
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int64_t s1 = plan.n1 - 1; s1 >= 0; --s1) {
+    const int64_t a0 = s1 * plan.K1, a1 = std::min(a0 + plan.K1, N);
+    const int64_t nseg = (a1 - a0 + plan.K - 1) / plan.K;
+    const void* c1 = (const char*)h->ck1.p + s1 * plan.ck1_bytes;
+    if (nseg > 1) {
+      CHK(load_checkpoint<T>(h, bg, c1, plan.bf16, scales + s1 * 18));
+      for (int64_t k = a0; k < a1; ++k) {
+        if ((k - a0) % plan.K == 0 && (k - a0) / plan.K < nseg)
+          CHK(save_checkpoint<T>(h, bg, (char*)h->ck2.p + ((k - a0) / plan.K) * plan.ck2_bytes, false, nullptr));
+        CHK((grad_fwd_step<T, false>(h, bg, sp, src_tr, k, nullptr)));
+      }
+      steps += a1 - a0;
+    }
+    for (int64_t s2 = nseg - 1; s2 >= 0; --s2) {
+      const int64_t j0 = a0 + s2 * plan.K, j1 = std::min(j0 + plan.K, a1);
+      if (nseg > 1) CHK(load_checkpoint<T>(h, bg, (const char*)h->ck2.p + s2 * plan.ck2_bytes, false, nullptr));
+      else CHK(load_checkpoint<T>(h, bg, c1, plan.bf16, scales + s1 * 18));
+      for (int64_t k = j0; k < j1; ++k) CHK((grad_fwd_step<T, true>(h, bg, sp, src_tr, k, drv + (k - j0) * 9 * np)));
+      for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(k, drv + (k - j0) * 9 * np));
+      steps += 2 * (j1 - j0);
+      const int64_t n8 = 8 * np;
+      hipLaunchKernelGGL((ela::k_fold<T>), dim3((unsigned)std::min<int64_t>((n8 + 255) / 256, 65536)), dim3(256), 0,
+                         h->stream, g32, g64, n8);
+    }
+  }
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  h->last_steps = steps;
+  CHK(copy_out_3d(h, grad8, g64, 8, mem, sizeof(double)));
+  off = 0;
+  for (int c = 0; c < 9; ++c) {
+    const int64_t n = sp.n[kCompGrid[c]];
+    launch_transpose<T>(h, tm_src + off, gs_tr[c], nts, n, nsrc_tot, nts);
+    off += n;
+  }
+  HIPCHK(hipGetLastError());
+  if (mem == ELA_MEM_HOST) {
+    for (int c = 0; c < 9; ++c) {
+      const int64_t n = sp_in->n[kCompGrid[c]] * nts;
+      if (n) HIPCHK(hipMemcpyAsync(gs_out->tr[c], gs_tr[c], n * h->esize, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  return ELA_OK;
+}
+
 template <typename T>
 int time_kernels(ela_handle* h, int reps, double* ms_v, double* ms_s) {
   Wave<T> w = wave_ptrs<T>(h->wave, h->stride);
   ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
   float tv = 0.f, ts = 0.f;
-  CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a, 3)));  // warm caches and code
+  CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a, 3)));  // warm caches and code
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a, 1)));
+  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a, 1)));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   HIPCHK(hipEventSynchronize(h->ev1));
   HIPCHK(hipEventElapsedTime(&tv, h->ev0, h->ev1));
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, false, false, false>(h, a, 2)));
+  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a, 2)));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   HIPCHK(hipEventSynchronize(h->ev1));
   HIPCHK(hipEventElapsedTime(&ts, h->ev0, h->ev1));
@@ -904,7 +1246,9 @@ int ela_create(const ela_config* cfg, ela_handle** out) {
   };
   if ((uint64_t)h->stride * (uint64_t)h->esize >= (1ull << 32))
     return bail(fail(ELA_ERR_ARG, "grid of %lld cells exceeds 4 GiB per array (32-bit offsets)", (long long)h->n_model));
-  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess)
+  // a blocking stream: it orders with HIP's null stream, on which callers
+  // (e.g. torch's default stream) produce inputs and consume outputs
+  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamDefault) != hipSuccess)
     return bail(fail(ELA_ERR_HIP, "stream create failed"));
   h->stream = h->own_stream;
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
@@ -974,6 +1318,7 @@ int ela_set_medium(ela_handle* h, const void* med8, int mem) {
   CHK(copy_in_3d(h, h->med.p, med8, 8, mem));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->have_medium = true;
+  h->grad_ready = false;
   return ELA_OK;
 }
 
@@ -1005,6 +1350,46 @@ int ela_born_adjoint(ela_handle* h, const ela_points* sp, const ela_traces* st, 
   DeviceGuard dg(h->cfg.device);
   return h->esize == 4 ? born_run<float>(h, sp, st, nullptr, dmod8, rp, rt, true, mem)
                        : born_run<double>(h, sp, st, nullptr, dmod8, rp, rt, true, mem);
+}
+
+int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_interval, int bf16,
+                                 int64_t budget_bytes) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  if (interval < 1) return fail(ELA_ERR_ARG, "checkpoint interval must be >= 1");
+  if (outer_interval < 0 || (outer_interval > 0 && outer_interval % interval))
+    return fail(ELA_ERR_ARG, "outer interval must be 0 (automatic) or a multiple of the interval");
+  if (budget_bytes < 0) return fail(ELA_ERR_ARG, "negative memory budget");
+  h->grad_interval = interval;
+  h->grad_outer = outer_interval;
+  h->grad_bf16 = bf16 ? 1 : 0;
+  h->grad_budget = budget_bytes;
+  h->grad_ready = false;
+  return ELA_OK;
+}
+
+int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* bf16,
+                           int64_t* checkpoint_bytes) {
+  if (!h || !interval || !outer_interval || !bf16 || !checkpoint_bytes) return fail(ELA_ERR_ARG, "null argument");
+  *interval = h->plan_K;
+  *outer_interval = h->plan_K1;
+  *bf16 = h->plan_bf16;
+  *checkpoint_bytes = (int64_t)h->ck1.bytes;
+  return ELA_OK;
+}
+
+int ela_gradient_forward(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp,
+                         ela_traces* rt, int mem) {
+  if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
+  DeviceGuard dg(h->cfg.device);
+  return h->esize == 4 ? grad_forward<float>(h, sp, st, rp, rt, mem) : grad_forward<double>(h, sp, st, rp, rt, mem);
+}
+
+int ela_gradient_backward(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp,
+                          const ela_traces* adj, double* grad8, ela_traces* src_grad, int mem) {
+  if (!h || !st || !adj || !grad8 || !src_grad) return fail(ELA_ERR_ARG, "null argument");
+  DeviceGuard dg(h->cfg.device);
+  return h->esize == 4 ? grad_backward<float>(h, sp, st, rp, adj, grad8, src_grad, mem)
+                       : grad_backward<double>(h, sp, st, rp, adj, grad8, src_grad, mem);
 }
 
 int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {

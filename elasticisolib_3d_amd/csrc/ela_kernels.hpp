@@ -43,6 +43,13 @@ enum { MODE_FWD = 0,       // n = w(w o + M D c)        stepFwdGpu_3D
        MODE_BORN_ADJ = 2   // n = w(w o - D (M c))      stepAdjGpuBorn_3D
 };
 
+// Optional work fused into a stepper launch (bit flags).
+enum { F_SS = 1,     // add the secondary sources ssl/ssr (Born forward)
+       F_ACC = 2,    // accumulate the new level into accl/accr (coarse-time interpolation)
+       F_STORE = 4,  // write the step's derivative combinations to dst (gradient recomputation)
+       F_GRAD = 8    // gradient imaging: grad[m] += new level x stored derivatives drv
+};
+
 template <typename T>
 struct StepArgs {
   T* o[9];              // level n-1 in, level n+1 out (in place)
@@ -52,6 +59,9 @@ struct StepArgs {
   const T* ssr[9];
   T* accl[9];           // coarse-time accumulation targets
   T* accr[9];
+  T* dst[9];            // F_STORE: sx sy sz exx eyy ezz gxz gxy gyz of this step
+  const T* drv[9];      // F_GRAD: the same combinations stored by the matching forward step
+  T* grad[8];           // F_GRAD: gradient accumulators, one per medium array
   T ssfl, ssfr;         // filter weights for the secondary source
   T accfl, accfr;       // filter weights for the accumulation
   T cx[4], cy[4], cz[4];
@@ -270,8 +280,9 @@ struct VelStage {
   T nxy, nyy, nyz;              // new queue values (plane + 4, + 3, + 4)
 };
 
-template <typename T, int BX, int MODE, bool SS, bool ACC>
+template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
+  constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0;
   __shared__ Plane<T, BX> l_xx[2], l_zz[2], l_xy[2], l_xz[2], l_yz[2];
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
   const uint32_t ys = (uint32_t)a.ys;
@@ -339,6 +350,10 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
       const T sz = dpx<T, BX>(l_xz[b], r, q, cx) + dq(qyz, cy) + dmz<T, BX>(l_zz[b], r, q, cz);
       const T w = damp_w(a, t.izo, t.ix, iy);
       const T s3[3] = {sx, sy, sz};
+      if (F & F_STORE) {
+#pragma unroll
+        for (int f = 0; f < 3; ++f) st_nt(a.dst[f], i, s3[f]);
+      }
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
         T u;
@@ -352,6 +367,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
           *at(a.accl[f], i) += u * a.accfl;
           *at(a.accr[f], i) += u * a.accfr;
         }
+        if (F & F_GRAD) *at(a.grad[f], i) += u * ld_nt(a.drv[f], i);  // BX_/BY_/BZ_ == VX/VY/VZ
       }
     }
 #pragma unroll
@@ -374,8 +390,9 @@ struct StressStage {
   T nx, ny, nz;              // new queue values (vx: plane+3, vy: +4, vz: +3)
 };
 
-template <typename T, int BX, int MODE, bool SS, bool ACC>
+template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
+  constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0;
   __shared__ Plane<T, BX> l_vx[2], l_vy[2], l_vz[2];
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
   const uint32_t ys = (uint32_t)a.ys;
@@ -444,6 +461,10 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       const T gxz = dmz<T, BX>(l_vx[b], r, q, cz) + dmx<T, BX>(l_vz[b], r, q, cx);
       const T gyz = dq(qz, cy) + dmz<T, BX>(l_vy[b], r, q, cz);
       const T w = damp_w(a, t.izo, t.ix, iy);
+      if (F & F_STORE) {
+        st_nt(a.dst[SXX], i, exx); st_nt(a.dst[SYY], i, eyy); st_nt(a.dst[SZZ], i, ezz);
+        st_nt(a.dst[SXZ], i, gxz); st_nt(a.dst[SXY], i, gxy); st_nt(a.dst[SYZ], i, gyz);
+      }
       T u[6];
       if (MODE == MODE_BORN_ADJ) {
         u[0] = w * o6[0] - exx;
@@ -467,11 +488,23 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
         T v = u[k];
         if (SS) v += *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr;
         v *= w;
+        u[k] = v;
         st_nt(a.o[f], i, v);
         if (ACC) {
           *at(a.accl[f], i) += v * a.accfl;
           *at(a.accr[f], i) += v * a.accfr;
         }
+      }
+      if (F & F_GRAD) {
+        // lamb2MuDtw multiplies exx, eyy, ezz in sxx, syy, szz; lambDtw the
+        // other two normal strains; the shear moduli one shear strain each
+        const T dxx = ld_nt(a.drv[SXX], i), dyy = ld_nt(a.drv[SYY], i), dzz = ld_nt(a.drv[SZZ], i);
+        const T l2 = u[0] * dxx + u[1] * dyy + u[2] * dzz;
+        *at(a.grad[L2M], i) += l2;
+        *at(a.grad[LAM], i) += (u[0] + u[1] + u[2]) * (dxx + dyy + dzz) - l2;
+        *at(a.grad[MXZ], i) += u[3] * ld_nt(a.drv[SXZ], i);
+        *at(a.grad[MXY], i) += u[4] * ld_nt(a.drv[SXY], i);
+        *at(a.grad[MYZ], i) += u[5] * ld_nt(a.drv[SYZ], i);
       }
     }
 #pragma unroll
@@ -503,6 +536,9 @@ struct PointArgs {
   T* accl[9];
   T* accr[9];
   T accfl, accfr;
+  // optional gradient imaging of the injected value (adjoint-state gradient)
+  const T* drv[9];
+  T* grad[8];
   // damping (injection only)
   const T* taper;
   int min_pad, nz, nx, ny, nzi, zoff;
@@ -531,7 +567,7 @@ __device__ __forceinline__ T point_taper(const PointArgs<T>& a, int32_t p) {
 // field[pos] += w(pos) * (tr[its] fa + tr[its+1] fb)
 // (ker_inject_source_* / ker_inject_data_*, kernelsElaGpu_3D.cu:11-52, with the
 // sponge factor that the reference applies right after injection).
-template <typename T, bool ACC>
+template <typename T, bool ACC, bool GRAD = false>
 __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.off[9]) return;
@@ -546,6 +582,20 @@ __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   if (ACC && a.accl[c]) {
     a.accl[c][p] += v * a.accfl;
     a.accr[c][p] += v * a.accfr;
+  }
+  if (GRAD) {
+    // the stepper imaged the level before this injection; add the injected
+    // part (the three normal stresses share lamb2MuDtw / lambDtw: atomics)
+    if (c < 3) {
+      atomicAdd(&a.grad[c][p], v * a.drv[c][p]);
+    } else if (c < 6) {
+      const T dxx = a.drv[SXX][p], dyy = a.drv[SYY][p], dzz = a.drv[SZZ][p];
+      const T own = c == SXX ? dxx : (c == SYY ? dyy : dzz);
+      atomicAdd(&a.grad[L2M][p], v * own);
+      atomicAdd(&a.grad[LAM][p], v * (dxx + dyy + dzz - own));
+    } else {
+      atomicAdd(&a.grad[MXZ + (c - SXZ)][p], v * a.drv[c][p]);
+    }
   }
 }
 
@@ -713,6 +763,101 @@ __global__ void __launch_bounds__(256) k_fill(T* p, int64_t n, T v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = i; k < n; k += stride) p[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoints of the forward state for the adjoint-state gradient: the 18
+// arrays (two time levels x 9 fields) packed to the FD interior, either as
+// they are or in bfloat16 scaled per array and per snapshot by a power of two
+// 2^e >= max |x| (the scaling is exact; bfloat16 keeps 8 significant bits,
+// rounded to nearest even).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct CkArgs {
+  T* f[18];
+  int nzI, nxI, nyI;     // interior extents (n - 2 FAT)
+  int nzi, zoff;         // internal row pitch and z origin
+  int64_t ys;            // nx * nzi
+  int64_t nI;            // nzI * nxI * nyI
+};
+
+template <typename T>
+__device__ __forceinline__ int64_t ck_offset(const CkArgs<T>& a, int64_t q) {
+  const int64_t iz = q % a.nzI;
+  const int64_t r = q / a.nzI;
+  const int64_t ix = r % a.nxI, iy = r / a.nxI;
+  return (iy + FAT) * a.ys + (ix + FAT) * a.nzi + a.zoff + FAT + iz;
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Power-of-two scale >= amax (1 for an all-zero array).
+__device__ __forceinline__ float pow2_scale(uint32_t amax_bits) {
+  const float m = __uint_as_float(amax_bits);
+  if (!(m > 0.f)) return 1.f;
+  int e;
+  const float fr = frexpf(m, &e);  // m = fr * 2^e, fr in [0.5, 1)
+  (void)fr;
+  return ldexpf(1.f, e);
+}
+
+// amax[f] = max |x| over the interior of array f (blockIdx.y = f), as the bit
+// pattern of a non-negative float (ordered like unsigned integers).
+template <typename T>
+__global__ void __launch_bounds__(256) k_ck_amax(CkArgs<T> a, uint32_t* amax) {
+  __shared__ float red[256];
+  const int f = blockIdx.y;
+  float m = 0.f;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.nI; q += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf((float)a.f[f][ck_offset(a, q)]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(&amax[f], __float_as_uint(red[0]));
+}
+
+// Pack: out[f][q] = x / scale_f as bfloat16 (BF16 = true) or x (BF16 = false).
+template <typename T, bool BF16>
+__global__ void __launch_bounds__(256) k_ck_pack(CkArgs<T> a, void* out, const uint32_t* amax, float* scale) {
+  const int f = blockIdx.y;
+  const float s = BF16 ? pow2_scale(amax[f]) : 1.f;
+  const float inv = 1.f / s;   // exact: s is a power of two
+  if (BF16 && blockIdx.x == 0 && threadIdx.x == 0) scale[f] = s;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.nI; q += (int64_t)gridDim.x * blockDim.x) {
+    const T x = a.f[f][ck_offset(a, q)];
+    if (BF16) static_cast<uint16_t*>(out)[f * a.nI + q] = f32_to_bf16((float)x * inv);
+    else static_cast<T*>(out)[f * a.nI + q] = x;
+  }
+}
+
+template <typename T, bool BF16>
+__global__ void __launch_bounds__(256) k_ck_unpack(CkArgs<T> a, const void* in, const float* scale) {
+  const int f = blockIdx.y;
+  const float s = BF16 ? scale[f] : 1.f;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.nI; q += (int64_t)gridDim.x * blockDim.x) {
+    T x;
+    if (BF16) x = (T)(bf16_to_f32(static_cast<const uint16_t*>(in)[f * a.nI + q]) * s);
+    else x = static_cast<const T*>(in)[f * a.nI + q];
+    a.f[f][ck_offset(a, q)] = x;
+  }
+}
+
+// acc[i] += (double) part[i]; part[i] = 0 (FP64 time accumulation of the
+// gradient from the per-segment partial sums).
+template <typename T>
+__global__ void __launch_bounds__(256) k_fold(T* part, double* acc, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    acc[i] += (double)part[i];
+    part[i] = T(0);
+  }
 }
 
 }  // namespace ela

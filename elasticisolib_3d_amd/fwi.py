@@ -1,0 +1,222 @@
+"""Full-waveform-inversion misfit and its adjoint-state gradient, with a
+torch.autograd.Function front end.
+
+The misfit of a set of shots is J = 1/2 sum_shots ||S_r F(m; s) - d_obs||^2,
+F being the nonlinear forward operator of the propagator (regular-grid
+traces, medium-scaled sources), S_r the trilinear receiver interpolation and
+m = (rho, lambda, mu) on the padded grid. The gradient is computed shot by
+shot with the library's checkpointed adjoint propagation
+(ela_gradient_forward / ela_gradient_backward: FP32 stencil, bfloat16 or
+exact checkpoints, FP64 accumulation), then mapped to (rho, lambda, mu):
+
+    dJ/dM8   = adjoint-state imaging + source-scaling term (FP64)
+    dJ/dm    = medium_gradient(dJ/dM8)                      (FP64, host)
+
+The residual, the misfit reduction and the receiver interpolation run in FP64
+on the GPU (torch). The reference's inversion (python_float/efwiMain_3D.py)
+uses its Born adjoint as the gradient of the same misfit; the Born operator's
+coarse-time secondary sources make that an approximation of the exact
+gradient computed here.
+
+`ElasticFwiMisfit.apply(rho, lam, mu, problem)` returns J as a 0-d float64
+tensor; backward() returns the stored gradient scaled by the incoming
+gradient. Forward does all the propagation (the gradient is needed as soon
+as J is, and keeping every shot's checkpoints alive until backward would not
+fit in memory).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import propagator as P
+from .acquisition import COMP_GRID, PaddedAxes, ShotGeometry, scale_regular_traces, source_scaling_gradient
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class _DeviceInterp:
+    """Trilinear receiver interpolation of one grid on the GPU, FP64: each
+    device reads 8 regular points with fixed weights (gather); the adjoint
+    scatters with index_add."""
+
+    def __init__(self, interp, device):
+        torch = _torch()
+        self.nbr = torch.tensor(interp.nbr, dtype=torch.int64, device=device)       # [n_dev, 8]
+        self.w = torch.tensor(interp.weight, dtype=torch.float64, device=device)    # [n_dev, 8]
+        self.n_reg = interp.n_reg
+
+    def forward(self, reg):
+        """[n_reg, nt] -> [n_dev, nt]"""
+        return (reg.double()[self.nbr] * self.w[:, :, None]).sum(dim=1)
+
+    def adjoint(self, irr):
+        """[n_dev, nt] -> [n_reg, nt]"""
+        torch = _torch()
+        out = torch.zeros((self.n_reg, irr.shape[1]), dtype=torch.float64, device=irr.device)
+        out.index_add_(0, self.nbr.reshape(-1), (self.w[:, :, None] * irr[:, None, :]).reshape(-1, irr.shape[1]))
+        return out
+
+
+@dataclass
+class FwiShot:
+    """One shot: source and receiver geometries (7 grids each), the 9
+    irregular source signals [n_src, nts] and the observed irregular data
+    (9 arrays [n_rec, nts], filled by FwiProblem.synthesize)."""
+
+    src: ShotGeometry
+    rec: ShotGeometry
+    signals: List[np.ndarray]
+    observed: Optional[list] = None
+    _cache: dict = field(default_factory=dict, repr=False)
+
+
+class FwiProblem:
+    """Shots on one padded grid, propagated on one GPU."""
+
+    def __init__(self, axes: PaddedAxes, nts: int, dts: float, sub: int, alpha_cos: float, shots: Sequence[FwiShot],
+                 precision: str = "f32", interval: int = 20, outer_interval: int = 0, bf16: bool = True,
+                 device: int = 0):
+        self.axes = axes
+        self.nts, self.dts, self.sub = int(nts), float(dts), int(sub)
+        self.dtw = self.dts / self.sub
+        self.shots = list(shots)
+        self.precision = precision
+        self.dev = f"cuda:{device}"
+        self.prop = P.Propagator(axes.nz, axes.nx, axes.ny, nts, sub, axes.dz, axes.dx, axes.dy, dts, axes.min_pad,
+                                 alpha_cos, precision, device)
+        self.prop.set_gradient_checkpoints(interval, outer_interval, bf16)
+        self.dxyz = axes.dx * axes.dy * axes.dz
+        self.last_timing = {}
+
+    # -- helpers -----------------------------------------------------------
+    def _tdtype(self):
+        torch = _torch()
+        return torch.float32 if self.precision == "f32" else torch.float64
+
+    def _shot_device(self, shot: FwiShot):
+        """Positions and interpolation matrices of a shot on the GPU (cached)."""
+        c = shot._cache
+        if "spos" not in c:
+            torch = _torch()
+            c["spos"] = [torch.tensor(p, device=self.dev) for p in shot.src.positions()]
+            c["rpos"] = [torch.tensor(p, device=self.dev) for p in shot.rec.positions()]
+            c["Mr"] = [_DeviceInterp(g, self.dev) for g in shot.rec.grids]
+            c["raw_reg"] = [np.asarray(x, np.float64) for x in shot.src.to_regular(shot.signals)]
+        return c
+
+    def _scaled_sources(self, med8, shot):
+        torch = _torch()
+        c = self._shot_device(shot)
+        reg = scale_regular_traces(med8, shot.src, c["raw_reg"], self.dxyz)
+        return [torch.tensor(r, dtype=self._tdtype(), device=self.dev) for r in reg]
+
+    def _set_model(self, rho, lam, mu):
+        med8 = P.scale_medium(np.asarray(rho, np.float64), np.asarray(lam, np.float64), np.asarray(mu, np.float64),
+                              self.dtw)
+        torch = _torch()
+        self.prop.set_medium(torch.tensor(med8, dtype=self._tdtype(), device=self.dev))
+        return med8
+
+    def _to_irregular(self, shot, reg):
+        """Regular traces (device) -> 9 irregular FP64 arrays (device)."""
+        torch = _torch()
+        c = self._shot_device(shot)
+        return [c["Mr"][COMP_GRID[k]].forward(reg[k]) for k in range(9)]
+
+    def _to_regular(self, shot, irr):
+        c = self._shot_device(shot)
+        return [c["Mr"][COMP_GRID[k]].adjoint(irr[k]).to(self._tdtype()).contiguous() for k in range(9)]
+
+    # -- public ------------------------------------------------------------
+    def synthesize(self, rho, lam, mu) -> None:
+        """Observed data of every shot from a (true) model (FP64 device)."""
+        torch = _torch()
+        med8 = self._set_model(rho, lam, mu)
+        for shot in self.shots:
+            c = self._shot_device(shot)
+            reg = self.prop.nl_forward(c["spos"], self._scaled_sources(med8, shot), c["rpos"])
+            shot.observed = [x.clone() for x in self._to_irregular(shot, reg)]
+        torch.cuda.synchronize()
+
+    def misfit(self, rho, lam, mu) -> float:
+        """J only (plain forward propagations; FP64 residual reduction)."""
+        torch = _torch()
+        med8 = self._set_model(rho, lam, mu)
+        J = torch.zeros((), dtype=torch.float64, device=self.dev)
+        for shot in self.shots:
+            c = self._shot_device(shot)
+            reg = self.prop.nl_forward(c["spos"], self._scaled_sources(med8, shot), c["rpos"])
+            pred = self._to_irregular(shot, reg)
+            for k in range(9):
+                J += 0.5 * torch.sum((pred[k] - shot.observed[k]) ** 2)
+        return float(J)
+
+    def misfit_and_gradient(self, rho, lam, mu):
+        """(J, dJ/d(rho, lambda, mu) as a float64 [3, ny, nx, nz] array)."""
+        torch = _torch()
+        med8 = self._set_model(rho, lam, mu)
+        J = torch.zeros((), dtype=torch.float64, device=self.dev)
+        g8 = torch.zeros((8,) + tuple(med8.shape[1:]), dtype=torch.float64, device=self.dev)
+        g8_src = np.zeros(med8.shape, np.float64)
+        ms_f = ms_b = 0.0
+        for shot in self.shots:
+            c = self._shot_device(shot)
+            src = self._scaled_sources(med8, shot)
+            reg = self.prop.gradient_forward(c["spos"], src, c["rpos"])
+            ms_f += self.prop.last_timing()[0]
+            pred = self._to_irregular(shot, reg)
+            res = [pred[k] - shot.observed[k] for k in range(9)]
+            for k in range(9):
+                J += 0.5 * torch.sum(res[k] ** 2)
+            adj = self._to_regular(shot, res)
+            g8, gsrc = self.prop.gradient_backward(c["spos"], src, c["rpos"], adj, g8)
+            ms_b += self.prop.last_timing()[0]
+            g8_src += source_scaling_gradient(med8, shot.src, c["raw_reg"], [x.double().cpu().numpy() for x in gsrc],
+                                              self.dxyz)
+        torch.cuda.synchronize()
+        self.last_timing = {"forward_ms": ms_f, "backward_ms": ms_b}
+        g8h = g8.cpu().numpy() + g8_src
+        return float(J), P.medium_gradient(g8h, med8, self.dtw)
+
+
+def _autograd_function():
+    torch = _torch()
+
+    class ElasticFwiMisfit(torch.autograd.Function):
+        """J(rho, lambda, mu) for an FwiProblem; gradient by the adjoint state."""
+
+        @staticmethod
+        def forward(ctx, rho, lam, mu, problem: FwiProblem):
+            J, g3 = problem.misfit_and_gradient(rho.detach().double().cpu().numpy(), lam.detach().double().cpu().numpy(),
+                                                mu.detach().double().cpu().numpy())
+            g = torch.from_numpy(g3)
+            ctx.save_for_backward(g)
+            ctx.dtypes = (rho.dtype, lam.dtype, mu.dtype)
+            ctx.devices = (rho.device, lam.device, mu.device)
+            return torch.tensor(J, dtype=torch.float64, device=rho.device)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            (g,) = ctx.saved_tensors
+            s = gJ.detach().double().cpu()
+            out = [(s * g[k]).to(dtype=ctx.dtypes[k], device=ctx.devices[k]) for k in range(3)]
+            return out[0], out[1], out[2], None
+
+    return ElasticFwiMisfit
+
+
+_FN = None
+
+
+def fwi_misfit(rho, lam, mu, problem: FwiProblem):
+    """Differentiable misfit: torch tensors in, 0-d float64 tensor out."""
+    global _FN
+    if _FN is None:
+        _FN = _autograd_function()
+    return _FN.apply(rho, lam, mu, problem)

@@ -194,6 +194,59 @@ class Propagator:
         self._keep.clear()
         return dmod8
 
+    def gradient_forward(self, src_pos, src_traces, rec_pos, rec_out=None):
+        """Nonlinear forward that keeps checkpoints for gradient_backward."""
+        mem = self._mem_of(*src_traces, *(rec_out or []))
+        sp, rp = self._points(src_pos), self._points(rec_pos)
+        rc = self._counts(rec_pos)
+        if mem == _lib.ELA_MEM_HOST:
+            src_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in src_traces]
+        rec = rec_out if rec_out is not None else self._alloc_traces(rc, mem == _lib.ELA_MEM_DEVICE)
+        st, rt = self._traces(src_traces, self._counts(src_pos)), self._traces(rec, rc)
+        check(self.lib.ela_gradient_forward(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(rp),
+                                            ctypes.byref(rt), mem), "ela_gradient_forward")
+        self._keep.clear()
+        return rec
+
+    def gradient_backward(self, src_pos, src_traces, rec_pos, adj_traces, grad8=None, src_grad_out=None):
+        """Adjoint propagation of dJ/d(receiver traces). Returns (grad8, src_grad):
+        grad8 (FP64 [8, ny, nx, nz], accumulated; allocated zero if None) and
+        dJ/d(scaled source traces)."""
+        mem = self._mem_of(*src_traces, *adj_traces, grad8)
+        if mem == _lib.ELA_MEM_HOST:
+            src_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in src_traces]
+            adj_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in adj_traces]
+            if grad8 is None:
+                grad8 = np.zeros((8,) + self.shape, np.float64)
+            if not (grad8.dtype == np.float64 and grad8.flags["C_CONTIGUOUS"]):
+                raise ValueError("grad8 must be a C-contiguous float64 array")
+        elif grad8 is None:
+            import torch
+            grad8 = torch.zeros((8,) + self.shape, dtype=torch.float64, device="cuda")
+        if tuple(grad8.shape) != (8,) + self.shape:
+            raise ValueError("grad8 must have shape (8, ny, nx, nz)")
+        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sc = self._counts(src_pos)
+        gs = src_grad_out if src_grad_out is not None else self._alloc_traces(sc, mem == _lib.ELA_MEM_DEVICE)
+        st, at, gt = self._traces(src_traces, sc), self._traces(adj_traces, self._counts(rec_pos)), self._traces(gs, sc)
+        gptr = ctypes.c_void_p(grad8.data_ptr()) if _is_torch(grad8) else ctypes.c_void_p(grad8.ctypes.data)
+        check(self.lib.ela_gradient_backward(self.h, ctypes.byref(sp), ctypes.byref(st), ctypes.byref(rp),
+                                             ctypes.byref(at), gptr, ctypes.byref(gt), mem), "ela_gradient_backward")
+        self._keep.clear()
+        return grad8, gs
+
+    def set_gradient_checkpoints(self, interval: int = 20, outer_interval: int = 0, bf16: bool = True,
+                                 budget_bytes: int = 0) -> None:
+        check(self.lib.ela_set_gradient_checkpoints(self.h, int(interval), int(outer_interval), int(bool(bf16)),
+                                                    int(budget_bytes)), "ela_set_gradient_checkpoints")
+
+    def gradient_plan_used(self):
+        """(interval, outer_interval, bf16, checkpoint bytes) of the last gradient forward."""
+        k, k1, b, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()
+        check(self.lib.ela_gradient_plan_used(self.h, ctypes.byref(k), ctypes.byref(k1), ctypes.byref(b),
+                                              ctypes.byref(nb)), "ela_gradient_plan_used")
+        return k.value, k1.value, bool(b.value), nb.value
+
     def set_tiling(self, tile_x: int, y_chunk: int) -> None:
         check(self.lib.ela_set_tiling(self.h, int(tile_x), int(y_chunk)), "ela_set_tiling")
 

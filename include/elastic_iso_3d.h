@@ -106,7 +106,10 @@ const char* ela_version(void);
 int ela_create(const ela_config* cfg, ela_handle** out);
 int ela_destroy(ela_handle* h);
 
-/* Use a caller-owned hipStream_t (NULL = the handle's own stream). */
+/* Use a caller-owned hipStream_t for device-memory calls. NULL selects the
+ * handle's own stream, a blocking stream that is ordered with HIP's null
+ * (legacy default) stream, so inputs written and outputs read by work on the
+ * null stream need no extra synchronisation. */
 int ela_set_stream(ela_handle* h, void* hip_stream);
 /* Wait for all work queued on the handle's stream. */
 int ela_synchronize(ela_handle* h);
@@ -178,6 +181,44 @@ int ela_set_born_storage(ela_handle* h, int64_t segment, int64_t budget_bytes);
 /* Plan used by the last Born adjoint: -1 all samples kept, K > 0 checkpoint
  * segment length. */
 int ela_born_storage_used(const ela_handle* h, int64_t* segment);
+
+/* ---- Adjoint-state gradient (no counterpart in the reference, whose
+ * inversion uses the Born adjoint as its gradient) ----------------------------
+ *
+ * Gradient of a data functional J(d) of the nonlinear forward data with
+ * respect to the 8 scaled medium arrays: the exact transpose of the
+ * linearised fine-step scheme. Two calls per shot:
+ *   ela_gradient_forward   = ela_nl_forward that also stores checkpoints of the
+ *                            propagation state (kept in the handle);
+ *   ela_gradient_backward  = the adjoint propagation driven by adj = dJ/d(rec)
+ *                            (regular-grid receiver traces, same layout as rec),
+ *                            recomputing the forward between checkpoints and
+ *                            imaging at every fine step. grad8 (8 arrays of
+ *                            nz*nx*ny doubles, host or device per `mem`) is
+ *                            ACCUMULATED in FP64; src_grad (the layout of src)
+ *                            receives dJ/d(scaled source traces), from which
+ *                            the caller adds the source-scaling term.
+ * The backward call must pass the same source positions and traces as the
+ * forward call; ela_set_medium invalidates the stored checkpoints. */
+int ela_gradient_forward(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
+                         const ela_points* rec_pos, ela_traces* rec, int mem);
+int ela_gradient_backward(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
+                          const ela_points* rec_pos, const ela_traces* adj, double* grad8,
+                          ela_traces* src_grad, int mem);
+/* Checkpointing of the gradient forward: every `interval` fine steps the
+ * backward pass recomputes a segment and stores its derivatives; outer
+ * checkpoints are taken every `outer_interval` steps (a multiple of
+ * `interval`; 0: the smallest memory-feasible choice, equal to `interval` when
+ * it fits), in bfloat16 scaled per array and per snapshot by a power of two
+ * (bf16 = 1) or in the handle's precision (bf16 = 0). With outer_interval >
+ * interval the outer segments are recomputed once more to store exact inner
+ * checkpoints. budget_bytes bounds the automatic choice (0: free device
+ * memory). Defaults: 20, 0, 1, 0. */
+int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_interval, int bf16,
+                                 int64_t budget_bytes);
+/* Plan of the last gradient forward and the bytes its outer checkpoints hold. */
+int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* bf16,
+                           int64_t* checkpoint_bytes);
 
 /* Axes of one acquisition grid (its origin already shifted by half a cell on
  * the staggered axes, as python_float/Elastic_iso_float_3D.py:97-110 does),

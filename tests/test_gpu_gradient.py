@@ -1,0 +1,82 @@
+"""Adjoint-state gradient on the GPU (through the C ABI) against the FP64
+oracle gradient, for exact and bfloat16 checkpoints, one and two levels."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.problems import random_problem, rel
+
+pytestmark = pytest.mark.gpu
+
+G = O.Grid(nz=29, nx=27, ny=25, nts=6, sub=4, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=5, alphaCos=0.95)
+
+
+@pytest.fixture(scope="module")
+def problem():
+    P = random_problem(G, seed=31)
+    g8, gsrc = O.lib_gradient(G, P["med"], P["src"], P["spos"], P["rec"], P["rpos"])
+    return P, g8, gsrc
+
+
+def make_prop(precision):
+    from elasticisolib_3d_amd.propagator import Propagator
+    g = G
+    return Propagator(g.nz, g.nx, g.ny, g.nts, g.sub, g.dz, g.dx, g.dy, g.dts, g.minPad, g.alphaCos, precision)
+
+
+CASES = [
+    # precision, interval, outer, bf16, tolerance (relative L2 vs the FP64 oracle)
+    ("f64", 3, 3, False, 1e-11),      # one level, exact checkpoints
+    ("f64", 2, 6, False, 1e-11),      # two levels (outer segment recomputed)
+    ("f64", 20, 0, False, 1e-11),     # one segment longer than the run
+    ("f32", 4, 4, False, 2e-5),
+    ("f32", 2, 8, True, 5e-3),        # bfloat16 outer checkpoints
+]
+
+
+@pytest.mark.parametrize("precision,K,K1,bf16,tol", CASES)
+def test_gradient_matches_oracle(problem, precision, K, K1, bf16, tol):
+    P, g8_ref, gsrc_ref = problem
+    prop = make_prop(precision)
+    prop.set_medium(P["med"])
+    prop.set_gradient_checkpoints(K, K1, bf16)
+    rec = prop.gradient_forward(P["spos"], P["src"], P["rpos"])
+    fwd = prop.nl_forward(P["spos"], P["src"], P["rpos"])
+    assert all(np.array_equal(a, b) for a, b in zip(rec, fwd)), "gradient forward data differ from nl_forward"
+    g8, gsrc = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
+    e8, es = rel(g8, g8_ref), rel(gsrc, gsrc_ref)
+    print(f"gradient {precision} K={K} K1={K1} bf16={bf16}: g8 rel={e8:.3e} src rel={es:.3e} "
+          f"plan={prop.gradient_plan_used()}", flush=True)
+    assert e8 < tol and es < tol
+
+
+def test_gradient_accumulates_and_requires_forward(problem):
+    from elasticisolib_3d_amd._lib import ElaError
+    P, g8_ref, _ = problem
+    prop = make_prop("f64")
+    prop.set_medium(P["med"])
+    with pytest.raises(ElaError):
+        prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
+    prop.set_gradient_checkpoints(3, 0, False)
+    prop.gradient_forward(P["spos"], P["src"], P["rpos"])
+    base = np.full((8,) + G.shape, 0.25)
+    g8, _ = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"], base.copy())
+    assert rel(g8 - base, g8_ref) < 1e-11
+
+
+def test_gradient_device_tensors(problem):
+    import torch
+    P, g8_ref, gsrc_ref = problem
+    prop = make_prop("f32")
+    dev = "cuda"
+    prop.set_medium(torch.tensor(P["med"], dtype=torch.float32, device=dev))
+    spos = [torch.tensor(p, device=dev) for p in P["spos"]]
+    rpos = [torch.tensor(p, device=dev) for p in P["rpos"]]
+    src = [torch.tensor(s, dtype=torch.float32, device=dev) for s in P["src"]]
+    adj = [torch.tensor(s, dtype=torch.float32, device=dev) for s in P["rec"]]
+    prop.set_gradient_checkpoints(4, 0, False)
+    prop.gradient_forward(spos, src, rpos)
+    g8, gsrc = prop.gradient_backward(spos, src, rpos, adj)
+    torch.cuda.synchronize()
+    assert rel(g8.cpu().numpy(), g8_ref) < 2e-5
+    assert rel([x.cpu().numpy() for x in gsrc], gsrc_ref) < 2e-5

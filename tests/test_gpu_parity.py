@@ -174,3 +174,27 @@ def test_born_dot_product_fp32_fp64_reductions(segment):
     err = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
     print(f"Born dot test FP32 (segment {segment}): {lhs:.9e} {rhs:.9e} rel {err:.2e}", flush=True)
     assert err < 1e-6
+
+
+def test_device_outputs_ordered_with_default_stream(problems):
+    """Outputs of a device-memory call are consumed by torch work on the
+    default stream without an explicit synchronize (the library's stream is
+    ordered with it), and repeated calls reuse fresh output tensors."""
+    import torch
+    g, P = GRIDS["odd"], problems["odd"]
+    prop = make_prop(g, "f64")
+    spos = [torch.tensor(p, device="cuda") for p in P["spos"]]
+    rpos = [torch.tensor(p, device="cuda") for p in P["rpos"]]
+    src = [torch.tensor(s, dtype=torch.float64, device="cuda") for s in P["src"]]
+    host = make_prop(g, "f64")
+    host.set_medium(P["med"])
+    ref = host.nl_forward(P["spos"], P["src"], P["rpos"])
+    ref_norm = sum(float(np.sum(r * r)) for r in ref)
+    for scale in (1.0, 1.05, 1.0):
+        prop.set_medium(torch.tensor(P["med"] * scale, dtype=torch.float64, device="cuda"))
+        out = prop.nl_forward(spos, src, rpos)
+        n = float(sum((o * o).sum() for o in out))      # no synchronize before the reduction
+        if scale == 1.0:
+            assert abs(n - ref_norm) <= 1e-10 * ref_norm, (n, ref_norm)
+        else:
+            assert abs(n - ref_norm) > 1e-6 * ref_norm
