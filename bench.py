@@ -243,6 +243,204 @@ def bench_fp64_check(args):
     print(json.dumps(out), flush=True)
 
 
+def _fwi_shots(w, ax):
+    from elasticisolib_3d_amd.acquisition import ShotGeometry
+    from elasticisolib_3d_amd.fwi import FwiShot
+    rec = ShotGeometry(ax, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
+    shots = []
+    for xyz in w.extra["shots"]:
+        src = ShotGeometry(ax, xyz[:, 2], xyz[:, 0], xyz[:, 1])
+        sig = [np.zeros((1, w.nts)) for _ in range(9)]
+        for c in (3, 4, 5):
+            sig[c][0] = w.wavelet
+        shots.append(FwiShot(src, rec, sig))
+    return shots
+
+
+def bench_fwi(args):
+    """FWI misfit gradient (adjoint state) on the 256x256x160 workload: 4 shots,
+    FP32 stencil, bfloat16 checkpoints every 20 fine steps, FP64 gradient
+    accumulation; one shot per launch sequence (the plain path). Accuracy of
+    the FP32 gradient against the FP64 gradient (exact checkpoints) on shot 1,
+    and central differences of the FP64 misfit on `--fd-cells` cells."""
+    import torch
+    from elasticisolib_3d_amd.fwi import FwiProblem
+    from elasticisolib_3d_amd.models import fwi_256
+
+    t0 = time.time()
+    w = fwi_256(nts=args.nts if args.nts != 626 else 501)
+    ax = w.axes
+    start = w.extra["start"]
+    log(f"[bench] model {w.name} padded {ax.nz}x{ax.nx}x{ax.ny} built in {time.time() - t0:.1f}s")
+    shots = _fwi_shots(w, ax)
+    prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, bf16=True)
+    t0 = time.perf_counter()
+    prob.synthesize(w.rho, w.lame, w.mu)
+    log(f"[bench] observed data of {len(shots)} shots synthesized in {time.perf_counter() - t0:.1f}s")
+    times = []
+    for i in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        J, g3 = prob.misfit_and_gradient(*start)
+        times.append(time.perf_counter() - t0)
+        log(f"[bench] gradient {i + 1}: J = {J:.6e}, {times[-1]:.1f}s ({prob.last_timing})")
+    plan = prob.prop.gradient_plan_used()
+    del prob
+    torch.cuda.empty_cache()
+    t_plain = min(times)
+    shots_per_s = len(shots) / t_plain
+    cells = float(w.interior_cells)
+    steps = w.fine_steps
+
+    acc = {}
+    if args.fd_cells >= 0:
+        one = shots[:1]
+        p64 = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f64", interval=20, bf16=False)
+        p64.synthesize(w.rho, w.lame, w.mu)
+        t0 = time.perf_counter()
+        J64, g64 = p64.misfit_and_gradient(*start)
+        log(f"[bench] FP64 gradient (shot 1): J = {J64:.9e}, {time.perf_counter() - t0:.1f}s, "
+            f"plan {p64.prop.gradient_plan_used()}")
+        rng = np.random.default_rng(3)
+        score = np.abs(g64 * start).ravel()
+        top = np.argsort(score)[-max(1, score.size // 10):]
+        pick = rng.choice(top, size=args.fd_cells, replace=False) if args.fd_cells else []
+        fd_rows = []
+        h = 1e-3
+        for q in pick:
+            k, rest = divmod(int(q), start[0].size)
+            idx = np.unravel_index(rest, start[0].shape)
+            mp = [m.copy() for m in start]
+            mm = [m.copy() for m in start]
+            mp[k][idx] *= 1 + h
+            mm[k][idx] *= 1 - h
+            fd = (p64.misfit(*mp) - p64.misfit(*mm)) / (2 * h * start[k][idx])
+            fd_rows.append((k, idx, fd, float(g64[k][idx])))
+            log(f"[bench] FD cell param {k} {tuple(int(i) for i in idx)}: FD {fd:.6e} FP64 adjoint {g64[k][idx]:.6e}")
+        del p64
+        torch.cuda.empty_cache()
+        res = {}
+        for name, bf16 in (("f32_bf16", True), ("f32_exact", False)):
+            p = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f32", interval=20, bf16=bf16)
+            t0 = time.perf_counter()
+            Jx, gx = p.misfit_and_gradient(*start)
+            res[name] = (Jx, gx, time.perf_counter() - t0, p.prop.gradient_plan_used())
+            log(f"[bench] {name} gradient (shot 1): J = {Jx:.9e}, {res[name][2]:.1f}s, plan {res[name][3]}")
+            del p
+            torch.cuda.empty_cache()
+
+        def rel(a, b):
+            return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+        for name, (Jx, gx, tx, px) in res.items():
+            acc[name] = {
+                "grad_rel_l2_vs_fp64": rel(gx, g64),
+                "grad_rel_l2_per_param": [rel(gx[k], g64[k]) for k in range(3)],
+                "misfit_rel_vs_fp64": abs(Jx - J64) / J64,
+                "checkpoint_plan": {"interval": px[0], "outer_interval": px[1], "bf16": px[2],
+                                    "checkpoint_GB": round(px[3] / 1e9, 2)},
+            }
+            if fd_rows:
+                errs = [abs(float(gx[k][idx]) - fd) / abs(fd) for (k, idx, fd, _) in fd_rows]
+                acc[name]["fd_rel_err_max"] = max(errs)
+                acc[name]["fd_rel_err_median"] = float(np.median(errs))
+        if fd_rows:
+            e64 = [abs(g - fd) / abs(fd) for (_, _, fd, g) in fd_rows]
+            acc["f64_exact"] = {"fd_rel_err_max": max(e64), "fd_rel_err_median": float(np.median(e64)),
+                                "fd_cells": len(fd_rows), "fd_h_relative": h}
+
+    out = {
+        "metric": "FWI-gradient shots/s (plain: one shot per launch sequence)",
+        "value": round(shots_per_s, 4),
+        "unit": "shots/s",
+        "n_gpus": 1,
+        "steps": len(times),
+        "warmup": 0,
+        "ms_per_step": round(t_plain * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 stencil, bf16 checkpoints, f64 gradient accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": "FWI gradient, 256x256x160 layered model (padded 232x328x324), d=15 m, start = true smoothed "
+                        f"sigma=20 cells, {len(shots)} explosive shots (8 Hz Ricker), {w.nts} coarse samples x {w.sub} "
+                        f"fine steps, {len(w.rec_xyz)} surface receivers, L2 misfit, checkpoints every 20 fine steps",
+            "gradient_s_per_call": [round(t, 2) for t in times],
+            "checkpoint_plan": {"interval": plan[0], "outer_interval": plan[1], "bf16": plan[2],
+                                "checkpoint_GB_per_shot": round(plan[3] / 1e9, 2)},
+            "interior_cells": w.interior_cells,
+            "fine_steps_per_propagation": steps,
+            "propagation_gcell_updates_per_s": round(3 * cells * steps * len(shots) / t_plain / 1e9, 2),
+            "accuracy_shot1": acc,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_fwi_batch(args):
+    """Batched vs plain FWI gradient with shot-local windows on the 400x400x250
+    layered model: the first `--shots` shots of the 8x8 jittered grid, plain
+    = one shot per launch sequence, batched = `--batch` windows stacked in one
+    grid per launch sequence (same kernels, same checkpointing)."""
+    import torch
+    from elasticisolib_3d_amd.batch import BatchShot, BatchedFwi, Window
+    from elasticisolib_3d_amd.models import batch_layered_400
+
+    t0 = time.time()
+    cfg = batch_layered_400(nts=args.nts)
+    log(f"[bench] models built in {time.time() - t0:.1f}s")
+    nts = cfg["nts"]
+    shots = []
+    for src, rec, (x0, y0, wx, wy) in cfg["shots"][:args.shots]:
+        sig = [np.zeros((1, nts)) for _ in range(9)]
+        for c in (3, 4, 5):
+            sig[c][0] = cfg["wavelet"]
+        shots.append(BatchShot(src, rec, Window(x0, y0, wx, wy), sig))
+    shape = cfg["true"][0].shape
+    kw = dict(precision="f32", interval=20, bf16=True)
+    ref = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], shots, batch=1, **kw)
+    t0 = time.perf_counter()
+    ref.synthesize(cfg["true"])
+    log(f"[bench] observed data of {len(shots)} shots in {time.perf_counter() - t0:.1f}s")
+    res = {}
+    for name, b in (("plain", 1), ("batched", args.batch)):
+        op = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], shots, batch=b, **kw)
+        J, g = op.misfit_and_gradient(cfg["start"])
+        res[name] = (J, g, op.timing)
+        log(f"[bench] {name} (batch {b}): J = {J:.9e}, {op.timing}")
+        torch.cuda.empty_cache()
+    gp, gb = res["plain"][1], res["batched"][1]
+    diff = float(np.linalg.norm(gb - gp) / np.linalg.norm(gp))
+    tp, tb = res["plain"][2]["total_s"], res["batched"][2]["total_s"]
+    wins = [s.window for s in shots]
+    out = {
+        "metric": "FWI-gradient shots/s, batched vs plain (shot-local windows)",
+        "value": round(len(shots) / tb, 4),
+        "unit": "shots/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(tb * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 stencil, bf16 checkpoints, f64 gradient accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": f"FWI gradient of {len(shots)} shots with shot-local windows on the 400x400x250 layered "
+                        f"model (d=12.5 m, 30-cell sponge per window), explosive 10 Hz Ricker, {nts} coarse samples x "
+                        f"{cfg['sub']} fine steps",
+            "windows": [[w.x0, w.y0, w.nx, w.ny] for w in wins],
+            "receivers": [int(len(s.rec_xyz)) for s in shots],
+            "plain_shots_per_s": round(len(shots) / tp, 4),
+            "batched_shots_per_s": round(len(shots) / tb, 4),
+            "batch": args.batch,
+            "speedup_batched_over_plain": round(tp / tb, 3),
+            "plain_s": round(tp, 2), "batched_s": round(tb, 2),
+            "plain_propagation_s": round(res["plain"][2]["propagation_s"], 2),
+            "batched_propagation_s": round(res["batched"][2]["propagation_s"], 2),
+            "gradient_rel_diff_batched_vs_plain": diff,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,9 +451,15 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for counter calibration)")
-    ap.add_argument("--workload", choices=("nl_forward", "born", "fp64_check"), default="nl_forward",
+    ap.add_argument("--workload", choices=("nl_forward", "born", "fp64_check", "fwi", "fwi_batch"),
+                    default="nl_forward",
                     help="nl_forward (default): the headline line; born: Born forward/adjoint and dot test; "
-                         "fp64_check: FP32 vs FP64 traces on the forward workload")
+                         "fp64_check: FP32 vs FP64 traces on the forward workload; fwi: FWI gradient shots/s "
+                         "and gradient accuracy")
+    ap.add_argument("--shots", type=int, default=4, help="fwi_batch: number of shots")
+    ap.add_argument("--batch", type=int, default=2, help="fwi_batch: shots per batched launch sequence")
+    ap.add_argument("--fd-cells", type=int, default=16,
+                    help="fwi: cells checked by central differences (-1: skip the accuracy part)")
     ap.add_argument("--born-segment", type=int, default=0,
                     help="Born adjoint storage: 0 automatic, -1 all samples, K checkpoint segment")
     args = ap.parse_args()
@@ -278,6 +482,10 @@ def main():
         return bench_born(args)
     if args.workload == "fp64_check":
         return bench_fp64_check(args)
+    if args.workload == "fwi":
+        return bench_fwi(args)
+    if args.workload == "fwi_batch":
+        return bench_fwi_batch(args)
 
     import torch
     from elasticisolib_3d_amd.models import layered_400

@@ -27,6 +27,7 @@ EXPORTS = (
     "ela_set_tile_order", "ela_set_born_storage", "ela_born_storage_used",
     "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
     "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
+    "ela_set_windows",
 )
 
 
@@ -106,8 +107,9 @@ def load() -> ctypes.CDLL:
     lib.ela_born_adjoint.argtypes = [vp, P, Tr, vp, P, Tr, i32]
     lib.ela_interp_linear.argtypes = [ctypes.POINTER(ElaInterpAxes), i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)]
     lib.ela_set_tiling.argtypes = [vp, i32, i32]
-    lib.ela_set_tile_order.argtypes = [vp, i32]
+    lib.ela_set_tile_order.argtypes = [vp, i32, i32]
     lib.ela_set_born_storage.argtypes = [vp, i64, i64]
+    lib.ela_set_windows.argtypes = [vp, i32, vp, vp, vp]
     lib.ela_gradient_forward.argtypes = [vp, P, Tr, P, Tr, i32]
     lib.ela_gradient_backward.argtypes = [vp, P, Tr, P, Tr, vp, Tr, i32]
     lib.ela_set_gradient_checkpoints.argtypes = [vp, i64, i64, i32, i64]

@@ -1,0 +1,109 @@
+"""Autotuner of the stencil launch shape.
+
+The stencil kernels take three speed-only parameters (results are
+bit-identical for every choice): the x extent of a 64-wide z tile (4, 8 or
+16 rows, i.e. 256, 512 or 1024 threads), the number of y planes one block
+marches through (the depth of its register queue pipeline before it hands the
+column to the next block), and the block-to-tile order (hardware round robin
+over the 8 XCDs, contiguous ranges per XCD, or y-major ranges per XCD in
+groups of x columns). The best choice depends on the grid (tile counts,
+partial tiles, L2 reuse of halo planes), so it is measured: every candidate
+is timed with HIP events on the propagator's own stream
+(ela_time_step_kernels: a velocity and a stress launch of a forward step on
+the handle's current wavefields, which it overwrites), and the fastest is
+applied. Results are cached per device name, precision and grid shape in a
+JSON file (ELA_TUNE_CACHE, default ~/.cache/elasticisolib_3d_amd/tuning.json).
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Dict, Iterable, Optional, Sequence, Tuple
+
+TILE_X = (16, 8, 4)
+Y_CHUNK = (4, 8, 16, 32)
+ORDERS = ((0, 1), (1, 1), (2, 1), (2, 2), (2, 4))
+
+Candidate = Tuple[int, int, int, int]  # tile_x, y_chunk, order, x_group
+
+
+def default_candidates(precision: str = "f32") -> Sequence[Candidate]:
+    tiles = TILE_X if precision == "f32" else (8, 4)
+    return [(tx, yc, o, g) for tx in tiles for yc in Y_CHUNK for (o, g) in ORDERS]
+
+
+def _cache_path() -> str:
+    return os.environ.get("ELA_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "elasticisolib_3d_amd",
+                                                         "tuning.json"))
+
+
+def _device_name() -> str:
+    try:
+        import torch
+        return torch.cuda.get_device_name(0)
+    except Exception:  # the name only keys the cache
+        return "unknown"
+
+
+def _key(prop) -> str:
+    ny, nx, nz = prop.shape
+    return f"{_device_name()}|{prop.precision}|{nz}x{nx}x{ny}"
+
+
+def _load() -> Dict[str, dict]:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table: Dict[str, dict]) -> None:
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # an unwritable cache only costs a re-tune
+
+
+def apply(prop, best: dict) -> None:
+    prop.set_tiling(best["tile_x"], best["y_chunk"])
+    prop.set_tile_order(best["order"], best["x_group"])
+
+
+def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = None, use_cache: bool = True,
+             verbose: bool = False) -> dict:
+    """Time every candidate launch shape on `prop` (its medium must be set),
+    apply the fastest and return {tile_x, y_chunk, order, x_group, ms_step,
+    table}. The wavefields of the handle are overwritten."""
+    key = _key(prop)
+    if use_cache and candidates is None:
+        hit = _load().get(key)
+        if hit:
+            apply(prop, hit)
+            return dict(hit, cached=True)
+    cands = list(candidates) if candidates is not None else list(default_candidates(prop.precision))
+    prop.set_tiling(*cands[0][:2])
+    prop.time_step_kernels(2)  # warm the code objects
+    table = []
+    for tx, yc, order, xg in cands:
+        prop.set_tiling(tx, yc)
+        prop.set_tile_order(order, xg)
+        kv, ks = prop.time_step_kernels(reps)
+        table.append((kv + ks, kv, ks, tx, yc, order, xg))
+        if verbose:
+            print(f"[autotune] tile {tx}x{yc} order {order}/{xg}: velocity {kv:.4f} ms stress {ks:.4f} ms", flush=True)
+    table.sort()
+    t, kv, ks, tx, yc, order, xg = table[0]
+    best = {"tile_x": tx, "y_chunk": yc, "order": order, "x_group": xg, "ms_step": t, "ms_velocity": kv,
+            "ms_stress": ks}
+    apply(prop, best)
+    if use_cache and candidates is None:
+        tab = _load()
+        tab[key] = best
+        _store(tab)
+    return dict(best, cached=False, table=[list(r) for r in table])

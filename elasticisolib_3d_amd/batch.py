@@ -1,0 +1,293 @@
+"""Batched multi-shot propagation with shot-local windows.
+
+Each shot is modelled on a window of the model (a horizontal sub-rectangle,
+full depth) padded with its own sponge, as the reference's padding tool pads
+a model (padElasticFileGpuMain_3D.cpp:47-86). A batch of B windows is stacked
+along y in one grid (ela_set_windows): the stencil, injection and recording
+launches of a fine step then advance all B shots at once, each window keeping
+its own FAT layers and sponge, so every shot's result equals its propagation
+alone. B = 1 is the plain path (one shot per launch sequence) on the same
+code.
+
+WindowStack maps between the full model and the stack: medium (staggered and
+scaled per window, never across windows), point positions, and gradients back
+onto the full model (the transpose of the cut-and-pad).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import propagator as P
+from .acquisition import COMP_GRID, PaddedAxes, ShotGeometry, scale_regular_traces, source_scaling_gradient
+from .models import FAT, pad_sizes
+
+
+@dataclass
+class Window:
+    """Model cells x in [x0, x0 + nx), y in [y0, y0 + ny), all z."""
+    x0: int
+    y0: int
+    nx: int
+    ny: int
+
+
+def _pad_index(n: int, lo: int, hi: int) -> np.ndarray:
+    """Source index of every padded cell under edge replication."""
+    return np.clip(np.arange(-lo, n + hi), 0, n - 1)
+
+
+def _unpad(g: np.ndarray, axis: int, lo: int, n: int) -> np.ndarray:
+    """Transpose of edge-replication padding along one axis: the n core
+    planes, with the low padding summed into the first and the high padding
+    into the last."""
+    g = np.moveaxis(g, axis, 0)
+    core = g[lo:lo + n].copy()
+    core[0] += g[:lo].sum(axis=0)
+    core[-1] += g[lo + n:].sum(axis=0)
+    return np.moveaxis(core, 0, axis)
+
+
+class WindowStack:
+    """B windows of a model of (nz, nx, ny) cells, spacing d, each padded
+    with `pad` sponge cells (plus FAT) and stacked along y."""
+
+    def __init__(self, model_shape: Tuple[int, int, int], d: float, pad: int, windows: Sequence[Window]):
+        nz, nxm, nym = model_shape
+        self.nz_model, self.nx_model, self.ny_model = nz, nxm, nym
+        self.d, self.pad = float(d), int(pad)
+        self.windows = list(windows)
+        zm, zp = pad_sizes(nz, pad)
+        self.zpads = (zm, zp)
+        self.nz = nz + zm + zp + 2 * FAT
+        self.axes: List[PaddedAxes] = []
+        self.xpads, self.ny_w, self.nx_w, self.y_off = [], [], [], []
+        y = 0
+        for w in self.windows:
+            if w.x0 < 0 or w.y0 < 0 or w.x0 + w.nx > nxm or w.y0 + w.ny > nym:
+                raise ValueError(f"window {w} leaves the model")
+            xm, xp = pad_sizes(w.nx, pad)
+            nxp = w.nx + xm + xp + 2 * FAT
+            nyp = w.ny + 2 * pad + 2 * FAT
+            self.xpads.append((xm, xp))
+            self.nx_w.append(nxp)
+            self.ny_w.append(nyp)
+            self.y_off.append(y)
+            y += nyp
+            self.axes.append(PaddedAxes(self.nz, nxp, nyp, -(FAT + zm) * d, (w.x0 - FAT - xm) * d,
+                                        (w.y0 - FAT - pad) * d, d, d, d, FAT, zm, zp, xm, xp, pad))
+        self.nx = max(self.nx_w)
+        self.ny = y
+        self.min_pad = min(min(zm, zp), min(min(x) for x in self.xpads))
+
+    # -- model ------------------------------------------------------------
+    def _cut_index(self, k: int):
+        w = self.windows[k]
+        zm, zp = self.zpads
+        xm, xp = self.xpads[k]
+        iz = _pad_index(self.nz_model, FAT + zm, FAT + zp)
+        ix = w.x0 + _pad_index(w.nx, FAT + xm, FAT + xp)
+        iy = w.y0 + _pad_index(w.ny, FAT + self.pad, FAT + self.pad)
+        return iy, ix, iz
+
+    def window_model(self, k: int, m: np.ndarray) -> np.ndarray:
+        """Padded window k of a [..., ny, nx, nz] model array."""
+        iy, ix, iz = self._cut_index(k)
+        return m[..., iy[:, None, None], ix[None, :, None], iz[None, None, :]]
+
+    def window_adjoint(self, k: int, g: np.ndarray, out: np.ndarray) -> None:
+        """out (full model [..., ny, nx, nz]) += the transpose of window_model
+        applied to g (padded window k): the replicated cells of each padding
+        slab add into the edge plane they copy."""
+        w = self.windows[k]
+        zm, zp = self.zpads
+        xm, xp = self.xpads[k]
+        g = _unpad(g, -1, FAT + zm, self.nz_model)
+        g = _unpad(g, -2, FAT + xm, w.nx)
+        g = _unpad(g, -3, FAT + self.pad, w.ny)
+        out[..., w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :] += g
+
+    def stack(self, per_window: Sequence[np.ndarray], dtype=None) -> np.ndarray:
+        """[..., ny_w, nx_w, nz] arrays -> one stacked [..., ny, nx, nz] array
+        (zeros beyond a narrower window)."""
+        lead = per_window[0].shape[:-3]
+        out = np.zeros(lead + (self.ny, self.nx, self.nz), dtype or per_window[0].dtype)
+        for k, a in enumerate(per_window):
+            out[..., self.y_off[k]:self.y_off[k] + self.ny_w[k], :self.nx_w[k], :] = a
+        return out
+
+    def unstack(self, k: int, a: np.ndarray) -> np.ndarray:
+        return a[..., self.y_off[k]:self.y_off[k] + self.ny_w[k], :self.nx_w[k], :]
+
+    def stacked_medium(self, rho, lam, mu, dtw: float):
+        """(stacked med8, list of per-window med8): staggering and scaling
+        window by window."""
+        meds = [P.scale_medium(self.window_model(k, rho), self.window_model(k, lam), self.window_model(k, mu), dtw)
+                for k in range(len(self.windows))]
+        return self.stack(meds), meds
+
+    # -- points -----------------------------------------------------------
+    def stacked_positions(self, k: int, pos: Sequence[np.ndarray]) -> List[np.ndarray]:
+        """Flat positions on window k's padded grid -> flat stacked positions."""
+        out = []
+        plane_w = self.nx_w[k] * self.nz
+        for p in pos:
+            p = np.asarray(p, np.int64)
+            iy, r = np.divmod(p, plane_w)
+            ix, iz = np.divmod(r, self.nz)
+            out.append(((iy + self.y_off[k]) * self.nx + ix) * self.nz + iz)
+        return out
+
+    def set_windows(self, prop: P.Propagator) -> None:
+        prop.set_windows(self.y_off, self.ny_w, self.nx_w)
+
+
+@dataclass
+class BatchShot:
+    src_xyz: np.ndarray          # [n, 3] (x, y, z) model coordinates
+    rec_xyz: np.ndarray          # [m, 3]
+    window: Window
+    signals: List[np.ndarray]    # 9 irregular [n, nts]
+    observed: Optional[List[np.ndarray]] = None   # 9 irregular [m, nts] (host FP64)
+
+
+class BatchedFwi:
+    """FWI misfit and gradient over shots with shot-local windows, `batch`
+    shots per launch sequence on one GPU (FP32 stencil, bfloat16 or exact
+    checkpoints, FP64 gradient accumulation, as FwiProblem). `model_shape`
+    is the [ny, nx, nz] shape of the (unpadded) model arrays."""
+
+    def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
+                 shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
+                 bf16: bool = True, device: int = 0, tiling=None):
+        self.model_shape, self.d, self.pad = tuple(model_shape), float(d), int(pad)
+        self.nts, self.dts, self.sub, self.alpha = int(nts), float(dts), int(sub), float(alpha_cos)
+        self.dtw = self.dts / self.sub
+        self.shots = list(shots)
+        self.batch = max(1, int(batch))
+        self.precision, self.interval, self.bf16, self.device = precision, int(interval), bool(bf16), int(device)
+        self.tiling = tiling
+        self.timing = {}
+
+    def _groups(self):
+        return [self.shots[i:i + self.batch] for i in range(0, len(self.shots), self.batch)]
+
+    def _setup(self, group, model):
+        """Stack, propagator and per-shot geometry of one batch."""
+        import torch
+        ws = WindowStack((self.model_shape[2], self.model_shape[1], self.model_shape[0]), self.d, self.pad,
+                         [s.window for s in group])
+        med, meds = ws.stacked_medium(model[0], model[1], model[2], self.dtw)
+        prop = P.Propagator(ws.nz, ws.nx, ws.ny, self.nts, self.sub, self.d, self.d, self.d, self.dts, ws.min_pad,
+                            self.alpha, self.precision, self.device)
+        ws.set_windows(prop)
+        if self.tiling:
+            prop.set_tiling(*self.tiling[:2])
+            if len(self.tiling) > 2:
+                prop.set_tile_order(*self.tiling[2:])
+        td = torch.float32 if self.precision == "f32" else torch.float64
+        prop.set_medium(torch.tensor(med, dtype=td, device="cuda"))
+        prop.set_gradient_checkpoints(self.interval, 0, self.bf16)
+        geos = []
+        for k, s in enumerate(group):
+            ax = ws.axes[k]
+            sg = ShotGeometry(ax, s.src_xyz[:, 2], s.src_xyz[:, 0], s.src_xyz[:, 1])
+            rg = ShotGeometry(ax, s.rec_xyz[:, 2], s.rec_xyz[:, 0], s.rec_xyz[:, 1])
+            geos.append((sg, rg))
+        return ws, meds, prop, geos, td
+
+    def _merged(self, ws, geos, which):
+        """Per grid: stacked positions of all shots of the batch, concatenated."""
+        pos = [[] for _ in range(7)]
+        for k, g in enumerate(geos):
+            sp = ws.stacked_positions(k, g[which].positions())
+            for j in range(7):
+                pos[j].append(sp[j])
+        return [np.concatenate(p) if p else np.zeros(0, np.int64) for p in pos]
+
+    def _src_traces(self, ws, meds, geos, group, td):
+        """Scaled regular source traces of the batch (9 concatenated arrays)
+        and the unscaled regular traces per shot."""
+        import torch
+        raw, scaled = [], []
+        for k, (s, (sg, _)) in enumerate(zip(group, geos)):
+            r = sg.to_regular(s.signals)
+            raw.append(r)
+            scaled.append(scale_regular_traces(meds[k], sg, r, self.d ** 3))
+        cat = [np.concatenate([sc[c] for sc in scaled]) for c in range(9)]
+        return [torch.tensor(c, dtype=td, device="cuda") for c in cat], raw
+
+    def _split_rec(self, geos, rec):
+        """Concatenated regular receiver traces -> per shot irregular FP64."""
+        out, off = [], [0] * 7
+        for sg, rg in geos:
+            reg = []
+            for c in range(9):
+                g = COMP_GRID[c]
+                n = rg.counts()[g]
+                reg.append(rec[c][off[g]:off[g] + n].double().cpu().numpy())
+            for g in range(7):
+                off[g] += rg.counts()[g]
+            out.append(rg.to_irregular(reg))
+        return out
+
+    def synthesize(self, model) -> None:
+        import torch
+        for group in self._groups():
+            ws, meds, prop, geos, td = self._setup(group, model)
+            src, _ = self._src_traces(ws, meds, geos, group, td)
+            rec = prop.nl_forward(self._merged(ws, geos, 0), src, self._merged(ws, geos, 1))
+            torch.cuda.synchronize()
+            for s, d in zip(group, self._split_rec(geos, rec)):
+                s.observed = d
+            del prop
+
+    def misfit_and_gradient(self, model):
+        """(J, dJ/d(rho, lambda, mu) on the full model [3, ny, nx, nz])."""
+        import torch
+        nym, nxm, nzm = self.model_shape
+        g3 = np.zeros((3, nym, nxm, nzm), np.float64)
+        J = 0.0
+        t_prop = 0.0
+        t0 = time.perf_counter()
+        for group in self._groups():
+            ws, meds, prop, geos, td = self._setup(group, model)
+            src, raw = self._src_traces(ws, meds, geos, group, td)
+            spos, rpos = self._merged(ws, geos, 0), self._merged(ws, geos, 1)
+            ta = time.perf_counter()
+            rec = prop.gradient_forward(spos, src, rpos)
+            torch.cuda.synchronize()
+            t_prop += time.perf_counter() - ta
+            pred = self._split_rec(geos, rec)
+            adj_parts = []
+            for s, p, (sg, rg) in zip(group, pred, geos):
+                res = [p[c] - s.observed[c] for c in range(9)]
+                J += 0.5 * sum(float(np.sum(r * r)) for r in res)
+                adj_parts.append(rg.to_regular(res))
+            adj = []
+            for c in range(9):
+                adj.append(torch.tensor(np.concatenate([a[c] for a in adj_parts]), dtype=td, device="cuda"))
+            ta = time.perf_counter()
+            g8, gsrc = prop.gradient_backward(spos, src, rpos, adj)
+            torch.cuda.synchronize()
+            t_prop += time.perf_counter() - ta
+            g8 = g8.cpu().numpy()
+            off = [0] * 7
+            for k, (sg, rg) in enumerate(geos):
+                gs = []
+                for c in range(9):
+                    g = COMP_GRID[c]
+                    n = sg.counts()[g]
+                    gs.append(gsrc[c][off[g]:off[g] + n].double().cpu().numpy())
+                for g in range(7):
+                    off[g] += sg.counts()[g]
+                g8k = ws.unstack(k, g8) + source_scaling_gradient(meds[k], sg, raw[k], gs, self.d ** 3)
+                ws.window_adjoint(k, P.medium_gradient(g8k, meds[k], self.dtw), g3)
+            del prop
+            torch.cuda.empty_cache()
+        self.timing = {"total_s": time.perf_counter() - t0, "propagation_s": t_prop}
+        return J, g3

@@ -88,6 +88,23 @@ struct DevBuf {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// ELA_MEM_DEVICE calls: every caller pointer must be memory the GPU can
+// access (device, managed or pinned host memory). A pageable host pointer is
+// an argument error, never a kernel that reads host addresses.
+int check_dev_ptr(const void* p, const char* what) {
+  if (!p) return ELA_OK;
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  const hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ELA_ERR_ARG, "%s is not GPU-accessible memory (device-memory call)", what);
+  }
+  if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged && attr.type != hipMemoryTypeHost)
+    return fail(ELA_ERR_ARG, "%s is not GPU-accessible memory (device-memory call)", what);
+  return ELA_OK;
+}
+
 }  // namespace
 
 struct ela_handle {
@@ -117,7 +134,8 @@ struct ela_handle {
   int64_t last_steps = 0;
   int by_x = 16;            // x extent of a tile
   int ychunk = 8;           // y planes per block
-  int remap = 0;            // 1: contiguous tile ranges per XCD
+  int remap = 0;            // tile order (ela_set_tile_order)
+  int xgroup = 1;           // x columns per group of the y-major order
   int64_t born_segment = 0; // Born adjoint storage: 0 auto, -1 all samples, K > 0 checkpoints every K
   int64_t born_budget = 0;  // device-memory budget of the automatic plan [bytes], 0: free memory
   int64_t born_last_segment = 0;  // plan of the last Born adjoint (-1 full storage)
@@ -136,6 +154,12 @@ struct ela_handle {
   bool grad_ready = false;      // a gradient forward has stored its checkpoints
   int64_t plan_K = 0, plan_K1 = 0, plan_n1 = 0;
   int plan_bf16 = 0;
+  // windows stacked along y (several shots in one grid)
+  std::vector<int64_t> win_y0, win_ny, win_nx;
+  DevBuf ytab;              // per plane {window y0, ny, nx, 0} (int32)
+  DevBuf chunks;            // per y chunk {y0, y1, window y0, ny, nx, 0, 0, 0} (int32)
+  int chunks_yc = 0;        // y chunk the table was built for
+  int nchunks = 0;
 };
 
 namespace {
@@ -183,12 +207,33 @@ struct Tiles {
   int nbz, nbx, nby, nblocks;
 };
 
+// Chunk table of the stacked windows for the current y chunk (rebuilt when
+// the y chunk changes; a synchronous upload, done before any launch of the
+// propagation that uses it).
+int ensure_chunks(ela_handle* h) {
+  if (h->win_y0.empty() || h->chunks_yc == h->ychunk) return ELA_OK;
+  std::vector<int> tab;
+  for (size_t w = 0; w < h->win_y0.size(); ++w) {
+    const int64_t a = h->win_y0[w] + ela::FAT, b = h->win_y0[w] + h->win_ny[w] - ela::FAT;
+    for (int64_t y = a; y < b; y += h->ychunk) {
+      const int row[8] = {(int)y, (int)std::min<int64_t>(y + h->ychunk, b), (int)h->win_y0[w], (int)h->win_ny[w],
+                          (int)h->win_nx[w], 0, 0, 0};
+      tab.insert(tab.end(), row, row + 8);
+    }
+  }
+  CHK(h->chunks.alloc(tab.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(h->chunks.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+  h->nchunks = (int)(tab.size() / 8);
+  h->chunks_yc = h->ychunk;
+  return ELA_OK;
+}
+
 Tiles tiles_for(const ela_handle* h, int bx, int ychunk) {
   Tiles t;
   const int64_t iz = h->cfg.nz - 2 * ela::FAT, ix = h->cfg.nx - 2 * ela::FAT, iy = h->cfg.ny - 2 * ela::FAT;
   t.nbz = (int)((iz + ela::TZ - 1) / ela::TZ);
   t.nbx = (int)((ix + bx - 1) / bx);
-  t.nby = (int)((iy + ychunk - 1) / ychunk);
+  t.nby = h->win_y0.empty() ? (int)((iy + ychunk - 1) / ychunk) : h->nchunks;
   t.nblocks = t.nbz * t.nbx * t.nby;
   return t;
 }
@@ -207,6 +252,11 @@ ela::StepArgs<T> base_args(ela_handle* h, T* o[9], T* c[9]) {
   a.ys = h->cfg.nx * h->nzi;
   a.ychunk = h->ychunk;
   a.remap = h->remap;
+  a.xgroup = h->xgroup;
+  if (!h->win_y0.empty()) {
+    (void)ensure_chunks(h);
+    a.chunks = (const int*)h->chunks.p;
+  }
   return a;
 }
 
@@ -242,6 +292,7 @@ int fill(ela_handle* h, void* p, int64_t n) {
 // 3-D array [k][ny][nx][nz] (caller) <-> library layout, k = 0..n_arrays-1,
 // elements of es bytes.
 int copy_in_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem, size_t es) {
+  if (mem == ELA_MEM_DEVICE) CHK(check_dev_ptr(src, "3-D input array"));
   const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   for (int k = 0; k < n_arrays; ++k)
     HIPCHK(hipMemcpy2DAsync((char*)dst + (k * h->stride + h->zoff) * es, h->nzi * es,
@@ -255,6 +306,7 @@ int copy_in_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem)
 }
 
 int copy_out_3d(ela_handle* h, void* dst, const void* src, int n_arrays, int mem, size_t es) {
+  if (mem == ELA_MEM_DEVICE) CHK(check_dev_ptr(dst, "3-D output array"));
   const hipMemcpyKind kind = mem == ELA_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   for (int k = 0; k < n_arrays; ++k)
     HIPCHK(hipMemcpy2DAsync((char*)dst + k * h->n_model * es, h->cfg.nz * es,
@@ -280,6 +332,11 @@ struct DevPoints {
 int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, int mem, DevPoints& ds, DevPoints& dr) {
   int64_t total = 0;
   for (int g = 0; g < 7; ++g) total += sp->n[g] + rp->n[g];
+  if (mem == ELA_MEM_DEVICE)
+    for (int g = 0; g < 7; ++g) {
+      if (sp->n[g]) CHK(check_dev_ptr(sp->pos[g], "a source position list"));
+      if (rp->n[g]) CHK(check_dev_ptr(rp->pos[g], "a receiver position list"));
+    }
   // layout: [int64 staging (host calls)] [int32 converted] [int flag]
   const size_t b64 = round_up(total * 8, 256), b32 = round_up(total * 4, 256);
   CHK(h->pos.alloc(b64 + b32 + 256));
@@ -304,7 +361,7 @@ int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, in
         }
         hipLaunchKernelGGL(ela::k_convert_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src,
                            s32 + off, n, (int)h->cfg.nz, (int)h->cfg.nx, (int)h->cfg.ny, (int)h->nzi, (int)h->zoff,
-                           flag);
+                           h->win_y0.empty() ? nullptr : (const int*)h->ytab.p, flag);
       }
       off += n;
     }
@@ -338,6 +395,7 @@ ela::PointArgs<T> make_points(ela_handle* h, const DevPoints& pts, T* const fiel
   a.min_pad = (int)h->cfg.min_pad;
   a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
   a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
+  a.ytab = h->win_y0.empty() ? nullptr : (const int*)h->ytab.p;
   return a;
 }
 
@@ -377,7 +435,12 @@ int64_t trace_elems(const ela_points* p, int64_t nts) {
 int stage_traces(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp,
                  const ela_traces* rt, bool copy_src, bool copy_rec, int mem, void* dsrc[9], void* drec[9]) {
   if (mem == ELA_MEM_DEVICE) {
-    for (int c = 0; c < 9; ++c) { dsrc[c] = st->tr[c]; drec[c] = rt->tr[c]; }
+    for (int c = 0; c < 9; ++c) {
+      if (sp->n[kCompGrid[c]]) CHK(check_dev_ptr(st->tr[c], "a source trace array"));
+      if (rp->n[kCompGrid[c]]) CHK(check_dev_ptr(rt->tr[c], "a receiver trace array"));
+      dsrc[c] = st->tr[c];
+      drec[c] = rt->tr[c];
+    }
     return ELA_OK;
   }
   const int64_t nts = h->cfg.nts;
@@ -436,8 +499,15 @@ struct TmViews {
   }
 };
 
+// Interior cells the stepper advances (the windows' interiors when windows
+// are stacked).
 int64_t interior_cells(const ela_handle* h) {
-  return (h->cfg.nz - 2 * ela::FAT) * (h->cfg.nx - 2 * ela::FAT) * (h->cfg.ny - 2 * ela::FAT);
+  if (h->win_y0.empty())
+    return (h->cfg.nz - 2 * ela::FAT) * (h->cfg.nx - 2 * ela::FAT) * (h->cfg.ny - 2 * ela::FAT);
+  int64_t n = 0;
+  for (size_t w = 0; w < h->win_y0.size(); ++w)
+    n += (h->cfg.nz - 2 * ela::FAT) * (h->win_nx[w] - 2 * ela::FAT) * (h->win_ny[w] - 2 * ela::FAT);
+  return n;
 }
 
 int64_t total_points(const DevPoints& p) {
@@ -593,6 +663,11 @@ ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* Vo, const T* Vc, const 
   a.ys = h->cfg.nx * h->nzi;
   a.ychunk = h->ychunk;
   a.remap = h->remap;
+  a.xgroup = h->xgroup;
+  if (!h->win_y0.empty()) {
+    (void)ensure_chunks(h);
+    a.chunks = (const int*)h->chunks.p;
+  }
   Tiles t = tiles_for(h, 8, a.ychunk);
   a.nbz = t.nbz; a.nbx = t.nbx; a.nby = t.nby;
   return a;
@@ -887,7 +962,12 @@ struct GradPlan {
   size_t ck1_bytes = 0, ck2_bytes = 0;  // one checkpoint
 };
 
-int64_t interior_count(const ela_handle* h) { return interior_cells(h); }
+// Cells of the checkpoint packing: the grid interior (with stacked windows
+// this includes the windows' FAT planes and the columns beyond a narrower
+// window, which hold zeros).
+int64_t interior_count(const ela_handle* h) {
+  return (h->cfg.nz - 2 * ela::FAT) * (h->cfg.nx - 2 * ela::FAT) * (h->cfg.ny - 2 * ela::FAT);
+}
 
 int grad_plan(ela_handle* h, GradPlan& p) {
   const int64_t N = (h->cfg.nts - 1) * h->cfg.sub;
@@ -1078,7 +1158,10 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   // source-trace gradient outputs: the caller's arrays or host-call staging
   T* gs_tr[9];
   if (mem == ELA_MEM_DEVICE) {
-    for (int c = 0; c < 9; ++c) gs_tr[c] = (T*)gs_out->tr[c];
+    for (int c = 0; c < 9; ++c) {
+      if (sp_in->n[kCompGrid[c]]) CHK(check_dev_ptr(gs_out->tr[c], "a source-gradient trace array"));
+      gs_tr[c] = (T*)gs_out->tr[c];
+    }
   } else {
     CHK(h->stage2.alloc(trace_elems(sp_in, nts) * h->esize + 9 * 256));
     size_t off = 0;
@@ -1392,6 +1475,38 @@ int ela_gradient_backward(ela_handle* h, const ela_points* sp, const ela_traces*
                        : grad_backward<double>(h, sp, st, rp, adj, grad8, src_grad, mem);
 }
 
+int ela_set_windows(ela_handle* h, int n, const int64_t* y0, const int64_t* ny, const int64_t* nx) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  if (n < 0 || (n > 0 && (!y0 || !ny || !nx))) return fail(ELA_ERR_ARG, "bad window list");
+  DeviceGuard dg(h->cfg.device);
+  h->win_y0.clear(); h->win_ny.clear(); h->win_nx.clear();
+  h->chunks_yc = 0;
+  h->grad_ready = false;
+  if (n == 0) return ELA_OK;
+  const int64_t NY = h->cfg.ny;
+  std::vector<int> tab(4 * NY, 0);
+  int64_t prev_end = 0;
+  for (int w = 0; w < n; ++w) {
+    if (y0[w] < prev_end || ny[w] < 2 * ela::FAT + 1 || y0[w] + ny[w] > NY || nx[w] < 2 * ela::FAT + 1 ||
+        nx[w] > h->cfg.nx)
+      return fail(ELA_ERR_ARG, "window %d (y0 %lld, ny %lld, nx %lld) overlaps, is too small or leaves the grid", w,
+                  (long long)y0[w], (long long)ny[w], (long long)nx[w]);
+    if (std::min(nx[w], h->cfg.nz) / 2 < h->cfg.min_pad || ny[w] / 2 < h->cfg.min_pad)
+      return fail(ELA_ERR_ARG, "window %d is narrower than twice the sponge", w);
+    for (int64_t y = y0[w]; y < y0[w] + ny[w]; ++y) {
+      tab[4 * y] = (int)y0[w];
+      tab[4 * y + 1] = (int)ny[w];
+      tab[4 * y + 2] = (int)nx[w];
+    }
+    prev_end = y0[w] + ny[w];
+    h->win_y0.push_back(y0[w]); h->win_ny.push_back(ny[w]); h->win_nx.push_back(nx[w]);
+  }
+  CHK(h->ytab.alloc(tab.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(h->ytab.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+  CHK(ensure_chunks(h));
+  return ELA_OK;
+}
+
 int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {
   if (!h) return fail(ELA_ERR_ARG, "null handle");
   if (tile_x != 4 && tile_x != 8 && tile_x != 16) return fail(ELA_ERR_ARG, "tile_x must be 4, 8 or 16");
@@ -1401,9 +1516,12 @@ int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {
   return ELA_OK;
 }
 
-int ela_set_tile_order(ela_handle* h, int xcd_contiguous) {
+int ela_set_tile_order(ela_handle* h, int order, int x_group) {
   if (!h) return fail(ELA_ERR_ARG, "null handle");
-  h->remap = xcd_contiguous ? 1 : 0;
+  if (order < 0 || order > 2) return fail(ELA_ERR_ARG, "tile order must be 0, 1 or 2");
+  if (x_group < 1) return fail(ELA_ERR_ARG, "x_group must be >= 1");
+  h->remap = order;
+  h->xgroup = x_group;
   return ELA_OK;
 }
 

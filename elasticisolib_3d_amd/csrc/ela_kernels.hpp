@@ -72,7 +72,9 @@ struct StepArgs {
   int64_t ys;           // nx * nzi
   int ychunk;
   int nbz, nbx, nby;    // tile counts along z, x, y-chunks
-  int remap;            // deal tiles to XCDs in contiguous ranges
+  int remap;            // tile order (see tile_coord)
+  int xgroup;           // x columns per group of the y-major order
+  const int* chunks;    // stacked windows: per y chunk {y0, y1, window y0, window ny, window nx}, or null
 };
 
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
@@ -81,10 +83,12 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// Sponge factor from the distance to the nearest face of the cell's grid (or
+// of its window when several windows are stacked along y).
 template <typename T>
-__device__ __forceinline__ T damp_w(const StepArgs<T>& a, int iz, int ix, int iy) {
-  int d = min(min(iz - FAT, ix - FAT), min(a.nz - iz - 1 - FAT, a.nx - ix - 1 - FAT));
-  d = min(d, min(iy - FAT, a.ny - iy - 1 - FAT));
+__device__ __forceinline__ T damp_w(const StepArgs<T>& a, int iz, int ix, int iy, int wy0, int wny, int wnx) {
+  int d = min(min(iz - FAT, ix - FAT), min(a.nz - iz - 1 - FAT, wnx - ix - 1 - FAT));
+  d = min(d, min(iy - wy0 - FAT, wy0 + wny - iy - 1 - FAT));
   return d < a.min_pad ? a.taper[d] : T(1);
 }
 
@@ -222,6 +226,8 @@ struct VelVal {
 // exactly two (FP32) lines and its z halos are the adjacent lines.
 struct TileCoord {
   int tz, ty, iz, izo, ix, y0, y1, z0, x0;   // iz internal, izo reference z
+  int wy0, wny, wnx;       // the tile's window: first plane, planes, x extent (whole grid without windows)
+  bool empty;              // no interior cell in the tile (its window is narrower than the tile's x)
   bool okc, active;        // own column inside the grid / inside the interior
   bool xh, okxl, okxh;     // x-halo loader (ty < 4) and its low/high validity
   bool zh, okzh;           // z-halo loader (lanes 0-7) and validity
@@ -229,14 +235,32 @@ struct TileCoord {
   uint32_t col, cxl, cxh, czh;
 };
 
+// Block -> tile. order 0: the hardware's dealing (block b runs on XCD b % 8)
+// over tiles enumerated z fastest, then x, then y. order 1: each XCD gets a
+// contiguous range of that enumeration. order 2: contiguous ranges of a
+// y-major enumeration (per z tile, groups of `xgroup` x columns, y chunks
+// within a group, columns fastest), so that the blocks an XCD runs at the
+// same time are y- and x-neighbours whose halo planes meet in its L2.
 template <int BX>
 __device__ __forceinline__ TileCoord tile_coord(int nbz, int nbx, int nby, int ychunk, int nz, int nzi, int zoff,
-                                                int nx, int ny, bool remap) {
+                                                int nx, int ny, int order, int xgroup, const int* chunks) {
   const int nb = nbz * nbx * nby;
-  const int lb = remap ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
-  const int bz = lb % nbz;
-  const int bx = (lb / nbz) % nbx;
-  const int by = lb / (nbz * nbx);
+  const int lb = order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
+  int bz, bx, by;
+  if (order == 2) {
+    const int per_z = nbx * nby;
+    bz = lb / per_z;
+    const int p = lb - bz * per_z;
+    const int grp = p / (xgroup * nby);
+    const int r = p - grp * xgroup * nby;
+    const int width = min(xgroup, nbx - grp * xgroup);
+    by = r / width;
+    bx = grp * xgroup + (r - by * width);
+  } else {
+    bz = lb % nbz;
+    bx = (lb / nbz) % nbx;
+    by = lb / (nbz * nbx);
+  }
   TileCoord t;
   t.tz = threadIdx.x;
   t.ty = threadIdx.y;
@@ -245,8 +269,16 @@ __device__ __forceinline__ TileCoord tile_coord(int nbz, int nbx, int nby, int y
   t.iz = t.z0 + t.tz;
   t.izo = t.iz - zoff;
   t.ix = t.x0 + t.ty;
-  t.y0 = FAT + by * ychunk;
-  t.y1 = min(t.y0 + ychunk, ny - FAT);
+  if (chunks) {
+    const int* c = chunks + 8 * by;
+    t.y0 = c[0]; t.y1 = c[1]; t.wy0 = c[2]; t.wny = c[3]; t.wnx = c[4];
+  } else {
+    t.y0 = FAT + by * ychunk;
+    t.y1 = min(t.y0 + ychunk, ny - FAT);
+    t.wy0 = 0; t.wny = ny; t.wnx = nx;
+  }
+  nx = t.wnx;
+  t.empty = t.x0 >= nx - FAT;
   t.okc = t.izo < nz && t.ix < nx;
   t.active = (t.izo < nz - FAT) && (t.ix < nx - FAT);
   t.xh = t.ty < FAT;
@@ -284,7 +316,8 @@ template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0;
   __shared__ Plane<T, BX> l_xx[2], l_zz[2], l_xy[2], l_xz[2], l_yz[2];
-  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
+  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
+  if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
   const StressVal<T, MODE> S{a, t.okc}, SXl{a, t.okxl}, SXh{a, t.okxh}, SZ{a, t.okzh};
 
@@ -348,7 +381,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
       const T sx = dmx<T, BX>(l_xx[b], r, q, cx) + dq(qxy, cy) + dpz<T, BX>(l_xz[b], r, q, cz);
       const T sy = dpx<T, BX>(l_xy[b], r, q, cx) + dq(qyy, cy) + dpz<T, BX>(l_yz[b], r, q, cz);
       const T sz = dpx<T, BX>(l_xz[b], r, q, cx) + dq(qyz, cy) + dmz<T, BX>(l_zz[b], r, q, cz);
-      const T w = damp_w(a, t.izo, t.ix, iy);
+      const T w = damp_w(a, t.izo, t.ix, iy, t.wy0, t.wny, t.wnx);
       const T s3[3] = {sx, sy, sz};
       if (F & F_STORE) {
 #pragma unroll
@@ -394,7 +427,8 @@ template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0;
   __shared__ Plane<T, BX> l_vx[2], l_vy[2], l_vz[2];
-  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
+  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
+  if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
   const VelVal<T, MODE> V{a, t.okc}, Vlo{a, t.okxl}, Vhi{a, t.okxh}, Vzh{a, t.okzh};
 
@@ -460,7 +494,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       const T gxy = dq(qx, cy) + dmx<T, BX>(l_vy[b], r, q, cx);
       const T gxz = dmz<T, BX>(l_vx[b], r, q, cz) + dmx<T, BX>(l_vz[b], r, q, cx);
       const T gyz = dq(qz, cy) + dmz<T, BX>(l_vy[b], r, q, cz);
-      const T w = damp_w(a, t.izo, t.ix, iy);
+      const T w = damp_w(a, t.izo, t.ix, iy, t.wy0, t.wny, t.wnx);
       if (F & F_STORE) {
         st_nt(a.dst[SXX], i, exx); st_nt(a.dst[SYY], i, eyy); st_nt(a.dst[SZZ], i, ezz);
         st_nt(a.dst[SXZ], i, gxz); st_nt(a.dst[SXY], i, gxy); st_nt(a.dst[SYZ], i, gyz);
@@ -542,6 +576,7 @@ struct PointArgs {
   // damping (injection only)
   const T* taper;
   int min_pad, nz, nx, ny, nzi, zoff;
+  const int* ytab;         // stacked windows: per plane {window y0, window ny, window nx, 0}, or null
 };
 
 template <typename T>
@@ -559,8 +594,10 @@ __device__ __forceinline__ T point_taper(const PointArgs<T>& a, int32_t p) {
   const int rem = p - iy * ys;
   const int ix = rem / a.nzi;
   const int iz = rem - ix * a.nzi - a.zoff;
-  int d = min(min(iz - FAT, ix - FAT), min(a.nz - iz - 1 - FAT, a.nx - ix - 1 - FAT));
-  d = min(d, min(iy - FAT, a.ny - iy - 1 - FAT));
+  int wy0 = 0, wny = a.ny, wnx = a.nx;
+  if (a.ytab) { wy0 = a.ytab[4 * iy]; wny = a.ytab[4 * iy + 1]; wnx = a.ytab[4 * iy + 2]; }
+  int d = min(min(iz - FAT, ix - FAT), min(a.nz - iz - 1 - FAT, wnx - ix - 1 - FAT));
+  d = min(d, min(iy - wy0 - FAT, wy0 + wny - iy - 1 - FAT));
   return d < a.min_pad ? a.taper[d] : T(1);
 }
 
@@ -651,13 +688,15 @@ struct ImagingArgs {
   int nz, nx, ny, nzi, zoff;
   int64_t ys;
   int ychunk, nbz, nbx, nby;
-  int remap;
+  int remap, xgroup;
+  const int* chunks;
 };
 
 template <typename T, int BX, bool ADJ>
 __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
   __shared__ Plane<T, BX> l_vx, l_vy, l_vz;
-  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap);
+  const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
+  if (t.empty) return;  // block-uniform, before any barrier
   const int64_t ys = a.ys;
   const bool ok = t.okc;
   const bool active = t.active;
@@ -745,15 +784,19 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
 // reported through *bad.
 __global__ void __launch_bounds__(256) k_convert_pos(const int64_t* __restrict__ in, int32_t* __restrict__ out,
                                                      int64_t n, int nz, int nx, int ny, int nzi, int zoff,
-                                                     int* bad) {
+                                                     const int* ytab, int* bad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t p = in[i];
   const int64_t ys = (int64_t)nz * nx;
-  const int64_t iy = p >= 0 ? p / ys : -1;
+  const int64_t iy = (p >= 0 && p < ys * ny) ? p / ys : -1;
   const int64_t r = p - iy * ys;
   const int64_t ix = r / nz, iz = r - ix * nz;
-  const bool ok = p >= 0 && iy >= FAT && iy < ny - FAT && ix >= FAT && ix < nx - FAT && iz >= FAT && iz < nz - FAT;
+  bool ok = iy >= FAT && iy < ny - FAT && ix >= FAT && ix < nx - FAT && iz >= FAT && iz < nz - FAT;
+  if (ok && ytab) {  // inside the interior of its window
+    const int wy0 = ytab[4 * iy], wny = ytab[4 * iy + 1], wnx = ytab[4 * iy + 2];
+    ok = wny > 0 && iy >= wy0 + FAT && iy < wy0 + wny - FAT && ix < wnx - FAT;
+  }
   out[i] = ok ? (int32_t)(iy * nx * nzi + ix * nzi + iz + zoff) : -1;
   if (!ok) atomicOr(bad, 1);
 }

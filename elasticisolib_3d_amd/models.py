@@ -202,6 +202,82 @@ def born_layered_400(nts: int = 501, seed: int = 2024, n_rec_axis: int = 150, si
                     ricker(nts, dts, 10.0, 0.1), "explosive", extra={"dm3": dmp})
 
 
+def fwi_256(nts: int = 501, seed: int = 2025, n_shot_axis: int = 2, rec_step: int = 4, sigma_start: float = 20.0):
+    """FWI workload: true model = the layered recipe on 256 x 256 x 160 cells
+    (x, y, z), d = 15 m, 30-cell sponge; starting model = the true one
+    smoothed with a sigma_start-cell Gaussian; 2 x 2 explosive shots at
+    z = 10 cells (8 Hz Ricker, delay 0.125 s), 2.0 s = 501 samples of 4 ms
+    with 8 fine steps each; a seeded surface receiver grid every `rec_step`
+    cells (jittered by up to a quarter cell). Returns (true Workload, start
+    (rho, lambda, mu) on the padded grid, list of source positions)."""
+    from scipy.ndimage import gaussian_filter
+    nxm, nym, nzm, d, pad = 256, 256, 160, 15.0, 30
+    vp, vs, rho0 = layered_model(nzm, nxm, nym, seed, sigma=8.0)
+    vp_s = gaussian_filter(vp.astype(np.float32), sigma_start, mode="nearest").astype(np.float64)
+    vs_s = gaussian_filter(vs.astype(np.float32), sigma_start, mode="nearest").astype(np.float64)
+    rho_s = gaussian_filter(rho0.astype(np.float32), sigma_start, mode="nearest").astype(np.float64)
+    true = np.stack(vpvsrho_to_rholamemu(vp, vs, rho0))
+    start = np.stack(vpvsrho_to_rholamemu(vp_s, vs_s, rho_s))
+    del vp, vs, rho0, vp_s, vs_s, rho_s
+    mt, pads = pad_model(true, pad, pad, pad)
+    ms, _ = pad_model(start, pad, pad, pad)
+    ny, nx, nz = mt.shape[1:]
+    ax = _axes(nz, nx, ny, d, pads)
+    dts = 0.004
+    rng = np.random.default_rng(seed + 1)
+    xs = np.arange(0, nxm - 1, rec_step) * d
+    gx, gy = np.meshgrid(xs, xs, indexing="ij")
+    jit = rng.uniform(-0.25, 0.25, (2,) + gx.shape) * d
+    rx = np.clip(gx + jit[0], 0, (nxm - 2) * d)
+    ry = np.clip(gy + jit[1], 0, (nym - 2) * d)
+    rec = np.stack([rx.ravel(), ry.ravel(), np.zeros(rx.size)], axis=1)
+    shots = []
+    for i in range(n_shot_axis):
+        for j in range(n_shot_axis):
+            shots.append(np.array([[(i + 0.5) * nxm / n_shot_axis * d, (j + 0.5) * nym / n_shot_axis * d, 10 * d]]))
+    w = Workload("fwi_256x256x160", mt[0], mt[1], mt[2], ax, nts, dts, 8, 0.99, shots[0], rec,
+                 ricker(nts, dts, 8.0, 0.125), "explosive", extra={"start": ms, "shots": shots})
+    return w
+
+
+def batch_layered_400(nts: int = 626, seed: int = 2024, seed_geom: int = 11, sigma_start: float = 20.0,
+                      n_axis: int = 8, pitch: int = 50, win_range=(180, 300), rec_range=(60, 200)):
+    """Multi-shot workload with shot-local windows on the 400 x 400 x 250
+    layered model (d = 12.5 m, 30-cell sponge around every window): 8 x 8
+    shots on a jittered 50-cell surface grid (explosive 10 Hz Ricker at
+    z = 10 cells), each with a window of U(180, 300) cells per horizontal axis
+    centred on the shot (clipped to the model) and U(60, 200) receivers
+    uniformly placed at the surface inside its window; 2.5 s = 626 samples of
+    4 ms, 8 fine steps each. Returns a dict with the true and starting
+    (rho, lambda, mu) [ny, nx, nz] (start: the true model smoothed with a
+    sigma_start-cell Gaussian) and the shots (src_xyz, rec_xyz, window
+    (x0, y0, nx, ny))."""
+    from scipy.ndimage import gaussian_filter
+    nxm, nym, nzm, d = 400, 400, 250, 12.5
+    vp, vs, rho0 = layered_model(nzm, nxm, nym, seed, sigma=8.0)
+    true = np.stack(vpvsrho_to_rholamemu(vp, vs, rho0))
+    sm = [gaussian_filter(a.astype(np.float32), sigma_start, mode="nearest").astype(np.float64) for a in (vp, vs, rho0)]
+    start = np.stack(vpvsrho_to_rholamemu(*sm))
+    del vp, vs, rho0, sm
+    rng = np.random.default_rng(seed_geom)
+    shots = []
+    for i in range(n_axis):
+        for j in range(n_axis):
+            sx = (i + 0.5) * (nxm / n_axis) + rng.uniform(-0.25, 0.25) * pitch
+            sy = (j + 0.5) * (nym / n_axis) + rng.uniform(-0.25, 0.25) * pitch
+            wx, wy = rng.integers(win_range[0], win_range[1] + 1, 2)
+            x0 = int(np.clip(round(sx - wx / 2), 0, nxm - wx))
+            y0 = int(np.clip(round(sy - wy / 2), 0, nym - wy))
+            nrec = int(rng.integers(rec_range[0], rec_range[1] + 1))
+            rx = rng.uniform(x0 + 1, x0 + wx - 3, nrec) * d
+            ry = rng.uniform(y0 + 1, y0 + wy - 3, nrec) * d
+            rec = np.stack([rx, ry, np.zeros(nrec)], axis=1)
+            src = np.array([[sx * d, sy * d, 10 * d]])
+            shots.append((src, rec, (x0, y0, int(wx), int(wy))))
+    return {"true": true, "start": start, "shots": shots, "d": d, "pad": 30, "nts": nts, "dts": 0.004, "sub": 8,
+            "alpha_cos": 0.99, "wavelet": ricker(nts, 0.004, 10.0, 0.1)}
+
+
 def source_traces(w: Workload) -> List[np.ndarray]:
     """9 irregular source traces [n_src, nts] for the workload's source type."""
     n = w.src_xyz.shape[0]

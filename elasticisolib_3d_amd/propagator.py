@@ -88,15 +88,27 @@ class Propagator:
             raise ValueError("host arrays must be C-contiguous with the handle dtype")
         return ctypes.c_void_p(x.ctypes.data)
 
-    def _points(self, pos: Sequence) -> ElaPoints:
+    def _points(self, pos: Sequence, mem: int) -> ElaPoints:
+        """7 position lists in the memory kind of the call: host arrays are
+        moved to the GPU for a device-memory call and device tensors to the
+        host for a host-memory call (the library reads device-call positions
+        on the GPU)."""
         if len(pos) != 7:
             raise ValueError("need 7 position lists (centre X Y Z XZ XY YZ)")
         p = ElaPoints()
         for g in range(7):
             a = pos[g]
-            if not _is_torch(a):
+            if mem == _lib.ELA_MEM_DEVICE:
+                if not (_is_torch(a) and a.is_cuda):
+                    import torch
+                    a = torch.as_tensor(np.ascontiguousarray(a, dtype=np.int64), device="cuda")
+                elif not a.is_contiguous():
+                    a = a.contiguous()
+            else:
+                if _is_torch(a):
+                    a = a.detach().cpu().numpy()
                 a = np.ascontiguousarray(a, dtype=np.int64)
-                self._keep.append(a)
+            self._keep.append(a)
             n = a.shape[0] if len(a.shape) else 0
             p.n[g] = int(n)
             p.pos[g] = self._ptr(a, np.int64).value if n else None
@@ -136,7 +148,7 @@ class Propagator:
 
     def nl_forward(self, src_pos, src_traces, rec_pos, rec_out=None):
         mem = self._mem_of(*src_traces, *(rec_out or []))
-        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)
         rc = self._counts(rec_pos)
         if mem == _lib.ELA_MEM_HOST:
             src_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in src_traces]
@@ -149,7 +161,7 @@ class Propagator:
 
     def nl_adjoint(self, src_pos, rec_traces, rec_pos, src_out=None):
         mem = self._mem_of(*rec_traces, *(src_out or []))
-        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)
         sc = self._counts(src_pos)
         if mem == _lib.ELA_MEM_HOST:
             rec_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in rec_traces]
@@ -167,7 +179,7 @@ class Propagator:
             dmod8 = np.ascontiguousarray(dmod8, dtype=self.dtype)
         if tuple(dmod8.shape) != (8,) + self.shape:
             raise ValueError("perturbation must have shape (8, ny, nx, nz)")
-        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)
         rc = self._counts(rec_pos)
         rec = rec_out if rec_out is not None else self._alloc_traces(rc, mem == _lib.ELA_MEM_DEVICE)
         st, rt = self._traces(src_traces, self._counts(src_pos)), self._traces(rec, rc)
@@ -187,7 +199,7 @@ class Propagator:
         elif dmod8 is None:
             import torch
             dmod8 = torch.zeros((8,) + self.shape, dtype=src_traces[0].dtype, device="cuda")
-        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)
         st, rt = self._traces(src_traces, self._counts(src_pos)), self._traces(rec_traces, self._counts(rec_pos))
         check(self.lib.ela_born_adjoint(self.h, ctypes.byref(sp), ctypes.byref(st), self._ptr(dmod8, self.dtype),
                                         ctypes.byref(rp), ctypes.byref(rt), mem), "ela_born_adjoint")
@@ -197,7 +209,7 @@ class Propagator:
     def gradient_forward(self, src_pos, src_traces, rec_pos, rec_out=None):
         """Nonlinear forward that keeps checkpoints for gradient_backward."""
         mem = self._mem_of(*src_traces, *(rec_out or []))
-        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)
         rc = self._counts(rec_pos)
         if mem == _lib.ELA_MEM_HOST:
             src_traces = [np.ascontiguousarray(s, dtype=self.dtype) for s in src_traces]
@@ -225,7 +237,7 @@ class Propagator:
             grad8 = torch.zeros((8,) + self.shape, dtype=torch.float64, device="cuda")
         if tuple(grad8.shape) != (8,) + self.shape:
             raise ValueError("grad8 must have shape (8, ny, nx, nz)")
-        sp, rp = self._points(src_pos), self._points(rec_pos)
+        sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)
         sc = self._counts(src_pos)
         gs = src_grad_out if src_grad_out is not None else self._alloc_traces(sc, mem == _lib.ELA_MEM_DEVICE)
         st, at, gt = self._traces(src_traces, sc), self._traces(adj_traces, self._counts(rec_pos)), self._traces(gs, sc)
@@ -247,11 +259,23 @@ class Propagator:
                                               ctypes.byref(nb)), "ela_gradient_plan_used")
         return k.value, k1.value, bool(b.value), nb.value
 
+    def set_windows(self, y0, ny, nx) -> None:
+        """Stack windows along y (see ela_set_windows); empty lists remove them."""
+        y0 = np.ascontiguousarray(y0, np.int64)
+        ny = np.ascontiguousarray(ny, np.int64)
+        nx = np.ascontiguousarray(nx, np.int64)
+        if not (len(y0) == len(ny) == len(nx)):
+            raise ValueError("y0, ny, nx must have the same length")
+        check(self.lib.ela_set_windows(self.h, len(y0), y0.ctypes.data, ny.ctypes.data, nx.ctypes.data),
+              "ela_set_windows")
+
     def set_tiling(self, tile_x: int, y_chunk: int) -> None:
         check(self.lib.ela_set_tiling(self.h, int(tile_x), int(y_chunk)), "ela_set_tiling")
 
-    def set_tile_order(self, xcd_contiguous: bool) -> None:
-        check(self.lib.ela_set_tile_order(self.h, int(bool(xcd_contiguous))), "ela_set_tile_order")
+    def set_tile_order(self, order: int, x_group: int = 1) -> None:
+        """0 hardware order, 1 contiguous per XCD, 2 y-major per XCD in groups
+        of x_group columns (speed only)."""
+        check(self.lib.ela_set_tile_order(self.h, int(order), int(x_group)), "ela_set_tile_order")
 
     def set_born_storage(self, segment: int = 0, budget_bytes: int = 0) -> None:
         """Coarse-wavefield storage of the Born adjoint: 0 automatic (within

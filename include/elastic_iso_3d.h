@@ -246,15 +246,30 @@ int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const double* z,
  * interior cells it advanced. */
 int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_t* cells);
 
+/* Several grids ("windows": e.g. the shot-local windows of a batch of
+ * shots) stacked along y in the handle's nz x nx x ny grid: window w occupies
+ * planes [y0[w], y0[w] + ny[w]) and x columns [0, nx[w]); windows do not
+ * overlap and each has its own FAT layers and sponge (distances are measured
+ * to the window's own faces), so that every propagation advances all windows
+ * in the same launches and each window's result equals a propagation on that
+ * window alone. Point positions must lie in a window's interior. The caller
+ * stacks media, perturbations and positions (the Python WindowStack does).
+ * n = 0 removes the windows. No counterpart in the reference (its shot-window
+ * option `ginsu` is read but unused, nonlinearPropElasticShotsGpu_3D.cpp:12). */
+int ela_set_windows(ela_handle* h, int n, const int64_t* y0, const int64_t* ny, const int64_t* nx);
+
 /* Tile shape of the stencil kernels: x extent of a 64-wide z tile (4, 8 or
  * 16 rows) and the number of y planes one block marches through. Any
  * setting gives the same result; it only changes speed. */
 int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk);
 int ela_get_tiling(const ela_handle* h, int* tile_x, int* y_chunk);
-/* Block-to-tile order: 0 (default) keeps the hardware's round-robin
- * dealing of blocks to the 8 XCDs, 1 gives each XCD a contiguous range of
- * tiles. Speed only. */
-int ela_set_tile_order(ela_handle* h, int xcd_contiguous);
+/* Block-to-tile order of the stencil launches (speed only): 0 (default)
+ * the hardware's round-robin dealing of blocks to the 8 XCDs over tiles
+ * enumerated z, x, y; 1 each XCD a contiguous range of that enumeration; 2
+ * each XCD a contiguous range of a y-major enumeration in groups of x_group
+ * x columns, so that concurrently running neighbours share halo planes in
+ * the XCD's L2. */
+int ela_set_tile_order(ela_handle* h, int order, int x_group);
 
 /* Diagnostic timing: launch the velocity and the stress stencil kernels of a
  * forward fine step `reps` times each on the handle's current wavefields and

@@ -198,3 +198,20 @@ def test_device_outputs_ordered_with_default_stream(problems):
             assert abs(n - ref_norm) <= 1e-10 * ref_norm, (n, ref_norm)
         else:
             assert abs(n - ref_norm) > 1e-6 * ref_norm
+
+
+@pytest.mark.parametrize("order,xgroup", [(1, 1), (2, 1), (2, 3), (2, 64)])
+def test_tile_orders_give_identical_results(problems, order, xgroup):
+    """Every block-to-tile order covers every tile exactly once: the results
+    are bit-identical to the hardware order, with and without partial x
+    groups, for several tile shapes."""
+    g, P = GRIDS["odd"], problems["odd"]
+    prop = make_prop(g, "f32")
+    prop.set_medium(P["med"])
+    for tx, yc in ((16, 8), (8, 3), (4, 5)):
+        prop.set_tiling(tx, yc)
+        prop.set_tile_order(0, 1)
+        ref = prop.nl_forward(P["spos"], P["src"], P["rpos"])
+        prop.set_tile_order(order, xgroup)
+        out = prop.nl_forward(P["spos"], P["src"], P["rpos"])
+        assert all(np.array_equal(a, b) for a, b in zip(out, ref)), (tx, yc)

@@ -165,3 +165,56 @@ def test_medium_gradient_is_transposed_jacobian(lib):
         lhs = np.sum(jd * g8)
         rhs = np.sum(d * g3[k])
         assert abs(lhs - rhs) <= 1e-7 * abs(rhs), (k, lhs, rhs)
+
+
+def test_autotuner_picks_fastest_and_caches(tmp_path, monkeypatch):
+    """The autotuner applies the candidate with the smallest measured step
+    time and reuses it from its cache for the same grid."""
+    from elasticisolib_3d_amd import autotune as A
+    monkeypatch.setenv("ELA_TUNE_CACHE", str(tmp_path / "t.json"))
+
+    class FakeProp:
+        precision = "f32"
+        shape = (40, 50, 60)
+
+        def __init__(self):
+            self.calls = 0
+            self.tiling, self.order = (16, 8), (0, 1)
+
+        def set_tiling(self, tx, yc):
+            self.tiling = (tx, yc)
+
+        def set_tile_order(self, o, g):
+            self.order = (o, g)
+
+        def time_step_kernels(self, reps):
+            self.calls += 1
+            tx, yc = self.tiling
+            o, g = self.order
+            t = 1.0 + 0.1 * abs(yc - 16) + 0.05 * (16 - tx) + (0.0 if (o, g) == (2, 2) else 0.02)
+            return t / 2, t / 2
+
+    p = FakeProp()
+    best = A.autotune(p)
+    assert (best["tile_x"], best["y_chunk"], best["order"], best["x_group"]) == (16, 16, 2, 2)
+    assert p.tiling == (16, 16) and p.order == (2, 2) and not best["cached"]
+    q = FakeProp()
+    again = A.autotune(q)
+    assert again["cached"] and q.calls == 0 and q.tiling == (16, 16) and q.order == (2, 2)
+
+
+def test_positions_follow_the_memory_kind_of_the_call():
+    """Position lists are handed to the library in the memory kind of the
+    call (host here: torch CPU tensors become numpy arrays), so that a
+    device-memory call never receives host position pointers."""
+    import torch
+    from elasticisolib_3d_amd import _lib
+    from elasticisolib_3d_amd.propagator import Propagator
+    prop = Propagator.__new__(Propagator)
+    prop._keep = []
+    pos = [torch.arange(3, dtype=torch.int64), np.arange(2), [], [5], np.zeros(0, np.int64), [1, 2], [7]]
+    p = prop._points(pos, _lib.ELA_MEM_HOST)
+    assert [p.n[g] for g in range(7)] == [3, 2, 0, 1, 0, 2, 1]
+    assert all(isinstance(a, np.ndarray) and a.dtype == np.int64 for a in prop._keep)
+    first = np.ctypeslib.as_array(ctypes.cast(p.pos[0], ctypes.POINTER(ctypes.c_int64)), shape=(3,))
+    assert list(first) == [0, 1, 2]
