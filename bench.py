@@ -449,6 +449,8 @@ def main():
     ap.add_argument("--nts", type=int, default=626, help="coarse samples (626 = 2.5 s)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for counter calibration)")
     ap.add_argument("--workload", choices=("nl_forward", "born", "fp64_check", "fwi", "fwi_batch"),
@@ -494,6 +496,14 @@ def main():
     w = layered_400(nts=args.nts)
     log(f"[bench] model {w.name} padded {w.axes.nz}x{w.axes.nx}x{w.axes.ny} built in {time.time() - t0:.1f}s")
     prop, spos, src, rpos, rec = gpu_setup(w)
+    tuned = None
+    if not args.no_autotune:
+        from elasticisolib_3d_amd.autotune import autotune
+        t0 = time.time()
+        tuned = autotune(prop, reps=10)
+        log(f"[bench] autotune: tile {tuned['tile_x']}x{tuned['y_chunk']} order {tuned['order']}/{tuned['x_group']} "
+            f"({tuned['ms_step']:.4f} ms/step, {'cached' if tuned['cached'] else f'{time.time() - t0:.1f}s'})")
+    prop.set_graph(not args.no_graph)
 
     for i in range(args.warmup):
         prop.nl_forward(spos, src, rpos, rec_out=rec)
@@ -541,6 +551,9 @@ def main():
             "event_loop_ms_per_step": round(ev_ms / args.steps, 2),
             "data_finite": finite,
             "data_l2": norm,
+            "launch": {"tile_x": prop.get_tiling()[0], "y_chunk": prop.get_tiling()[1],
+                       "order": None if tuned is None else [tuned["order"], tuned["x_group"]],
+                       "autotuned": tuned is not None, "graph": not args.no_graph},
         },
         "roofline": {
             "bound": "hbm",

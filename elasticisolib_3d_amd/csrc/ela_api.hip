@@ -160,6 +160,10 @@ struct ela_handle {
   DevBuf chunks;            // per y chunk {y0, y1, window y0, ny, nx, 0, 0, 0} (int32)
   int chunks_yc = 0;        // y chunk the table was built for
   int nchunks = 0;
+  // graph replay of the coarse step of the nonlinear propagations
+  bool use_graph = false;
+  DevBuf counter;           // device coarse-step counter (one int)
+  hipStream_t cap_stream = nullptr;
 };
 
 namespace {
@@ -553,28 +557,71 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
     for (int c = 0; c < 9; ++c) CHK(fill<T>(h, src_tr[c], sp.n[kCompGrid[c]] * nts));
   }
 
-  HIPCHK(hipEventRecord(h->ev0, h->stream));
   const double* f = h->filt.data();
-  if (!adjoint) {
-    for (int64_t its = 0; its < nts - 1; ++its)
+  // One coarse step: sub fine steps (stepper, injection, recording). With
+  // its_dev the coarse index is read from device memory (graph replays).
+  auto coarse_step = [&](int64_t its, const int* its_dev) -> int {
+    if (!adjoint) {
       for (int64_t it2 = 1; it2 <= sub; ++it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
         CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a)));
-        launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), (int)its, f[it2 - 1],
-                                f[hh + it2 - 1]);
-        launch_record<T>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2], f[hh + it2]);
+        ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
+        ela::PointArgs<T> R = make_points<T>(h, rp, w.p0, tmv, true, nrec_tot);
+        S.its_dev = R.its_dev = its_dev;
+        launch_inject<T, false>(h, S, (int)its, f[it2 - 1], f[hh + it2 - 1]);
+        launch_record<T>(h, R, (int)its, f[it2], f[hh + it2]);
         w.swap();
       }
-  } else {
-    for (int64_t its = nts - 2; its >= 0; --its)
+    } else {
       for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
         CHK((launch_step<T, ela::MODE_ADJ, 0, 0>(h, a)));
-        launch_inject<T, false>(h, make_points<T>(h, rp, w.p0, tmv, true, nrec_tot), (int)its, f[it2 + 1],
-                                f[hh + it2 + 1]);
-        launch_record<T>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), (int)its, f[it2], f[hh + it2]);
+        ela::PointArgs<T> R = make_points<T>(h, rp, w.p0, tmv, true, nrec_tot);
+        ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
+        S.its_dev = R.its_dev = its_dev;
+        launch_inject<T, false>(h, R, (int)its, f[it2 + 1], f[hh + it2 + 1]);
+        launch_record<T>(h, S, (int)its, f[it2], f[hh + it2]);
         w.swap();
       }
+    }
+    return ELA_OK;
+  };
+  // sub is even (the doubled sub-sampling) whenever the graph is used, so the
+  // wavefield pointers are back in place after one coarse step and the same
+  // captured step serves every coarse sample
+  const bool graph = h->use_graph && sub % 2 == 0;
+  hipGraphExec_t exec = nullptr;
+  if (graph) {
+    CHK(ensure_chunks(h));  // no synchronous upload inside the capture
+    CHK(h->counter.alloc(256));
+    int* cnt = (int*)h->counter.p;
+    hipLaunchKernelGGL((ela::k_fill<int>), dim3(1), dim3(64), 0, h->stream, cnt, (int64_t)1,
+                       adjoint ? (int)(nts - 2) : 0);
+    if (!h->cap_stream) HIPCHK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+    hipStream_t keep = h->stream;
+    h->stream = h->cap_stream;
+    int rc = coarse_step(0, cnt);
+    if (rc == ELA_OK) hipLaunchKernelGGL(ela::k_count, dim3(1), dim3(64), 0, h->stream, cnt, adjoint ? -1 : 1);
+    h->stream = keep;
+    const hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
+    if (rc != ELA_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (ec != hipSuccess) return fail(ELA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(ELA_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+  }
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  if (graph) {
+    hipError_t e = hipSuccess;
+    for (int64_t k = 0; k < nts - 1 && e == hipSuccess; ++k) e = hipGraphLaunch(exec, h->stream);
+    (void)hipGraphExecDestroy(exec);
+    if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
+  } else if (!adjoint) {
+    for (int64_t its = 0; its < nts - 1; ++its) CHK(coarse_step(its, nullptr));
+  } else {
+    for (int64_t its = nts - 2; its >= 0; --its) CHK(coarse_step(its, nullptr));
   }
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   HIPCHK(hipGetLastError());
@@ -1377,6 +1424,7 @@ int ela_destroy(ela_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
   return ELA_OK;
@@ -1504,6 +1552,12 @@ int ela_set_windows(ela_handle* h, int n, const int64_t* y0, const int64_t* ny, 
   CHK(h->ytab.alloc(tab.size() * sizeof(int)));
   HIPCHK(hipMemcpy(h->ytab.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
   CHK(ensure_chunks(h));
+  return ELA_OK;
+}
+
+int ela_set_graph(ela_handle* h, int on) {
+  if (!h) return fail(ELA_ERR_ARG, "null handle");
+  h->use_graph = on != 0;
   return ELA_OK;
 }
 

@@ -577,6 +577,7 @@ struct PointArgs {
   const T* taper;
   int min_pad, nz, nx, ny, nzi, zoff;
   const int* ytab;         // stacked windows: per plane {window y0, window ny, window nx, 0}, or null
+  const int* its_dev;      // coarse sample index read from device memory (graph replays), or null
 };
 
 template <typename T>
@@ -612,8 +613,9 @@ __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   const int64_t k = g - a.off[c];
   const int32_t p = a.pos[c][k];
   if (p < 0) return;
+  const int its = a.its_dev ? *a.its_dev : a.its;
   const T* tr = a.tr[c] + k * a.tr_stride_pt;
-  const T s = tr[(int64_t)a.its * a.tr_stride_t] * a.fa + tr[(int64_t)(a.its + 1) * a.tr_stride_t] * a.fb;
+  const T s = tr[(int64_t)its * a.tr_stride_t] * a.fa + tr[(int64_t)(its + 1) * a.tr_stride_t] * a.fb;
   const T v = point_taper(a, p) * s;
   a.field[c][p] += v;
   if (ACC && a.accl[c]) {
@@ -647,9 +649,10 @@ __global__ void __launch_bounds__(256) k_record(PointArgs<T> a) {
   const int32_t p = a.pos[c][k];
   if (p < 0) return;
   const T v = a.field[c][p];
+  const int its = a.its_dev ? *a.its_dev : a.its;
   T* tr = a.tr[c] + k * a.tr_stride_pt;
-  tr[(int64_t)a.its * a.tr_stride_t] += v * a.fa;
-  tr[(int64_t)(a.its + 1) * a.tr_stride_t] += v * a.fb;
+  tr[(int64_t)its * a.tr_stride_t] += v * a.fa;
+  tr[(int64_t)(its + 1) * a.tr_stride_t] += v * a.fb;
 }
 
 // Out-of-place transpose of a [rows][cols] block (row stride ld_in) into
@@ -799,6 +802,11 @@ __global__ void __launch_bounds__(256) k_convert_pos(const int64_t* __restrict__
   }
   out[i] = ok ? (int32_t)(iy * nx * nzi + ix * nzi + iz + zoff) : -1;
   if (!ok) atomicOr(bad, 1);
+}
+
+// Device-side coarse-step counter of a replayed graph: *c += delta.
+__global__ void k_count(int* c, int delta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c += delta;
 }
 
 template <typename T>

@@ -269,6 +269,10 @@ class Propagator:
         check(self.lib.ela_set_windows(self.h, len(y0), y0.ctypes.data, ny.ctypes.data, nx.ctypes.data),
               "ela_set_windows")
 
+    def set_graph(self, on: bool = True) -> None:
+        """Replay the coarse step of nl_forward / nl_adjoint as a HIP graph."""
+        check(self.lib.ela_set_graph(self.h, int(bool(on))), "ela_set_graph")
+
     def set_tiling(self, tile_x: int, y_chunk: int) -> None:
         check(self.lib.ela_set_tiling(self.h, int(tile_x), int(y_chunk)), "ela_set_tiling")
 

@@ -258,6 +258,12 @@ int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_
  * option `ginsu` is read but unused, nonlinearPropElasticShotsGpu_3D.cpp:12). */
 int ela_set_windows(ela_handle* h, int n, const int64_t* y0, const int64_t* ny, const int64_t* nx);
 
+/* Replay the coarse step of ela_nl_forward / ela_nl_adjoint as a HIP graph:
+ * the sub fine steps (stepper, injection and recording launches) are captured
+ * once per call and the graph is launched nts - 1 times, the coarse-sample
+ * index advancing in device memory. Same results; fewer host launches. */
+int ela_set_graph(ela_handle* h, int on);
+
 /* Tile shape of the stencil kernels: x extent of a 64-wide z tile (4, 8 or
  * 16 rows) and the number of y planes one block marches through. Any
  * setting gives the same result; it only changes speed. */
