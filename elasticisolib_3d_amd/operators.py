@@ -15,6 +15,8 @@ working with the MI355X library underneath:
                                                        BornElasticShotsGpu_3D.cpp:143-747
   FdParam (QC, staggering, scaling)                    fdParamElastic_3D.cpp:13-195, 288-355
   ElasticConv_3D / ElasticConvJab_3D                   seis_utils/python/elasticParamConvertModule_3D.py
+  ElasticDatComp_3D                                    seis_utils/python/dataCompModule_3D.py
+  main_nonlinear / main_born (command-line drivers)    nonlinearPythonFloatMain_3D.py, BornPythonFloatMain_3D.py
 
 Vectors are the numpy-backed `sep.Vector` (axis 1 fastest). Layouts are the
 reference's: elastic parameters [3, ny, nx, nz] (rho, lambda, mu); nonlinear
@@ -452,6 +454,53 @@ class ElasticConvJab_3D(Operator):
             self.background.getNdArray()[:] *= 1e3
 
 
+class ElasticDatComp_3D(Operator):
+    """Sampling of data components (seis_utils/python/dataCompModule_3D.py:11-141):
+    elastic data [nExp, 9, nRec, nts] -> the listed components in the listed
+    order (vx vy vz sxx syy szz sxz sxy syz and p = (sxx + syy + szz) / 3).
+    As in the reference, a component given twice raises, and the list is
+    rejected only when it names no known component."""
+
+    NAMES = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxz", "sxy", "syz")
+    P_FACTOR = 1.0 / 3.0
+
+    def __init__(self, components: str, domain: Vector):
+        comps = components.split(",")
+        if len(set(comps)) != len(comps):
+            raise ValueError("ERROR! A component was provided multiple times! Check your input arguments")
+        h = domain.getHyper()
+        rng = Vector(Hypercube([h.getAxis(1), h.getAxis(2), Axis(len(comps), label=f"Component {components}"),
+                                h.getAxis(4)]), dtype=domain.dtype)
+        self.setDomainRange(domain, rng)
+        self.comp_list = [c.lower() for c in comps]
+        if not any(c in self.comp_list for c in self.NAMES + ("p",)):
+            raise ValueError(f"ERROR! Provided unknown data components: {components}")
+
+    def forward(self, add, model, data):
+        self.checkDomainRange(model, data)
+        if not add:
+            data.zero()
+        m, d = model.getNdArray(), data.getNdArray()
+        for k, name in enumerate(self.NAMES):
+            if name in self.comp_list:
+                d[:, self.comp_list.index(name)] += m[:, k]
+        if "p" in self.comp_list:
+            d[:, self.comp_list.index("p")] += self.P_FACTOR * (m[:, 3] + m[:, 4] + m[:, 5])
+
+    def adjoint(self, add, model, data):
+        self.checkDomainRange(model, data)
+        if not add:
+            model.zero()
+        m, d = model.getNdArray(), data.getNdArray()
+        for k, name in enumerate(self.NAMES):
+            if name in self.comp_list:
+                m[:, k] += d[:, self.comp_list.index(name)]
+        if "p" in self.comp_list:
+            i = self.comp_list.index("p")
+            for k in (3, 4, 5):
+                m[:, k] += self.P_FACTOR * d[:, i]
+
+
 # ---------------------------------------------------------------------------
 # Shot operators
 # ---------------------------------------------------------------------------
@@ -705,3 +754,75 @@ def BornOpInitFloat_3D(args, precision: str = "f32"):
 
 def BornOpInitDouble_3D(args):
     return BornOpInitFloat_3D(args, precision="f64")
+
+
+# ---------------------------------------------------------------------------
+# Command-line drivers (python -m elasticisolib_3d_amd.operators nonlinear|born
+# par=... model=... data=... [adj=1] [dpTest=1] [precision=f64])
+# ---------------------------------------------------------------------------
+
+def main_nonlinear(argv) -> int:
+    """nonlinearPythonFloatMain_3D.py: forward (adj=0) reads model= and writes
+    data=, adjoint (adj=1) reads data= and writes model=; dpTest=1 runs the
+    dot-product test."""
+    from .sep import write_vector
+    par = ParObject(argv)
+    prec = par.getString("precision", "f32")
+    model, data, elastic, par, *geom = nonlinearOpInit_3D(par, prec)
+    cls = nonlinearPropElasticShotsGpu_3D if prec == "f32" else nonlinearPropElasticShotsGpu_3D_double
+    op = cls(model, data, elastic, par, *geom)
+    if par.getInt("dpTest", 0) == 1:
+        op.dotTest(True)
+        return 0
+    model_file = par.getString("model", "noModelFile")
+    data_file = par.getString("data", "noDataFile")
+    if model_file == "noModelFile":
+        raise IOError("**** ERROR: User did not provide model file [model] ****")
+    if data_file == "noDataFile":
+        raise IOError("**** ERROR: User did not provide data file name [data] ****")
+    if par.getInt("adj", 0) == 0:
+        model.getNdArray().flat[:] = read_vector(model_file, ndims=4).getNdArray().ravel()
+        op.forward(False, model, data)
+        write_vector(data_file, data)
+    else:
+        data.getNdArray().flat[:] = read_vector(data_file, ndims=4).getNdArray().ravel()
+        op.adjoint(False, model, data)
+        write_vector(model_file, model)
+    return 0
+
+
+def main_born(argv) -> int:
+    """BornPythonFloatMain_3D.py: forward reads model= (drho, dlambda, dmu)
+    and writes data=; adjoint reads data= and writes model=."""
+    from .sep import write_vector
+    par = ParObject(argv)
+    prec = par.getString("precision", "f32")
+    model, data, elastic, par, signals, *geom = BornOpInitFloat_3D(par, prec)
+    cls = BornElasticShotsGpu_3D if prec == "f32" else BornElasticShotsGpu_3D_double
+    op = cls(model, data, elastic, par, signals, *geom)
+    if par.getInt("dpTest", 0) == 1:
+        op.dotTest(True)
+        return 0
+    model_file = par.getString("model", "noModelFile")
+    data_file = par.getString("data", "noDataFile")
+    if model_file == "noModelFile":
+        raise IOError("**** ERROR: User did not provide model file ****")
+    if data_file == "noDataFile":
+        raise IOError("**** ERROR: User did not provide data file name ****")
+    if par.getInt("adj", 0) == 0:
+        model.getNdArray().flat[:] = read_vector(model_file).getNdArray().ravel()
+        op.forward(False, model, data)
+        write_vector(data_file, data)
+    else:
+        data.getNdArray().flat[:] = read_vector(data_file, ndims=4).getNdArray().ravel()
+        op.adjoint(False, model, data)
+        write_vector(model_file, model)
+    return 0
+
+
+if __name__ == "__main__":
+    if len(sys.argv) < 2 or sys.argv[1] not in ("nonlinear", "born"):
+        print("usage: python -m elasticisolib_3d_amd.operators nonlinear|born par=... model=... data=... [adj=1]",
+              file=sys.stderr)
+        sys.exit(2)
+    sys.exit(main_nonlinear(sys.argv[2:]) if sys.argv[1] == "nonlinear" else main_born(sys.argv[2:]))

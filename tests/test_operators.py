@@ -163,3 +163,48 @@ def test_shots_operator_refuses_domain_decomposition(monkeypatch):
     PB.use_oracle_backend(monkeypatch)
     with pytest.raises(NotImplementedError):
         OP._device_of(PB.make_par(domDec=1))
+
+
+def test_data_component_sampling():
+    hyp = Hypercube([Axis(7), Axis(5), Axis(9), Axis(2)])
+    model = Vector(hyp, np.random.default_rng(3).standard_normal(hyp.shape))
+    op = OP.ElasticDatComp_3D("p,vx,sxz", model)
+    data = op.getRange().clone()
+    op.forward(False, model, data)
+    m = model.getNdArray()
+    d = data.getNdArray()
+    assert d.shape == (2, 3, 5, 7)
+    assert np.allclose(d[:, 0], (m[:, 3] + m[:, 4] + m[:, 5]) / 3.0)
+    assert np.array_equal(d[:, 1], m[:, 0]) and np.array_equal(d[:, 2], m[:, 6])
+    assert op.dotTestCpp(maxError=1e-12, seed=4)
+    with pytest.raises(ValueError):
+        OP.ElasticDatComp_3D("vx,vx", model)
+    with pytest.raises(ValueError):
+        OP.ElasticDatComp_3D("foo,bar", model)
+
+
+def test_command_line_drivers_round_trip(tmp_path, monkeypatch):
+    """The nonlinear driver reads a par file and SEP files, runs forward and
+    adjoint through the operator and writes SEP files (oracle backend)."""
+    PB.use_oracle_backend(monkeypatch)
+    ep = PB.elastic_vector()
+    sgeo, rgeo = PB.geometries()
+    write_vector(str(tmp_path / "elastic.H"), ep)
+    write_vector(str(tmp_path / "src_geom.H"), Vector(Hypercube(ns=[2, 1, 3]), sgeo.astype(np.float32)))
+    write_vector(str(tmp_path / "rec_geom.H"), Vector(Hypercube(ns=[1, 5, 3]), rgeo.astype(np.float32)))
+    par = PB.make_par()
+    lines = [f"{k}={v}" for k, v in par._p.items()]
+    lines += [f"elasticParam={tmp_path / 'elastic.H'}", f"sourceGeomFile={tmp_path / 'src_geom.H'}",
+              f"receiverGeomFile={tmp_path / 'rec_geom.H'}", "precision=f64"]
+    (tmp_path / "run.par").write_text("\n".join(lines) + "\n")
+    nts = par.getInt("nts")
+    wav = np.zeros((1, 9, 1, nts), np.float32)
+    wav[0, 5, 0] = np.sin(np.arange(nts))
+    write_vector(str(tmp_path / "model.H"), Vector(Hypercube(ns=[nts, 1, 9, 1]), wav))
+    args = [f"par={tmp_path / 'run.par'}", f"model={tmp_path / 'model.H'}", f"data={tmp_path / 'data.H'}"]
+    assert OP.main_nonlinear(args) == 0
+    d = read_vector(str(tmp_path / "data.H"))
+    assert d.getNdArray().shape == (2, 9, 5, nts) and np.abs(d.getNdArray()).max() > 0
+    assert OP.main_nonlinear(args[:1] + [f"model={tmp_path / 'back.H'}", args[2], "adj=1"]) == 0
+    back = read_vector(str(tmp_path / "back.H"))
+    assert back.getNdArray().shape == (1, 9, 1, nts)
