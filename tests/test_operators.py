@@ -208,3 +208,22 @@ def test_command_line_drivers_round_trip(tmp_path, monkeypatch):
     assert OP.main_nonlinear(args[:1] + [f"model={tmp_path / 'back.H'}", args[2], "adj=1"]) == 0
     back = read_vector(str(tmp_path / "back.H"))
     assert back.getNdArray().shape == (1, 9, 1, nts)
+
+
+def test_padding_tool(tmp_path):
+    from elasticisolib_3d_amd import pad as PD
+    from elasticisolib_3d_amd.models import pad_model
+    rng = np.random.default_rng(0)
+    m = Vector(Hypercube([Axis(20, 0.0, 10.0), Axis(25, 5.0, 10.0), Axis(7, 0.0, 10.0), Axis(3)]),
+               rng.standard_normal((3, 7, 25, 20)).astype(np.float32))
+    padded, p = PD.pad_vector(m, 5, 6, 3, fat=4, block=16)
+    assert (p["nz"] - 8 - 0) % 16 == 0 and (p["nx"] - 8) % 16 == 0
+    ref, pads = pad_model(m.getNdArray().astype(np.float64), 5, 6, 3)
+    assert pads == (5, p["zPadPlus"], 6, p["xPadPlus"], 3)
+    assert np.array_equal(padded.getNdArray(), ref.astype(np.float32))
+    assert padded.getHyper().getAxis(2).o == 5.0 - (4 + 6) * 10.0
+    fs = PD.padding(20, 25, 7, 5, 6, 3, free_surface=1)
+    assert fs["zPadMinus"] == 4 and (fs["zPadMinus"] + fs["zPadPlus"] + 20) % 16 == 0
+    write_vector(str(tmp_path / "m.H"), m)
+    assert PD.main([f"model={tmp_path / 'm.H'}", f"data={tmp_path / 'p.H'}", "zPad=5", "xPad=6", "yPad=3"]) == 0
+    assert np.array_equal(read_vector(str(tmp_path / "p.H")).getNdArray(), padded.getNdArray())
