@@ -220,3 +220,58 @@ def fwi_misfit(rho, lam, mu, problem: FwiProblem):
     if _FN is None:
         _FN = _autograd_function()
     return _FN.apply(rho, lam, mu, problem)
+
+
+def invert(problem, m0, n_iter: int = 5, mask=None, bounds=None, history_size: int = 10,
+           line_search: str = "strong_wolfe", log=None):
+    """FWI outer loop: L-BFGS (torch.optim.LBFGS) on x = m / m0 (each of
+    rho, lambda, mu normalised by the starting model, which scales the three
+    parameters alike), the gradient from `problem.misfit_and_gradient`
+    (adjoint state). `mask` ([3, ny, nx, nz] or broadcastable) multiplies the
+    gradient (the reference's maskGradient); `bounds` = ((min_rho, min_lambda,
+    min_mu), (max_rho, max_lambda, max_mu)) are enforced by projection after
+    every evaluation (scalars or arrays; None for no bound). Returns
+    (model (3 arrays), history of (evaluation, J)). The counterpart of the
+    reference's efwiMain_3D.py driver, whose pySolver solvers are not part of
+    this project."""
+    torch = _torch()
+    m0 = [np.asarray(m, np.float64) for m in m0]
+    ref = [torch.from_numpy(m) for m in m0]
+    x = [torch.ones_like(r, requires_grad=True) for r in ref]
+    msk = None if mask is None else torch.as_tensor(np.asarray(mask, np.float64))
+    lo = hi = None
+    if bounds is not None:
+        lo = [None if b is None else torch.as_tensor(np.asarray(b, np.float64)) / r for b, r in zip(bounds[0], ref)]
+        hi = [None if b is None else torch.as_tensor(np.asarray(b, np.float64)) / r for b, r in zip(bounds[1], ref)]
+    opt = torch.optim.LBFGS(x, lr=1.0, max_iter=n_iter, max_eval=int(1.5 * n_iter) + 2, history_size=history_size,
+                            line_search_fn=line_search, tolerance_grad=0.0, tolerance_change=0.0)
+    history = []
+
+    def project():
+        if bounds is None:
+            return
+        with torch.no_grad():
+            for k in range(3):
+                if lo[k] is not None:
+                    x[k].copy_(torch.maximum(x[k], lo[k]))
+                if hi[k] is not None:
+                    x[k].copy_(torch.minimum(x[k], hi[k]))
+
+    def closure():
+        opt.zero_grad()
+        project()
+        m = [xi.detach().numpy() * mi for xi, mi in zip(x, m0)]
+        J, g3 = problem.misfit_and_gradient(*m)
+        for k in range(3):
+            g = torch.from_numpy(np.ascontiguousarray(g3[k] * m0[k]))   # d/dx = d/dm * m0
+            if msk is not None:
+                g = g * (msk[k] if msk.dim() == 4 else msk)
+            x[k].grad = g
+        history.append((len(history), float(J)))
+        if log is not None:
+            log(f"[fwi] evaluation {len(history)}: J = {J:.9e}")
+        return torch.tensor(J, dtype=torch.float64)
+
+    opt.step(closure)
+    project()
+    return [xi.detach().numpy() * mi for xi, mi in zip(x, m0)], history

@@ -227,3 +227,33 @@ def test_padding_tool(tmp_path):
     write_vector(str(tmp_path / "m.H"), m)
     assert PD.main([f"model={tmp_path / 'm.H'}", f"data={tmp_path / 'p.H'}", "zPad=5", "xPad=6", "yPad=3"]) == 0
     assert np.array_equal(read_vector(str(tmp_path / "p.H")).getNdArray(), padded.getNdArray())
+
+
+def test_fwi_outer_loop_on_a_quadratic_misfit():
+    """The L-BFGS driver reduces a separable quadratic misfit (the interface
+    of FwiProblem.misfit_and_gradient) and honours bounds and the mask."""
+    from elasticisolib_3d_amd.fwi import invert
+    rng = np.random.default_rng(0)
+    shape = (3, 4, 5)
+    true = [2000 + 100 * rng.random(shape), 5e9 * (1 + 0.1 * rng.random(shape)), 3e9 * (1 + 0.1 * rng.random(shape))]
+    w = [1.0 / t ** 2 for t in true]
+
+    class Quad:
+        def misfit_and_gradient(self, r, l, m):
+            ms = (r, l, m)
+            J = 0.5 * sum(float(np.sum(wk * (mk - tk) ** 2)) for wk, mk, tk in zip(w, ms, true))
+            return J, np.stack([wk * (mk - tk) for wk, mk, tk in zip(w, ms, true)])
+
+    m0 = [t * 0.9 for t in true]
+    out, hist = invert(Quad(), m0, n_iter=20)
+    assert hist[-1][1] < 1e-6 * hist[0][1]
+    assert all(np.allclose(o, t, rtol=1e-3) for o, t in zip(out, true))
+    # bounds: rho capped below its true value
+    cap = 0.95 * true[0]
+    out, _ = invert(Quad(), m0, n_iter=20, bounds=((None, None, None), (cap, None, None)))
+    assert np.all(out[0] <= cap * (1 + 1e-12))
+    # mask: a zero mask on mu keeps it at its start
+    mask = np.ones((3,) + shape)
+    mask[2] = 0
+    out, _ = invert(Quad(), m0, n_iter=10, mask=mask)
+    assert np.allclose(out[2], m0[2])
