@@ -403,12 +403,12 @@ ela::PointArgs<T> make_points(ela_handle* h, const DevPoints& pts, T* const fiel
   return a;
 }
 
-template <typename T, bool ACC, bool GRAD = false>
+template <typename T, bool ACC, bool GRAD = false, bool PM = false>
 void launch_inject(ela_handle* h, ela::PointArgs<T> a, int its, double fa, double fb) {
   if (a.off[9] == 0) return;
   a.its = its; a.fa = T(fa); a.fb = T(fb);
   const unsigned nb = (unsigned)((a.off[9] + 255) / 256);
-  hipLaunchKernelGGL((ela::k_inject<T, ACC, GRAD>), dim3(nb), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL((ela::k_inject<T, ACC, GRAD, PM>), dim3(nb), dim3(256), 0, h->stream, a);
 }
 
 template <typename T>
@@ -474,23 +474,45 @@ int stage_out_traces(ela_handle* h, const ela_points* p, void* const dev[9], con
 
 // ---- the fine-step loops --------------------------------------------------
 
+// Two time levels (p0 = level n-1, overwritten by n+1; p1 = level n) and,
+// for the premultiplied Born-adjoint stepper, their medium products q0, q1.
 template <typename T>
 struct Wave {
   T* p0[9];
   T* p1[9];
+  T* q0[9];
+  T* q1[9];
   void swap() {
-    for (int k = 0; k < 9; ++k) std::swap(p0[k], p1[k]);
+    for (int k = 0; k < 9; ++k) {
+      std::swap(p0[k], p1[k]);
+      std::swap(q0[k], q1[k]);
+    }
   }
 };
 
 template <typename T>
-Wave<T> wave_ptrs(DevBuf& b, int64_t stride) {
+Wave<T> wave_ptrs(DevBuf& b, int64_t stride, bool premultiplied = false) {
   Wave<T> w;
   for (int k = 0; k < 9; ++k) {
     w.p0[k] = arr<T>(b, stride, k);
     w.p1[k] = arr<T>(b, stride, 9 + k);
+    w.q0[k] = premultiplied ? arr<T>(b, stride, 18 + k) : nullptr;
+    w.q1[k] = premultiplied ? arr<T>(b, stride, 27 + k) : nullptr;
   }
   return w;
+}
+
+// Premultiplied Born-adjoint step arguments: read M c from q1, write M n to q0.
+template <typename T>
+void set_pm(ela_handle* h, ela::StepArgs<T>& a, const Wave<T>& w) {
+  for (int k = 0; k < 9; ++k) { a.q[k] = w.q1[k]; a.qn[k] = w.q0[k]; }
+  (void)h;
+}
+
+template <typename T>
+void set_pm(ela_handle* h, ela::PointArgs<T>& a, const Wave<T>& w) {
+  for (int k = 0; k < 9; ++k) a.q[k] = w.q0[k];
+  for (int k = 0; k < 8; ++k) a.m[k] = arr<T>(h->med, h->stride, k);
 }
 
 // The 9 components of a time-major trace buffer share one base pointer
@@ -735,8 +757,9 @@ struct BornPlan {
 };
 
 // Arrays (of h->stride elements) the Born adjoint needs besides V and the
-// checkpoints: medium 8, two wavefields 36, secondary sources 18, images 8.
-constexpr int64_t kBornFixedArrays = 8 + 36 + 18 + 8;
+// checkpoints: medium 8, background wavefield 18, adjoint wavefield with its
+// premultiplied levels 36, secondary sources 18, images 8.
+constexpr int64_t kBornFixedArrays = 8 + 18 + 36 + 18 + 8;
 
 int born_plan(ela_handle* h, BornPlan& p) {
   const int64_t nts = h->cfg.nts, ncoarse = nts - 1;
@@ -836,14 +859,14 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
   if (h->coarse.bytes != v_bytes) h->coarse.release();
   CHK(h->ckpt.alloc(ck_bytes));
   CHK(h->coarse.alloc(v_bytes));
-  CHK(h->wave2.alloc(18 * np * sizeof(T)));
+  CHK(h->wave2.alloc(36 * np * sizeof(T)));
   CHK(h->ss.alloc(18 * np * sizeof(T)));
   T* pool = (T*)h->coarse.p;
   T* ck = (T*)h->ckpt.p;
   T* ssL[9];
   T* ssR[9];
   for (int k = 0; k < 9; ++k) { ssL[k] = arr<T>(h->ss, np, k); ssR[k] = arr<T>(h->ss, np, 9 + k); }
-  Wave<T> bg = wave_ptrs<T>(h->wave, np), ad = wave_ptrs<T>(h->wave2, np);
+  Wave<T> bg = wave_ptrs<T>(h->wave, np), ad = wave_ptrs<T>(h->wave2, np, true);
   const double* f = h->filt.data();
   int64_t steps = 0;
 
@@ -879,18 +902,20 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
   }
 
   // adjoint sweep
-  CHK(fill<T>(h, h->wave2.p, 18 * np));
+  CHK(fill<T>(h, h->wave2.p, 36 * np));
   CHK(fill<T>(h, h->ss.p, 18 * np));
   auto adjoint_step = [&](int64_t its) -> int {
     for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
       ela::StepArgs<T> a = base_args<T>(h, ad.p0, ad.p1);
+      set_pm<T>(h, a, ad);
       for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
       a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-      CHK((launch_step<T, ela::MODE_BORN_ADJ, ela::F_ACC, ela::F_ACC>(h, a)));
+      CHK((launch_step<T, ela::MODE_BORN_ADJ_PM, ela::F_ACC, ela::F_ACC>(h, a)));
       ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm).v, true, nrec_tot);
+      set_pm<T>(h, R, ad);
       for (int k = 0; k < 9; ++k) { R.accl[k] = ssL[k]; R.accr[k] = ssR[k]; }
       R.accfl = a.accfl; R.accfr = a.accfr;
-      launch_inject<T, true>(h, R, (int)its, f[it2 + 1], f[hh + it2 + 1]);
+      launch_inject<T, true, false, true>(h, R, (int)its, f[it2 + 1], f[hh + it2 + 1]);
       ad.swap();
     }
     launch_imaging<T, true>(h, imaging_args<T>(h, V(its), V(its + 1), V(its + 2), its + 1, nullptr, dm, ssR));
@@ -1024,7 +1049,7 @@ int grad_plan(ela_handle* h, GradPlan& p) {
   p.ck1_bytes = (size_t)18 * nI * (p.bf16 ? 2 : h->esize);
   p.ck2_bytes = (size_t)18 * nI * h->esize;
   const double arr = double(h->stride) * h->esize;
-  const double fixed = arr * (9.0 * p.K + 8 + 18) + double(h->stride) * 8.0 * 8;  // drv, g32, wave2, g64
+  const double fixed = arr * (9.0 * p.K + 8 + 36) + double(h->stride) * 8.0 * 8;  // drv, g32, wave2, g64
   auto mem_for = [&](int64_t K1) {
     const int64_t n1 = (N + K1 - 1) / K1;
     return fixed + double(n1) * p.ck1_bytes + (K1 > p.K ? double(K1 / p.K) * p.ck2_bytes : 0.0);
@@ -1236,15 +1261,15 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   CHK(h->drv.alloc((size_t)plan.K * 9 * np * sizeof(T)));
   CHK(h->g32.alloc((size_t)8 * np * sizeof(T)));
   CHK(h->g64.alloc((size_t)8 * np * sizeof(double)));
-  CHK(h->wave2.alloc((size_t)18 * np * sizeof(T)));
+  CHK(h->wave2.alloc((size_t)36 * np * sizeof(T)));
   T* drv = (T*)h->drv.p;
   T* g32 = (T*)h->g32.p;
   double* g64 = (double*)h->g64.p;
   CHK(fill<T>(h, g32, 8 * np));
   CHK(fill<double>(h, g64, 8 * np));
   CHK(copy_in_3d(h, g64, grad8, 8, mem, sizeof(double)));
-  CHK(fill<T>(h, h->wave2.p, 18 * np));
-  Wave<T> bg = wave_ptrs<T>(h->wave, np), ad = wave_ptrs<T>(h->wave2, np);
+  CHK(fill<T>(h, h->wave2.p, 36 * np));
+  Wave<T> bg = wave_ptrs<T>(h->wave, np), ad = wave_ptrs<T>(h->wave2, np, true);
   const double* f = h->filt.data();
   const float* scales = (const float*)h->ckscale.p;
   int64_t steps = 0;
@@ -1252,13 +1277,15 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   auto adj_step = [&](int64_t k, const T* d) -> int {
     const int its = (int)(k / sub), i2 = (int)(k % sub);
     ela::StepArgs<T> a = base_args<T>(h, ad.p0, ad.p1);
+    set_pm<T>(h, a, ad);
     for (int c = 0; c < 9; ++c) a.drv[c] = d + c * np;
     for (int m = 0; m < 8; ++m) a.grad[m] = g32 + m * np;
-    CHK((launch_step<T, ela::MODE_BORN_ADJ, ela::F_GRAD, ela::F_GRAD>(h, a)));
+    CHK((launch_step<T, ela::MODE_BORN_ADJ_PM, ela::F_GRAD, ela::F_GRAD>(h, a)));
     ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm_adj).v, true, nrec_tot);
+    set_pm<T>(h, R, ad);
     for (int c = 0; c < 9; ++c) R.drv[c] = a.drv[c];
     for (int m = 0; m < 8; ++m) R.grad[m] = a.grad[m];
-    launch_inject<T, false, true>(h, R, its, f[i2 + 1], f[hh + i2 + 1]);
+    launch_inject<T, false, true, true>(h, R, its, f[i2 + 1], f[hh + i2 + 1]);
     launch_record<T>(h, make_points<T>(h, sp, ad.p0, TmViews<T>(tm_src).v, true, nsrc_tot), its, f[i2], f[hh + i2]);
     ad.swap();
     return ELA_OK;

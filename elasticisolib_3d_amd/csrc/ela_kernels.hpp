@@ -40,7 +40,8 @@ enum { BX_ = 0, BY_, BZ_, L2M, LAM, MXZ, MXY, MYZ };
 // Stepper modes.
 enum { MODE_FWD = 0,       // n = w(w o + M D c)        stepFwdGpu_3D
        MODE_ADJ = 1,       // n = w(w o - M D c)        stepAdjGpu_3D
-       MODE_BORN_ADJ = 2   // n = w(w o - D (M c))      stepAdjGpuBorn_3D
+       MODE_BORN_ADJ = 2,  // n = w(w o - D (M c))      stepAdjGpuBorn_3D
+       MODE_BORN_ADJ_PM = 3  // the same with M c read premultiplied (q) and M n written (qn)
 };
 
 // Optional work fused into a stepper launch (bit flags).
@@ -62,6 +63,8 @@ struct StepArgs {
   T* dst[9];            // F_STORE: sx sy sz exx eyy ezz gxz gxy gyz of this step
   const T* drv[9];      // F_GRAD: the same combinations stored by the matching forward step
   T* grad[8];           // F_GRAD: gradient accumulators, one per medium array
+  const T* q[9];        // MODE_BORN_ADJ_PM: M c of the current level
+  T* qn[9];             // MODE_BORN_ADJ_PM: M n of the new level (written)
   T ssfl, ssfr;         // filter weights for the secondary source
   T accfl, accfr;       // filter weights for the accumulation
   T cx[4], cy[4], cz[4];
@@ -171,38 +174,47 @@ __device__ __forceinline__ void st_nt(T* base, uint32_t i, T v) {
 
 // Values that the derivative operators see. In MODE_BORN_ADJ they are the
 // medium-multiplied fields (stepAdjGpuBorn_3D multiplies at every loaded
-// position, kernelsElaGpu_3D.cu:864-1040).
+// position, kernelsElaGpu_3D.cu:864-1040); MODE_BORN_ADJ_PM reads the same
+// products from the premultiplied level that the previous step wrote, so its
+// halo and queue loads are one array per field as in the forward stepper.
 template <typename T, int MODE>
 struct StressVal {
   const StepArgs<T>& a;
   bool ok;
+  __device__ __forceinline__ T pm(int f, uint32_t i) const { return ok ? *at(a.q[f], i) : T(0); }
   __device__ __forceinline__ T sxx(uint32_t i) const {
+    if (MODE == MODE_BORN_ADJ_PM) return pm(SXX, i);
     if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
       return *at(a.c[SXX], i) * *at(a.m[L2M], i) + (*at(a.c[SYY], i) + *at(a.c[SZZ], i)) * *at(a.m[LAM], i);
     return *at(a.c[SXX], i);
   }
   __device__ __forceinline__ T syy(uint32_t i) const {
+    if (MODE == MODE_BORN_ADJ_PM) return pm(SYY, i);
     if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
       return *at(a.c[SYY], i) * *at(a.m[L2M], i) + (*at(a.c[SXX], i) + *at(a.c[SZZ], i)) * *at(a.m[LAM], i);
     return *at(a.c[SYY], i);
   }
   __device__ __forceinline__ T szz(uint32_t i) const {
+    if (MODE == MODE_BORN_ADJ_PM) return pm(SZZ, i);
     if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
       return *at(a.c[SZZ], i) * *at(a.m[L2M], i) + (*at(a.c[SXX], i) + *at(a.c[SYY], i)) * *at(a.m[LAM], i);
     return *at(a.c[SZZ], i);
   }
   __device__ __forceinline__ T sxz(uint32_t i) const {
+    if (MODE == MODE_BORN_ADJ_PM) return pm(SXZ, i);
     if (!ok) return T(0);
     return MODE == MODE_BORN_ADJ ? *at(a.c[SXZ], i) * *at(a.m[MXZ], i) : *at(a.c[SXZ], i);
   }
   __device__ __forceinline__ T sxy(uint32_t i) const {
+    if (MODE == MODE_BORN_ADJ_PM) return pm(SXY, i);
     if (!ok) return T(0);
     return MODE == MODE_BORN_ADJ ? *at(a.c[SXY], i) * *at(a.m[MXY], i) : *at(a.c[SXY], i);
   }
   __device__ __forceinline__ T syz(uint32_t i) const {
+    if (MODE == MODE_BORN_ADJ_PM) return pm(SYZ, i);
     if (!ok) return T(0);
     return MODE == MODE_BORN_ADJ ? *at(a.c[SYZ], i) * *at(a.m[MYZ], i) : *at(a.c[SYZ], i);
   }
@@ -214,6 +226,7 @@ struct VelVal {
   bool ok;
   __device__ __forceinline__ T v(int f, uint32_t i) const {
     if (!ok) return T(0);
+    if (MODE == MODE_BORN_ADJ_PM) return *at(a.q[f], i);
     return MODE == MODE_BORN_ADJ ? *at(a.c[f], i) * *at(a.m[f], i) : *at(a.c[f], i);  // BX_/BY_/BZ_ == VX/VY/VZ
   }
 };
@@ -396,6 +409,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
         if (SS) u += *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr;
         u *= w;
         st_nt(a.o[f], i, u);
+        if (MODE == MODE_BORN_ADJ_PM) st_nt(a.qn[f], i, u * m3[f]);
         if (ACC) {
           *at(a.accl[f], i) += u * a.accfl;
           *at(a.accr[f], i) += u * a.accfr;
@@ -500,7 +514,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
         st_nt(a.dst[SXZ], i, gxz); st_nt(a.dst[SXY], i, gxy); st_nt(a.dst[SYZ], i, gyz);
       }
       T u[6];
-      if (MODE == MODE_BORN_ADJ) {
+      if (MODE == MODE_BORN_ADJ || MODE == MODE_BORN_ADJ_PM) {
         u[0] = w * o6[0] - exx;
         u[1] = w * o6[1] - eyy;
         u[2] = w * o6[2] - ezz;
@@ -528,6 +542,14 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
           *at(a.accl[f], i) += v * a.accfl;
           *at(a.accr[f], i) += v * a.accfr;
         }
+      }
+      if (MODE == MODE_BORN_ADJ_PM) {
+        st_nt(a.qn[SXX], i, u[0] * l2m + (u[1] + u[2]) * lam);
+        st_nt(a.qn[SYY], i, u[1] * l2m + (u[0] + u[2]) * lam);
+        st_nt(a.qn[SZZ], i, u[2] * l2m + (u[0] + u[1]) * lam);
+        st_nt(a.qn[SXZ], i, u[3] * mxz);
+        st_nt(a.qn[SXY], i, u[4] * mxy);
+        st_nt(a.qn[SYZ], i, u[5] * myz);
       }
       if (F & F_GRAD) {
         // lamb2MuDtw multiplies exx, eyy, ezz in sxx, syy, szz; lambDtw the
@@ -573,6 +595,9 @@ struct PointArgs {
   // optional gradient imaging of the injected value (adjoint-state gradient)
   const T* drv[9];
   T* grad[8];
+  // optional premultiplied level to keep consistent (MODE_BORN_ADJ_PM)
+  T* q[9];
+  const T* m[8];
   // damping (injection only)
   const T* taper;
   int min_pad, nz, nx, ny, nzi, zoff;
@@ -605,7 +630,7 @@ __device__ __forceinline__ T point_taper(const PointArgs<T>& a, int32_t p) {
 // field[pos] += w(pos) * (tr[its] fa + tr[its+1] fb)
 // (ker_inject_source_* / ker_inject_data_*, kernelsElaGpu_3D.cu:11-52, with the
 // sponge factor that the reference applies right after injection).
-template <typename T, bool ACC, bool GRAD = false>
+template <typename T, bool ACC, bool GRAD = false, bool PM = false>
 __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.off[9]) return;
@@ -621,6 +646,19 @@ __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   if (ACC && a.accl[c]) {
     a.accl[c][p] += v * a.accfl;
     a.accr[c][p] += v * a.accfr;
+  }
+  if (PM) {
+    // keep q = M n: the normal stresses share lamb2MuDtw / lambDtw (atomics)
+    if (c < 3) {
+      atomicAdd(&a.q[c][p], v * a.m[c][p]);
+    } else if (c < 6) {
+      const T l2 = a.m[L2M][p], la = a.m[LAM][p];
+      atomicAdd(&a.q[SXX][p], v * (c == SXX ? l2 : la));
+      atomicAdd(&a.q[SYY][p], v * (c == SYY ? l2 : la));
+      atomicAdd(&a.q[SZZ][p], v * (c == SZZ ? l2 : la));
+    } else {
+      atomicAdd(&a.q[c][p], v * a.m[MXZ + (c - SXZ)][p]);
+    }
   }
   if (GRAD) {
     // the stepper imaged the level before this injection; add the injected
