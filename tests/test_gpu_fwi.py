@@ -92,9 +92,12 @@ def test_autograd_front_end():
     J = fwi_misfit(ts[0], ts[1], ts[2], prob)
     (2.0 * J).backward()
     Jr, g3 = prob.misfit_and_gradient(*m0)
-    assert abs(float(J) - Jr) <= 1e-12 * Jr
+    assert abs(float(J.detach()) - Jr) <= 1e-12 * Jr
+    # repeated runs agree to rounding (atomic additions at the receivers make
+    # the FP64 summation order run-dependent)
     for k in range(3):
-        assert np.allclose(ts[k].grad.numpy(), 2.0 * g3[k], rtol=1e-12, atol=0)
+        g = ts[k].grad.numpy()
+        assert np.linalg.norm(g - 2.0 * g3[k]) <= 1e-10 * np.linalg.norm(2.0 * g3[k])
 
 
 def test_outer_loop_reduces_the_misfit():
