@@ -23,7 +23,7 @@ EXPORTS = (
     "ela_last_error", "ela_version", "ela_create", "ela_destroy", "ela_set_stream",
     "ela_synchronize", "ela_set_medium", "ela_scale_medium", "ela_nl_forward",
     "ela_nl_adjoint", "ela_born_forward", "ela_born_adjoint", "ela_last_timing",
-    "ela_interp_linear", "ela_time_step_kernels", "ela_set_tiling", "ela_get_tiling",
+    "ela_interp_linear", "ela_interp_weights", "ela_time_step_kernels", "ela_set_tiling", "ela_get_tiling",
     "ela_set_tile_order", "ela_set_born_storage", "ela_born_storage_used",
     "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
     "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
@@ -106,6 +106,9 @@ def load() -> ctypes.CDLL:
     lib.ela_born_forward.argtypes = [vp, P, Tr, vp, P, Tr, i32]
     lib.ela_born_adjoint.argtypes = [vp, P, Tr, vp, P, Tr, i32]
     lib.ela_interp_linear.argtypes = [ctypes.POINTER(ElaInterpAxes), i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)]
+    lib.ela_interp_weights.argtypes = [ctypes.POINTER(ElaInterpAxes), i64, vp, vp, vp, i32, i32, i32,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp, vp,
+                                       ctypes.POINTER(i64)]
     lib.ela_set_tiling.argtypes = [vp, i32, i32]
     lib.ela_set_tile_order.argtypes = [vp, i32, i32]
     lib.ela_set_born_storage.argtypes = [vp, i64, i64]

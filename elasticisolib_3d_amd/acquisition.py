@@ -63,29 +63,43 @@ class PaddedAxes:
 
 
 class SpaceInterp:
-    """Linear interpolation of one device set onto one grid (spaceInterpGpu_3D)."""
+    """Interpolation of one device set onto one grid (spaceInterpGpu_3D):
+    linear (8 corners) or sinc with half-length `half_length` per axis, each
+    optionally with a dipole, a second negative pole at the device shifted by
+    `dipole_shift` = (dz, dx, dy) in model units."""
 
-    def __init__(self, axes: PaddedAxes, z, x, y):
+    def __init__(self, axes: PaddedAxes, z, x, y, method: str = "linear", half_length: int = 1, dipole: int = 0,
+                 dipole_shift=(0.0, 0.0, 0.0)):
         lib = _lib.load()
         z = np.ascontiguousarray(z, np.float64)
         x = np.ascontiguousarray(x, np.float64)
         y = np.ascontiguousarray(y, np.float64)
+        kinds = {"linear": 0, "sinc": 1}
+        if method not in kinds:
+            raise ValueError(f"spatial interpolation method {method!r} is not supported (linear or sinc)")
+        if method == "linear" and half_length != 1:
+            raise ValueError("Half-length of interpolation filter should be set to 1 for linear interpolation")
         n = z.shape[0]
-        nbr = np.empty(8 * max(n, 1), np.int64)
-        w = np.empty(8 * max(n, 1), np.float64)
-        uniq = np.empty(8 * max(n, 1), np.int64)
+        nf = (8 if method == "linear" else 8 * int(half_length) ** 3) * (2 if dipole else 1)
+        nbr = np.empty(nf * max(n, 1), np.int64)
+        w = np.empty(nf * max(n, 1), np.float64)
+        uniq = np.empty(nf * max(n, 1), np.int64)
         nu = ctypes.c_int64()
         ax = axes.c()
-        rc = lib.ela_interp_linear(ctypes.byref(ax), n, z.ctypes.data, x.ctypes.data, y.ctypes.data, nbr.ctypes.data,
-                                   w.ctypes.data, uniq.ctypes.data, ctypes.byref(nu))
+        sz, sx, sy = (float(v) for v in dipole_shift)
+        rc = lib.ela_interp_weights(ctypes.byref(ax), n, z.ctypes.data, x.ctypes.data, y.ctypes.data, kinds[method],
+                                    int(half_length), int(dipole), sz, sx, sy, nbr.ctypes.data, w.ctypes.data,
+                                    uniq.ctypes.data, ctypes.byref(nu))
         if rc != _lib.ELA_OK:
             raise ValueError("acquisition device out of bounds (or its interpolation stencil leaves the padded interior)")
         self.n_dev = n
+        self.method, self.half_length, self.dipole = method, int(half_length), int(dipole)
         self.pos_unique = uniq[: nu.value].copy()
-        self.nbr = nbr[: 8 * n].reshape(n, 8)
-        self.weight = w[: 8 * n].reshape(n, 8)
-        rows = np.repeat(np.arange(n), 8)
-        # M: [n_dev, n_reg]; forward (reg -> irreg) is M @ reg, adjoint is M.T @ irreg
+        self.nbr = nbr[: nf * n].reshape(n, nf)
+        self.weight = w[: nf * n].reshape(n, nf)
+        rows = np.repeat(np.arange(n), nf)
+        # M: [n_dev, n_reg]; forward (reg -> irreg) is M @ reg, adjoint is M.T @ irreg (duplicate
+        # entries of one device, e.g. the two poles of a dipole sharing a point, are summed)
         self.M = sp.csr_matrix((self.weight.ravel(), (rows, self.nbr.ravel())), shape=(n, nu.value))
 
     @property
@@ -104,8 +118,8 @@ class SpaceInterp:
 class ShotGeometry:
     """The 7 grid interpolators of one device set (one shot's sources or receivers)."""
 
-    def __init__(self, axes: PaddedAxes, z, x, y):
-        self.grids: List[SpaceInterp] = [SpaceInterp(axes.grid(g), z, x, y) for g in range(7)]
+    def __init__(self, axes: PaddedAxes, z, x, y, **interp):
+        self.grids: List[SpaceInterp] = [SpaceInterp(axes.grid(g), z, x, y, **interp) for g in range(7)]
         self.n_dev = len(np.atleast_1d(z))
 
     @classmethod

@@ -5,9 +5,10 @@ bit-identical for every choice): the x extent of a 64-wide z tile (4, 8 or
 16 rows, i.e. 256, 512 or 1024 threads), the number of y planes one block
 marches through (the depth of its register queue pipeline before it hands the
 column to the next block), and the block-to-tile order (hardware round robin
-over the 8 XCDs, contiguous ranges per XCD, or y-major ranges per XCD in
-groups of x columns). The best choice depends on the grid (tile counts,
-partial tiles, L2 reuse of halo planes), so it is measured: every candidate
+over the 8 XCDs, contiguous ranges per XCD, y-major ranges per XCD in groups
+of x columns, or the round robin over groups of whole z columns). The best
+choice depends on the grid (tile counts, partial tiles, L2 reuse of halo
+planes), so it is measured: every candidate
 is timed with HIP events on the propagator's own stream
 (ela_time_step_kernels: a velocity and a stress launch of a forward step on
 the handle's current wavefields, which it overwrites), and the fastest is
@@ -22,7 +23,7 @@ from typing import Dict, Iterable, Optional, Sequence, Tuple
 
 TILE_X = (16, 8, 4)
 Y_CHUNK = (4, 8, 16, 32)
-ORDERS = ((0, 1), (1, 1), (2, 1), (2, 2), (2, 4))
+ORDERS = ((0, 1), (1, 1), (2, 1), (3, 1), (3, 2), (3, 4))
 
 Candidate = Tuple[int, int, int, int]  # tile_x, y_chunk, order, x_group
 

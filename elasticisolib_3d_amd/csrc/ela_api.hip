@@ -1599,7 +1599,7 @@ int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {
 
 int ela_set_tile_order(ela_handle* h, int order, int x_group) {
   if (!h) return fail(ELA_ERR_ARG, "null handle");
-  if (order < 0 || order > 2) return fail(ELA_ERR_ARG, "tile order must be 0, 1 or 2");
+  if (order < 0 || order > 3) return fail(ELA_ERR_ARG, "tile order must be 0, 1, 2 or 3");
   if (x_group < 1) return fail(ELA_ERR_ARG, "x_group must be >= 1");
   h->remap = order;
   h->xgroup = x_group;

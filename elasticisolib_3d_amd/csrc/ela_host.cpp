@@ -185,22 +185,107 @@ extern "C" int ela_medium_gradient(int64_t nz, int64_t nx, int64_t ny, double dt
 }
 
 // ---------------------------------------------------------------------------
-// Trilinear interpolation of acquisition devices onto one staggered grid: the
-// linear method of spaceInterpGpu_3D (elastic_iso_float/src/spaceInterpGpu_3D.cpp:
-// bounds check :177-211, weights :214-279, unique points :113-138).
+// Interpolation of acquisition devices onto one staggered grid: the linear and
+// sinc methods of spaceInterpGpu_3D (elastic_iso_float/src/spaceInterpGpu_3D.cpp:
+// constructor :13-79, bounds check :177-211, linear weights :214-279, dipole
+// :281-346, sinc weights :351-480, unique points :113-138).
 //
 // The fractional position is formed in single precision, as the reference
 // does (float wz = (z - oz) / dz; long long zReg = wz; wz = 1 - (wz - zReg)),
 // so that grid-point indices agree with the reference for coordinates that
-// sit on, or within rounding of, a grid node.
+// sit on, or within rounding of, a grid node; weights are stored rounded to
+// single precision as the reference stores them.
+//
+// Three corrections to the reference, which make its dipole and sinc paths
+// usable: the sinc support check accepts a support that lies inside the
+// stepped interior (the reference's comparison `i + h + 1 < n` is inverted
+// and rejects every device, :371-382); the dipole pole's y weight divides by
+// dy (the reference divides by dz, :287); the sinc dipole pole is negative as
+// in the linear method (the reference leaves its sign positive, :470).
 // ---------------------------------------------------------------------------
+#include <cmath>
 #include <unordered_map>
 
-extern "C" int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const double* zc, const double* xc,
-                                 const double* yc, int64_t* nbr, double* weight, int64_t* unique_pos,
-                                 int64_t* n_unique) {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// sin(pi t) / (pi t), 1 at t = 0 (the reference's boost::math::sinc_pi(M_PI * t)).
+double sinc_pi_t(double t) {
+  if (t == 0.0) return 1.0;
+  const double a = kPi * t;
+  return std::sin(a) / a;
+}
+
+struct InterpBounds {
+  int64_t lo[3], hi[3];  // allowed index range [lo, hi) per axis z, x, y
+};
+
+// Weights of one pole (sign +1 or -1) at (z, x, y) appended to (idx, w).
+int pole_weights(const ela_interp_axes* ax, int method, int h, float z, float x, float y, double sign,
+                 const InterpBounds& lin, const InterpBounds& snc, int64_t* idx, double* w) {
+  const int64_t nz = ax->nz, nx = ax->nx;
+  const float fz = (z - float(ax->oz)) / float(ax->dz);
+  const float fx = (x - float(ax->ox)) / float(ax->dx);
+  const float fy = (y - float(ax->oy)) / float(ax->dy);
+  const int64_t zr = (int64_t)fz, xr = (int64_t)fx, yr = (int64_t)fy;
+  if (method == ELA_INTERP_LINEAR) {
+    if (yr < lin.lo[2] || yr + 1 >= lin.hi[2]) return ELA_ERR_ARG;
+    if (xr < lin.lo[1] || xr + 1 >= lin.hi[1]) return ELA_ERR_ARG;
+    if (zr < lin.lo[0] || zr + 1 >= lin.hi[0]) return ELA_ERR_ARG;
+    const double az = 1.0f - (fz - float(zr)), ax_ = 1.0f - (fx - float(xr)), ay = 1.0f - (fy - float(yr));
+    const int64_t b = yr * nz * nx + xr * nz + zr;
+    const int64_t off[8] = {0, 1, nz, nz + 1, nz * nx, nz * nx + 1, nz * nx + nz, nz * nx + nz + 1};
+    const double v[8] = {az * ax_ * ay,
+                         (1.0 - az) * ax_ * ay,
+                         az * (1.0 - ax_) * ay,
+                         (1.0 - az) * (1.0 - ax_) * ay,
+                         az * ax_ * (1.0 - ay),
+                         (1.0 - az) * ax_ * (1.0 - ay),
+                         az * (1.0 - ax_) * (1.0 - ay),
+                         (1.0 - az) * (1.0 - ax_) * (1.0 - ay)};
+    for (int k = 0; k < 8; ++k) {
+      idx[k] = b + off[k];
+      w[k] = double(float(sign * v[k]));
+    }
+    return ELA_OK;
+  }
+  // sinc: support zr-h+1 .. zr+h per axis, y slowest then x then z
+  const int64_t r0[3] = {zr - h + 1, xr - h + 1, yr - h + 1};
+  for (int a = 0; a < 3; ++a)
+    if (r0[a] < snc.lo[a] || r0[a] + 2 * h > snc.hi[a]) return ELA_ERR_ARG;
+  const int nf = 2 * h;
+  std::vector<double> wz(nf), wx(nf), wy(nf);
+  for (int k = 0; k < nf; ++k) {
+    // argument (device - grid point) / d in single precision, as the reference forms it
+    wz[k] = double((z - (float(r0[0] + k) * float(ax->dz) + float(ax->oz))) / float(ax->dz));
+    wx[k] = double((x - (float(r0[1] + k) * float(ax->dx) + float(ax->ox))) / float(ax->dx));
+    wy[k] = double((y - (float(r0[2] + k) * float(ax->dy) + float(ax->oy))) / float(ax->dy));
+    wz[k] = sinc_pi_t(wz[k]);
+    wx[k] = sinc_pi_t(wx[k]);
+    wy[k] = sinc_pi_t(wy[k]);
+  }
+  int64_t e = 0;
+  for (int iy = 0; iy < nf; ++iy)
+    for (int ix = 0; ix < nf; ++ix)
+      for (int iz = 0; iz < nf; ++iz, ++e) {
+        idx[e] = nz * nx * (r0[2] + iy) + nz * (r0[1] + ix) + (r0[0] + iz);
+        w[e] = double(float(sign * wz[iz] * wx[ix] * wy[iy]));
+      }
+  return ELA_OK;
+}
+
+}  // namespace
+
+extern "C" int ela_interp_weights(const ela_interp_axes* ax, int64_t n_dev, const double* zc, const double* xc,
+                                  const double* yc, int method, int half_length, int dipole, double z_shift,
+                                  double x_shift, double y_shift, int64_t* nbr, double* weight,
+                                  int64_t* unique_pos, int64_t* n_unique) {
   if (!ax || n_dev < 0 || (n_dev > 0 && (!zc || !xc || !yc || !nbr || !weight || !unique_pos || !n_unique)))
     return ELA_ERR_ARG;
+  if (method != ELA_INTERP_LINEAR && method != ELA_INTERP_SINC) return ELA_ERR_ARG;
+  if (method == ELA_INTERP_LINEAR && half_length != 1) return ELA_ERR_ARG;
+  if (half_length < 1 || half_length > 32 || (dipole != 0 && dipole != 1)) return ELA_ERR_ARG;
   const int64_t nz = ax->nz, nx = ax->nx, ny = ax->ny, fat = ax->fat;
   // device bounds (checkOutOfBounds, spaceInterpGpu_3D.cpp:177-211)
   const float zMin = float(ax->oz + (fat + ax->z_pad_minus) * ax->dz);
@@ -210,49 +295,46 @@ extern "C" int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const
   const float xMax = xMin + float((nx - 2 * fat - ax->x_pad_minus - ax->x_pad_plus - 1) * ax->dx);
   const float yMax = yMin + float((ny - 2 * fat - 2 * ax->y_pad - 1) * ax->dy);
   const float tol = float(ax->tolerance);
+  // linear: both neighbours inside the unpadded model; sinc: the support inside the stepped interior
+  const InterpBounds lin{{fat + ax->z_pad_minus, fat + ax->x_pad_minus, fat + ax->y_pad},
+                         {nz - fat - ax->z_pad_plus, nx - fat - ax->x_pad_plus, ny - fat - ax->y_pad}};
+  const InterpBounds snc{{fat, fat, fat}, {nz - fat, nx - fat, ny - fat}};
+  const int64_t nf1 = method == ELA_INTERP_LINEAR ? 8 : int64_t(8) * half_length * half_length * half_length;
+  const int64_t nf = dipole ? 2 * nf1 : nf1;
+  std::vector<int64_t> idx(nf);
   std::unordered_map<int64_t, int64_t> seen;
-  seen.reserve(size_t(n_dev) * 8);
+  seen.reserve(size_t(n_dev) * size_t(nf));
   int64_t nu = 0;
   for (int64_t d = 0; d < n_dev; ++d) {
     const float z = float(zc[d]), x = float(xc[d]), y = float(yc[d]);
     if (z - zMax > tol || z - zMin < -tol || x - xMax > tol || x - xMin < -tol || y - yMax > tol || y - yMin < -tol)
       return ELA_ERR_ARG;
-    float wz = (z - float(ax->oz)) / float(ax->dz);
-    float wx = (x - float(ax->ox)) / float(ax->dx);
-    float wy = (y - float(ax->oy)) / float(ax->dy);
-    const int64_t zr = (int64_t)wz, xr = (int64_t)wx, yr = (int64_t)wy;
-    wz = 1.0f - (wz - float(zr));
-    wx = 1.0f - (wx - float(xr));
-    wy = 1.0f - (wy - float(yr));
-    if (yr < fat + ax->y_pad || yr + 1 >= ny - fat - ax->y_pad) return ELA_ERR_ARG;
-    if (xr < fat + ax->x_pad_minus || xr + 1 >= nx - fat - ax->x_pad_plus) return ELA_ERR_ARG;
-    if (zr < fat + ax->z_pad_minus || zr + 1 >= nz - fat - ax->z_pad_plus) return ELA_ERR_ARG;
-    const int64_t b = yr * nz * nx + xr * nz + zr;
-    const int64_t off[8] = {0, 1, nz, nz + 1, nz * nx, nz * nx + 1, nz * nx + nz, nz * nx + nz + 1};
-    const double az = wz, ax_ = wx, ay = wy;
-    const double w[8] = {az * ax_ * ay,
-                         (1.0 - az) * ax_ * ay,
-                         az * (1.0 - ax_) * ay,
-                         (1.0 - az) * (1.0 - ax_) * ay,
-                         az * ax_ * (1.0 - ay),
-                         (1.0 - az) * ax_ * (1.0 - ay),
-                         az * (1.0 - ax_) * (1.0 - ay),
-                         (1.0 - az) * (1.0 - ax_) * (1.0 - ay)};
-    for (int k = 0; k < 8; ++k) {
-      const int64_t p = b + off[k];
-      auto it = seen.find(p);
+    double* w = weight + d * nf;
+    if (pole_weights(ax, method, half_length, z, x, y, 1.0, lin, snc, idx.data(), w) != ELA_OK) return ELA_ERR_ARG;
+    if (dipole &&
+        pole_weights(ax, method, half_length, float(zc[d] + z_shift), float(xc[d] + x_shift), float(yc[d] + y_shift),
+                     -1.0, lin, snc, idx.data() + nf1, w + nf1) != ELA_OK)
+      return ELA_ERR_ARG;
+    for (int64_t k = 0; k < nf; ++k) {
+      auto it = seen.find(idx[k]);
       int64_t u;
       if (it == seen.end()) {
         u = nu++;
-        seen.emplace(p, u);
-        unique_pos[u] = p;
+        seen.emplace(idx[k], u);
+        unique_pos[u] = idx[k];
       } else {
         u = it->second;
       }
-      nbr[d * 8 + k] = u;
-      weight[d * 8 + k] = double(float(w[k]));  // the reference stores weights as float
+      nbr[d * nf + k] = u;
     }
   }
   *n_unique = nu;
   return ELA_OK;
+}
+
+extern "C" int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const double* zc, const double* xc,
+                                 const double* yc, int64_t* nbr, double* weight, int64_t* unique_pos,
+                                 int64_t* n_unique) {
+  return ela_interp_weights(ax, n_dev, zc, xc, yc, ELA_INTERP_LINEAR, 1, 0, 0.0, 0.0, 0.0, nbr, weight, unique_pos,
+                            n_unique);
 }

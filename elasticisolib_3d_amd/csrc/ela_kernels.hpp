@@ -86,6 +86,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// Groups of `gs` consecutive tiles dealt whole to one XCD: the k-th block that
+// XCD x receives runs tile ((k / gs) * NXCD + x) * gs + k % gs, so the tiles
+// running at one time still form one contiguous range of the enumeration (as
+// in the hardware order) but each group's halo lines meet in one L2. Blocks
+// past the last full round of NXCD groups keep the identity mapping.
+__device__ __forceinline__ int xcd_group(int bid, int nb, int gs) {
+  const int full = nb / (NXCD * gs) * (NXCD * gs);
+  if (bid >= full) return bid;
+  const int x = bid % NXCD, k = bid / NXCD;
+  return ((k / gs) * NXCD + x) * gs + k % gs;
+}
+
 // Sponge factor from the distance to the nearest face of the cell's grid (or
 // of its window when several windows are stacked along y).
 template <typename T>
@@ -254,11 +266,16 @@ struct TileCoord {
 // y-major enumeration (per z tile, groups of `xgroup` x columns, y chunks
 // within a group, columns fastest), so that the blocks an XCD runs at the
 // same time are y- and x-neighbours whose halo planes meet in its L2.
+// order 3: the z-fastest enumeration dealt to the XCDs in groups of whole z
+// columns (`xgroup` adjacent columns of nbz tiles), so that z (and, with
+// xgroup > 1, x) neighbours share an L2 while the grid is swept in the
+// hardware order.
 template <int BX>
 __device__ __forceinline__ TileCoord tile_coord(int nbz, int nbx, int nby, int ychunk, int nz, int nzi, int zoff,
                                                 int nx, int ny, int order, int xgroup, const int* chunks) {
   const int nb = nbz * nbx * nby;
-  const int lb = order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
+  const int lb = order == 3 ? xcd_group(blockIdx.x, nb, nbz * xgroup)
+                 : order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
   int bz, bx, by;
   if (order == 2) {
     const int per_z = nbx * nby;

@@ -258,19 +258,21 @@ def _device_of(par: ParObject) -> int:
 
 class spaceInterpGpu_3D(SpaceInterp):
     """One device set on one (possibly shifted) grid: the reference's
-    spaceInterpGpu_3D constructor (spaceInterpGpu_3D.cpp:13-79), linear
-    method only. The hypercube's origin is used as given (the builders shift
-    it by half a cell for the staggered grids)."""
+    spaceInterpGpu_3D constructor (spaceInterpGpu_3D.cpp:13-79) with the
+    linear or sinc method and optional dipoles (:214-480). The hypercube's
+    origin is used as given (the builders shift it by half a cell for the
+    staggered grids). The sinc support must lie inside the stepped interior,
+    the dipole shifts are real numbers and the dipole's y weight uses dy; the
+    reference's sinc check rejects every device, it truncates the shifts to
+    integers and divides the dipole y position by dz."""
 
     def __init__(self, zCoord, xCoord, yCoord, hyper: Hypercube, nts: int, par: ParObject, dipole: int = 0,
                  zDipoleShift: float = 0.0, xDipoleShift: float = 0.0, yDipoleShift: float = 0.0,
                  method: str = "linear", hFilter1d: int = 1):
         if method == "linear" and hFilter1d != 1:
             raise ValueError("Half-length of interpolation filter should be set to 1 for linear interpolation")
-        if method != "linear":
-            raise NotImplementedError(f"spatial interpolation method {method!r} is not supported (linear only)")
-        if dipole != 0:
-            raise NotImplementedError("dipole sources/receivers are not supported")
+        if method not in ("linear", "sinc"):
+            raise ValueError(f"spatial interpolation method {method!r} is not supported (linear or sinc)")
         z = np.asarray(zCoord.getNdArray() if hasattr(zCoord, "getNdArray") else zCoord, np.float64).ravel()
         x = np.asarray(xCoord.getNdArray() if hasattr(xCoord, "getNdArray") else xCoord, np.float64).ravel()
         y = np.asarray(yCoord.getNdArray() if hasattr(yCoord, "getNdArray") else yCoord, np.float64).ravel()
@@ -278,7 +280,8 @@ class spaceInterpGpu_3D(SpaceInterp):
         axes = PaddedAxes(az.n, ax_.n, ay.n, az.o, ax_.o, ay.o, az.d, ax_.d, ay.d, par.getInt("fat"),
                           par.getInt("zPadMinus"), par.getInt("zPadPlus"), par.getInt("xPadMinus"),
                           par.getInt("xPadPlus"), par.getInt("yPad"), par.getFloat("errorTolerance", 1e-6))
-        super().__init__(axes, z, x, y)
+        super().__init__(axes, z, x, y, method, int(hFilter1d), int(dipole),
+                         (float(zDipoleShift), float(xDipoleShift), float(yDipoleShift)))
         self.nt = int(nts)
 
     def getRegPosUnique(self) -> np.ndarray:
@@ -348,8 +351,8 @@ def buildReceiversGeometry_3D(parObject: ParObject, elasticParam: Vector, geomet
     ([3, nReceiverPerShot, 1 or nExp]) replaces receiverGeomFile."""
     nts = parObject.getInt("nts")
     dipole = parObject.getInt("dipole", 0)
-    # the reference reads the receiver dipole shifts as ints
-    zs, xs, ys = (parObject.getInt(k, 0) for k in ("zDipoleShift", "xDipoleShift", "yDipoleShift"))
+    # real shifts (the reference reads the receivers' with getInt, truncating them)
+    zs, xs, ys = (parObject.getFloat(k, 0.0) for k in ("zDipoleShift", "xDipoleShift", "yDipoleShift"))
     n_exp = parObject.getInt("nExp", -1)
     if n_exp == -1:
         raise ValueError("**** ERROR [buildReceiversGeometry_3D]: User must provide the total number of shots ****")

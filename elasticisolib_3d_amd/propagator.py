@@ -278,7 +278,8 @@ class Propagator:
 
     def set_tile_order(self, order: int, x_group: int = 1) -> None:
         """0 hardware order, 1 contiguous per XCD, 2 y-major per XCD in groups
-        of x_group columns (speed only)."""
+        of x_group columns, 3 hardware order with groups of x_group whole z
+        columns per XCD (speed only)."""
         check(self.lib.ela_set_tile_order(self.h, int(order), int(x_group)), "ela_set_tile_order")
 
     def set_born_storage(self, segment: int = 0, budget_bytes: int = 0) -> None:

@@ -20,6 +20,8 @@
  *                                 (elastic_iso_float/src/fdParamElastic_3D.cpp:130-183)
  *   ela_interp_linear             spaceInterpGpu_3D's constructor, linear method
  *                                 (elastic_iso_float/include/spaceInterpGpu_3D.h:41)
+ *   ela_interp_weights            the same constructor with the sinc method and dipoles
+ *                                 (elastic_iso_float/src/spaceInterpGpu_3D.cpp:13-79, 214-480)
  *   ela_born_stagger              perturbation staggering/scaling of BornElasticShotsGpu_3D::forward
  *                                 (elastic_iso_float/src/BornElasticShotsGpu_3D.cpp:167-235)
  *   ela_born_unstagger            its adjoint in BornElasticShotsGpu_3D::adjoint
@@ -241,6 +243,30 @@ int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const double* z,
                       const double* y, int64_t* nbr, double* weight, int64_t* unique_pos,
                       int64_t* n_unique);
 
+#define ELA_INTERP_LINEAR 0
+#define ELA_INTERP_SINC   1
+
+/* Interpolation weights of n_dev devices on one grid with the linear or the
+ * sinc method and an optional dipole: spaceInterpGpu_3D's constructor with
+ * interpMethod, hFilter1d, dipole and the dipole shifts
+ * (elastic_iso_float/src/spaceInterpGpu_3D.cpp:13-79, 214-480).
+ * Each device has nf = (dipole ? 2 : 1) * (2*half_length)^3 entries
+ * (linear: half_length must be 1, 8 corners in the reference's order; sinc:
+ * the (2h)^3 support y slowest, weights sinc(dz)sinc(dx)sinc(dy)); with a
+ * dipole the second nf/2 entries are the negative pole at the device shifted
+ * by (z_shift, x_shift, y_shift). Outputs nbr[nf*n_dev], weight[nf*n_dev],
+ * unique_pos[<= nf*n_dev], *n_unique as for ela_interp_linear.
+ * Corrections to the reference (its sinc and dipole paths are otherwise
+ * unusable): the sinc support must lie inside the stepped interior (the
+ * reference's inverted check rejects every device), the dipole y weight
+ * divides by dy (not dz), and the sinc dipole pole is negative as the linear
+ * one is. Returns ELA_ERR_ARG for a device or support point out of bounds or
+ * a bad method/half-length/dipole flag. */
+int ela_interp_weights(const ela_interp_axes* ax, int64_t n_dev, const double* z, const double* x,
+                       const double* y, int method, int half_length, int dipole, double z_shift,
+                       double x_shift, double y_shift, int64_t* nbr, double* weight, int64_t* unique_pos,
+                       int64_t* n_unique);
+
 /* Time of the last propagation's fine-step loop on the GPU, measured with
  * HIP events on the handle's stream [ms], and the number of fine steps and
  * interior cells it advanced. */
@@ -274,7 +300,8 @@ int ela_get_tiling(const ela_handle* h, int* tile_x, int* y_chunk);
  * enumerated z, x, y; 1 each XCD a contiguous range of that enumeration; 2
  * each XCD a contiguous range of a y-major enumeration in groups of x_group
  * x columns, so that concurrently running neighbours share halo planes in
- * the XCD's L2. */
+ * the XCD's L2; 3 the order-0 sweep with whole z columns (x_group adjacent
+ * columns) dealt to one XCD, so that z and x neighbours share its L2. */
 int ela_set_tile_order(ela_handle* h, int order, int x_group);
 
 /* Diagnostic timing: launch the velocity and the stress stencil kernels of a

@@ -200,7 +200,7 @@ def test_device_outputs_ordered_with_default_stream(problems):
             assert abs(n - ref_norm) > 1e-6 * ref_norm
 
 
-@pytest.mark.parametrize("order,xgroup", [(1, 1), (2, 1), (2, 3), (2, 64)])
+@pytest.mark.parametrize("order,xgroup", [(1, 1), (2, 1), (2, 3), (2, 64), (3, 1), (3, 2), (3, 64)])
 def test_tile_orders_give_identical_results(problems, order, xgroup):
     """Every block-to-tile order covers every tile exactly once: the results
     are bit-identical to the hardware order, with and without partial x

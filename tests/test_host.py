@@ -191,16 +191,16 @@ def test_autotuner_picks_fastest_and_caches(tmp_path, monkeypatch):
             self.calls += 1
             tx, yc = self.tiling
             o, g = self.order
-            t = 1.0 + 0.1 * abs(yc - 16) + 0.05 * (16 - tx) + (0.0 if (o, g) == (2, 2) else 0.02)
+            t = 1.0 + 0.1 * abs(yc - 16) + 0.05 * (16 - tx) + (0.0 if (o, g) == (3, 2) else 0.02)
             return t / 2, t / 2
 
     p = FakeProp()
     best = A.autotune(p)
-    assert (best["tile_x"], best["y_chunk"], best["order"], best["x_group"]) == (16, 16, 2, 2)
-    assert p.tiling == (16, 16) and p.order == (2, 2) and not best["cached"]
+    assert (best["tile_x"], best["y_chunk"], best["order"], best["x_group"]) == (16, 16, 3, 2)
+    assert p.tiling == (16, 16) and p.order == (3, 2) and not best["cached"]
     q = FakeProp()
     again = A.autotune(q)
-    assert again["cached"] and q.calls == 0 and q.tiling == (16, 16) and q.order == (2, 2)
+    assert again["cached"] and q.calls == 0 and q.tiling == (16, 16) and q.order == (3, 2)
 
 
 def test_positions_follow_the_memory_kind_of_the_call():

@@ -74,8 +74,13 @@ def test_geometry_builders_and_layouts():
     assert len(cx) == 8 and src[1][0].getNDeviceIrreg() == 1
     with pytest.raises(ValueError):
         OP.buildReceiversGeometry_3D(PB.make_par(nReceiverPerShot=4), ep, rgeo)
-    with pytest.raises(NotImplementedError):
-        OP.buildSourceGeometry_3D(PB.make_par(spaceInterpMethod="sinc", hFilter1d=2), ep, sgeo)
+    sinc = OP.buildSourceGeometry_3D(PB.make_par(spaceInterpMethod="sinc", hFilter1d=2), ep, sgeo)
+    assert sinc[0][0].weight.shape == (1, 64) and sinc[0][0].getNDeviceReg() == 64
+    dip = OP.buildReceiversGeometry_3D(PB.make_par(dipole=1, zDipoleShift=0.5 * PB.D), ep, rgeo)
+    assert dip[0][0].weight.shape == (5, 16)
+    assert np.allclose(dip[0][0].weight.sum(axis=1), 0.0, atol=1e-6)
+    with pytest.raises(ValueError):
+        OP.buildSourceGeometry_3D(PB.make_par(spaceInterpMethod="cubic"), ep, sgeo)
 
 
 def test_elastic_conversion():
@@ -103,9 +108,9 @@ def test_elastic_conversion():
     assert np.allclose(lin.getNdArray(), fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
 
 
-def _nl_op(monkeypatch, const_src=True):
+def _nl_op(monkeypatch, const_src=True, **extra):
     PB.use_oracle_backend(monkeypatch)
-    par = PB.make_par()
+    par = PB.make_par(**extra)
     ep = PB.elastic_vector()
     sgeo, rgeo = PB.geometries()
     src = OP.buildSourceGeometry_3D(par, ep, sgeo)
@@ -122,6 +127,16 @@ def test_nonlinear_shots_operator_dot_product(monkeypatch, const_src):
     op, model, data, *_ = _nl_op(monkeypatch, const_src)
     assert op.dotTestCpp(maxError=1e-10, seed=5)
     op.dotTest(maxError=1e-10, seed=6)
+
+
+@pytest.mark.parametrize("extra", [dict(spaceInterpMethod="sinc", hFilter1d=2),
+                                   dict(dipole=1, zDipoleShift=0.5 * PB.D, xDipoleShift=-0.25 * PB.D),
+                                   dict(spaceInterpMethod="sinc", hFilter1d=1, dipole=1, yDipoleShift=PB.D)])
+def test_nonlinear_shots_operator_dot_product_sinc_and_dipole(monkeypatch, extra):
+    """Sinc and dipole devices keep the operator pair adjoint (their weights
+    enter both directions through the same sparse matrix)."""
+    op, model, data, *_ = _nl_op(monkeypatch, False, **extra)
+    assert op.dotTestCpp(maxError=1e-10, seed=5)
 
 
 def test_nonlinear_shots_operator_matches_single_shot(monkeypatch):
