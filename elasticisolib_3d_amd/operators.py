@@ -133,11 +133,13 @@ class FdParam:
     def __init__(self, elasticParam: Vector, par: ParObject):
         self.par = par
         hyp = elasticParam.getHyper()
+        # freeSurface=1 only changes the sponge width rule (zPad = zPadPlus,
+        # fdParamElastic_3D.cpp:44-45): the reference prints the Robertsson
+        # free-surface message but its stepper applies no free-surface condition
+        # and still damps towards the top face, and so does this one.
         self.free_surface = par.getInt("freeSurface", 0)
         if self.free_surface not in (0, 1):
             raise ValueError("freeSurface must be 0 or 1")
-        if self.free_surface == 1:
-            raise NotImplementedError("freeSurface=1 is not supported by this propagator")
         self.nts = par.getInt("nts")
         self.dts = par.getFloat("dts", 0.0)
         self.ots = par.getFloat("ots", 0.0)
@@ -149,7 +151,7 @@ class FdParam:
         self.nz = par.getInt("nz")
         self.z_pad_plus = par.getInt("zPadPlus")
         self.z_pad_minus = par.getInt("zPadMinus")
-        self.z_pad = min(self.z_pad_minus, self.z_pad_plus)
+        self.z_pad = self.z_pad_plus if self.free_surface == 1 else min(self.z_pad_minus, self.z_pad_plus)
         self.mod_par = par.getInt("mod_par", 2)
         self.dz = par.getFloat("dz", -1.0)
         self.oz = hyp.getAxis(1).o

@@ -53,6 +53,13 @@ def test_fdparam_qc_and_staggering():
         OP.FdParam(ep, PB.make_par(nz=PB.NZ + 1))
     with pytest.raises(ValueError, match="SUB_MAX"):
         OP.FdParam(ep, PB.make_par(sub=51))
+    # freeSurface=1 takes the sponge width from zPadPlus and the x pads only
+    # (fdParamElastic_3D.cpp:44-45, 88); no other change
+    fs = OP.FdParam(ep, PB.make_par(freeSurface=1, zPadMinus=4, zPadPlus=6, xPadMinus=6, xPadPlus=6))
+    assert fs.min_pad == 6
+    assert OP.FdParam(ep, PB.make_par(zPadMinus=4, zPadPlus=6, xPadMinus=6, xPadPlus=6)).min_pad == 4
+    with pytest.raises(ValueError):
+        OP.FdParam(ep, PB.make_par(freeSurface=2))
     # without mod_par the C++ default (2) multiplies spacings by 1000: the
     # metre-valued hypercube then fails the consistency check
     par2 = PB.make_par()
