@@ -273,7 +273,7 @@ def bench_fwi(args):
     start = w.extra["start"]
     log(f"[bench] model {w.name} padded {ax.nz}x{ax.nx}x{ax.ny} built in {time.time() - t0:.1f}s")
     shots = _fwi_shots(w, ax)
-    prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, bf16=True)
+    prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, checkpoint=args.checkpoint)
     t0 = time.perf_counter()
     prob.synthesize(w.rho, w.lame, w.mu)
     log(f"[bench] observed data of {len(shots)} shots synthesized in {time.perf_counter() - t0:.1f}s")
@@ -294,7 +294,7 @@ def bench_fwi(args):
     acc = {}
     if args.fd_cells >= 0:
         one = shots[:1]
-        p64 = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f64", interval=20, bf16=False)
+        p64 = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f64", interval=20, checkpoint="native")
         p64.synthesize(w.rho, w.lame, w.mu)
         t0 = time.perf_counter()
         J64, g64 = p64.misfit_and_gradient(*start)
@@ -319,8 +319,8 @@ def bench_fwi(args):
         del p64
         torch.cuda.empty_cache()
         res = {}
-        for name, bf16 in (("f32_bf16", True), ("f32_exact", False)):
-            p = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f32", interval=20, bf16=bf16)
+        for name, ck in (("f32_bf16", "bf16"), ("f32_s16", "s16"), ("f32_exact", "native")):
+            p = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f32", interval=20, checkpoint=ck)
             t0 = time.perf_counter()
             Jx, gx = p.misfit_and_gradient(*start)
             res[name] = (Jx, gx, time.perf_counter() - t0, p.prop.gradient_plan_used())
@@ -336,7 +336,7 @@ def bench_fwi(args):
                 "grad_rel_l2_vs_fp64": rel(gx, g64),
                 "grad_rel_l2_per_param": [rel(gx[k], g64[k]) for k in range(3)],
                 "misfit_rel_vs_fp64": abs(Jx - J64) / J64,
-                "checkpoint_plan": {"interval": px[0], "outer_interval": px[1], "bf16": px[2],
+                "checkpoint_plan": {"interval": px[0], "outer_interval": px[1], "format": px[2],
                                     "checkpoint_GB": round(px[3] / 1e9, 2)},
             }
             if fd_rows:
@@ -359,14 +359,14 @@ def bench_fwi(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 stencil, bf16 checkpoints, f64 gradient accumulation",
+        "dtype": f"f32 stencil, {args.checkpoint} checkpoints, f64 gradient accumulation",
         "data": "synthetic",
         "config": {
             "workload": "FWI gradient, 256x256x160 layered model (padded 232x328x324), d=15 m, start = true smoothed "
                         f"sigma=20 cells, {len(shots)} explosive shots (8 Hz Ricker), {w.nts} coarse samples x {w.sub} "
                         f"fine steps, {len(w.rec_xyz)} surface receivers, L2 misfit, checkpoints every 20 fine steps",
             "gradient_s_per_call": [round(t, 2) for t in times],
-            "checkpoint_plan": {"interval": plan[0], "outer_interval": plan[1], "bf16": plan[2],
+            "checkpoint_plan": {"interval": plan[0], "outer_interval": plan[1], "format": plan[2],
                                 "checkpoint_GB_per_shot": round(plan[3] / 1e9, 2)},
             "interior_cells": w.interior_cells,
             "fine_steps_per_propagation": steps,
@@ -397,7 +397,7 @@ def bench_fwi_batch(args):
             sig[c][0] = cfg["wavelet"]
         shots.append(BatchShot(src, rec, Window(x0, y0, wx, wy), sig))
     shape = cfg["true"][0].shape
-    kw = dict(precision="f32", interval=20, bf16=True)
+    kw = dict(precision="f32", interval=20, checkpoint=args.checkpoint)
     ref = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], shots, batch=1, **kw)
     t0 = time.perf_counter()
     ref.synthesize(cfg["true"])
@@ -420,7 +420,7 @@ def bench_fwi_batch(args):
         "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(tb * 1e3, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 stencil, bf16 checkpoints, f64 gradient accumulation",
+        "dtype": f"f32 stencil, {args.checkpoint} checkpoints, f64 gradient accumulation",
         "data": "synthetic",
         "config": {
             "workload": f"FWI gradient of {len(shots)} shots with shot-local windows on the 400x400x250 layered "
@@ -462,6 +462,8 @@ def main():
     ap.add_argument("--batch", type=int, default=2, help="fwi_batch: shots per batched launch sequence")
     ap.add_argument("--fd-cells", type=int, default=16,
                     help="fwi: cells checked by central differences (-1: skip the accuracy part)")
+    ap.add_argument("--checkpoint", choices=("bf16", "s16", "native"), default="bf16",
+                    help="fwi, fwi_batch: storage of the gradient's outer checkpoints")
     ap.add_argument("--born-segment", type=int, default=0,
                     help="Born adjoint storage: 0 automatic, -1 all samples, K checkpoint segment")
     args = ap.parse_args()

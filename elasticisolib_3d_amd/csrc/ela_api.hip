@@ -140,7 +140,7 @@ struct ela_handle {
   int64_t born_budget = 0;  // device-memory budget of the automatic plan [bytes], 0: free memory
   int64_t born_last_segment = 0;  // plan of the last Born adjoint (-1 full storage)
   // adjoint-state gradient
-  DevBuf ck1;               // outer checkpoints [n1][18][interior] (bfloat16 or native)
+  DevBuf ck1;               // outer checkpoints [n1][18][interior] (bfloat16, block-scaled int16 or native)
   DevBuf ck2;               // inner checkpoints of one outer segment [K1/K][18][interior] (native)
   DevBuf ckscale;           // per-checkpoint array scales [n1][18] + amax scratch [18]
   DevBuf drv;               // derivative combinations of one inner segment [K][9][stride]
@@ -149,11 +149,11 @@ struct ela_handle {
   DevBuf stage2;            // host-call staging of the source-trace gradient
   int64_t grad_interval = 20;   // inner interval K (fine steps)
   int64_t grad_outer = 0;       // outer interval K1 (0: automatic)
-  int grad_bf16 = 1;            // outer checkpoints in bfloat16
+  int grad_fmt = ELA_CK_BF16;   // storage of the outer checkpoints
   int64_t grad_budget = 0;      // device-memory budget of the automatic plan (0: free memory)
   bool grad_ready = false;      // a gradient forward has stored its checkpoints
   int64_t plan_K = 0, plan_K1 = 0, plan_n1 = 0;
-  int plan_bf16 = 0;
+  int plan_fmt = 0;
   // windows stacked along y (several shots in one grid)
   std::vector<int64_t> win_y0, win_ny, win_nx;
   DevBuf ytab;              // per plane {window y0, ny, nx, 0} (int32)
@@ -1030,7 +1030,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
 
 struct GradPlan {
   int64_t K = 20, K1 = 20, n1 = 0, n2 = 1;
-  bool bf16 = true;
+  int fmt = ELA_CK_BF16;
   size_t ck1_bytes = 0, ck2_bytes = 0;  // one checkpoint
 };
 
@@ -1041,12 +1041,19 @@ int64_t interior_count(const ela_handle* h) {
   return (h->cfg.nz - 2 * ela::FAT) * (h->cfg.nx - 2 * ela::FAT) * (h->cfg.ny - 2 * ela::FAT);
 }
 
+// Bytes of one checkpoint (18 interior arrays) in a storage format.
+size_t ck_bytes(int fmt, int64_t nI, int esize) {
+  if (fmt == ELA_CK_BF16) return (size_t)18 * nI * 2;
+  if (fmt == ELA_CK_S16) return (size_t)18 * (nI * 2 + (nI + ela::CK_BLOCK - 1) / ela::CK_BLOCK * 4);
+  return (size_t)18 * nI * esize;
+}
+
 int grad_plan(ela_handle* h, GradPlan& p) {
   const int64_t N = (h->cfg.nts - 1) * h->cfg.sub;
   const int64_t nI = interior_count(h);
   p.K = std::max<int64_t>(1, std::min<int64_t>(h->grad_interval, N));
-  p.bf16 = h->grad_bf16 != 0;
-  p.ck1_bytes = (size_t)18 * nI * (p.bf16 ? 2 : h->esize);
+  p.fmt = h->grad_fmt;
+  p.ck1_bytes = ck_bytes(p.fmt, nI, h->esize);
   p.ck2_bytes = (size_t)18 * nI * h->esize;
   const double arr = double(h->stride) * h->esize;
   const double fixed = arr * (9.0 * p.K + 8 + 36) + double(h->stride) * 8.0 * 8;  // drv, g32, wave2, g64
@@ -1103,11 +1110,15 @@ ela::CkArgs<T> ck_args(ela_handle* h, const Wave<T>& w) {
 dim3 ck_grid(int64_t nI) { return dim3((unsigned)std::min<int64_t>((nI + 255) / 256, 4096), 18); }
 
 template <typename T>
-int save_checkpoint(ela_handle* h, const Wave<T>& w, void* out, bool bf16, float* scale) {
+int save_checkpoint(ela_handle* h, const Wave<T>& w, void* out, int fmt, float* scale) {
   ela::CkArgs<T> a = ck_args<T>(h, w);
   const dim3 grid = ck_grid(a.nI);
+  if (fmt == ELA_CK_S16) {
+    hipLaunchKernelGGL((ela::k_ck_pack_s16<T>), grid, dim3(256), 0, h->stream, a, static_cast<int16_t*>(out));
+    return ELA_OK;
+  }
   uint32_t* amax = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(h->ckscale.p) + h->ckscale.bytes - 18 * 4);
-  if (bf16) {
+  if (fmt == ELA_CK_BF16) {
     hipLaunchKernelGGL((ela::k_fill<uint32_t>), dim3(1), dim3(64), 0, h->stream, amax, (int64_t)18, 0u);
     hipLaunchKernelGGL((ela::k_ck_amax<T>), grid, dim3(256), 0, h->stream, a, amax);
     hipLaunchKernelGGL((ela::k_ck_pack<T, true>), grid, dim3(256), 0, h->stream, a, out, amax, scale);
@@ -1118,9 +1129,12 @@ int save_checkpoint(ela_handle* h, const Wave<T>& w, void* out, bool bf16, float
 }
 
 template <typename T>
-int load_checkpoint(ela_handle* h, const Wave<T>& w, const void* in, bool bf16, const float* scale) {
+int load_checkpoint(ela_handle* h, const Wave<T>& w, const void* in, int fmt, const float* scale) {
   ela::CkArgs<T> a = ck_args<T>(h, w);
-  if (bf16) hipLaunchKernelGGL((ela::k_ck_unpack<T, true>), ck_grid(a.nI), dim3(256), 0, h->stream, a, in, scale);
+  if (fmt == ELA_CK_S16)
+    hipLaunchKernelGGL((ela::k_ck_unpack_s16<T>), ck_grid(a.nI), dim3(256), 0, h->stream, a,
+                       static_cast<const int16_t*>(in));
+  else if (fmt == ELA_CK_BF16) hipLaunchKernelGGL((ela::k_ck_unpack<T, true>), ck_grid(a.nI), dim3(256), 0, h->stream, a, in, scale);
   else hipLaunchKernelGGL((ela::k_ck_unpack<T, false>), ck_grid(a.nI), dim3(256), 0, h->stream, a, in, scale);
   return ELA_OK;
 }
@@ -1176,7 +1190,7 @@ int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in
   for (int64_t k = 0; k < N; ++k) {
     if (k % plan.K1 == 0) {
       const int64_t s1 = k / plan.K1;
-      CHK(save_checkpoint<T>(h, w, (char*)h->ck1.p + s1 * plan.ck1_bytes, plan.bf16,
+      CHK(save_checkpoint<T>(h, w, (char*)h->ck1.p + s1 * plan.ck1_bytes, plan.fmt,
                              (float*)h->ckscale.p + s1 * 18));
     }
     const int its = (int)(k / sub), i2 = (int)(k % sub);
@@ -1200,7 +1214,7 @@ int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in
     CHK(stage_out_traces(h, rp_in, dr, rt_in));
     HIPCHK(hipStreamSynchronize(h->stream));
   }
-  h->plan_K = plan.K; h->plan_K1 = plan.K1; h->plan_n1 = plan.n1; h->plan_bf16 = plan.bf16;
+  h->plan_K = plan.K; h->plan_K1 = plan.K1; h->plan_n1 = plan.n1; h->plan_fmt = plan.fmt;
   h->grad_ready = true;
   return ELA_OK;
 }
@@ -1214,9 +1228,9 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, N = (nts - 1) * sub, np = h->stride;
   GradPlan plan;
   plan.K = h->plan_K; plan.K1 = h->plan_K1; plan.n1 = h->plan_n1; plan.n2 = plan.K1 / plan.K;
-  plan.bf16 = h->plan_bf16 != 0;
+  plan.fmt = h->plan_fmt;
   const int64_t nI = interior_count(h);
-  plan.ck1_bytes = (size_t)18 * nI * (plan.bf16 ? 2 : h->esize);
+  plan.ck1_bytes = ck_bytes(plan.fmt, nI, h->esize);
   plan.ck2_bytes = (size_t)18 * nI * h->esize;
 
   DevPoints sp, rp;
@@ -1297,18 +1311,20 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
     const int64_t nseg = (a1 - a0 + plan.K - 1) / plan.K;
     const void* c1 = (const char*)h->ck1.p + s1 * plan.ck1_bytes;
     if (nseg > 1) {
-      CHK(load_checkpoint<T>(h, bg, c1, plan.bf16, scales + s1 * 18));
+      CHK(load_checkpoint<T>(h, bg, c1, plan.fmt, scales + s1 * 18));
       for (int64_t k = a0; k < a1; ++k) {
         if ((k - a0) % plan.K == 0 && (k - a0) / plan.K < nseg)
-          CHK(save_checkpoint<T>(h, bg, (char*)h->ck2.p + ((k - a0) / plan.K) * plan.ck2_bytes, false, nullptr));
+          CHK(save_checkpoint<T>(h, bg, (char*)h->ck2.p + ((k - a0) / plan.K) * plan.ck2_bytes, ELA_CK_NATIVE,
+                                 nullptr));
         CHK((grad_fwd_step<T, false>(h, bg, sp, src_tr, k, nullptr)));
       }
       steps += a1 - a0;
     }
     for (int64_t s2 = nseg - 1; s2 >= 0; --s2) {
       const int64_t j0 = a0 + s2 * plan.K, j1 = std::min(j0 + plan.K, a1);
-      if (nseg > 1) CHK(load_checkpoint<T>(h, bg, (const char*)h->ck2.p + s2 * plan.ck2_bytes, false, nullptr));
-      else CHK(load_checkpoint<T>(h, bg, c1, plan.bf16, scales + s1 * 18));
+      if (nseg > 1)
+        CHK(load_checkpoint<T>(h, bg, (const char*)h->ck2.p + s2 * plan.ck2_bytes, ELA_CK_NATIVE, nullptr));
+      else CHK(load_checkpoint<T>(h, bg, c1, plan.fmt, scales + s1 * 18));
       for (int64_t k = j0; k < j1; ++k) CHK((grad_fwd_step<T, true>(h, bg, sp, src_tr, k, drv + (k - j0) * 9 * np)));
       for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(k, drv + (k - j0) * 9 * np));
       steps += 2 * (j1 - j0);
@@ -1510,27 +1526,29 @@ int ela_born_adjoint(ela_handle* h, const ela_points* sp, const ela_traces* st, 
                        : born_run<double>(h, sp, st, nullptr, dmod8, rp, rt, true, mem);
 }
 
-int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_interval, int bf16,
+int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_interval, int format,
                                  int64_t budget_bytes) {
   if (!h) return fail(ELA_ERR_ARG, "null handle");
   if (interval < 1) return fail(ELA_ERR_ARG, "checkpoint interval must be >= 1");
   if (outer_interval < 0 || (outer_interval > 0 && outer_interval % interval))
     return fail(ELA_ERR_ARG, "outer interval must be 0 (automatic) or a multiple of the interval");
   if (budget_bytes < 0) return fail(ELA_ERR_ARG, "negative memory budget");
+  if (format != ELA_CK_NATIVE && format != ELA_CK_BF16 && format != ELA_CK_S16)
+    return fail(ELA_ERR_ARG, "checkpoint format must be ELA_CK_NATIVE, ELA_CK_BF16 or ELA_CK_S16");
   h->grad_interval = interval;
   h->grad_outer = outer_interval;
-  h->grad_bf16 = bf16 ? 1 : 0;
+  h->grad_fmt = format;
   h->grad_budget = budget_bytes;
   h->grad_ready = false;
   return ELA_OK;
 }
 
-int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* bf16,
+int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* format,
                            int64_t* checkpoint_bytes) {
-  if (!h || !interval || !outer_interval || !bf16 || !checkpoint_bytes) return fail(ELA_ERR_ARG, "null argument");
+  if (!h || !interval || !outer_interval || !format || !checkpoint_bytes) return fail(ELA_ERR_ARG, "null argument");
   *interval = h->plan_K;
   *outer_interval = h->plan_K1;
-  *bf16 = h->plan_bf16;
+  *format = h->plan_fmt;
   *checkpoint_bytes = (int64_t)h->ck1.bytes;
   return ELA_OK;
 }

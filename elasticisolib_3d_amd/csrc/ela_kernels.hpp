@@ -956,6 +956,47 @@ __global__ void __launch_bounds__(256) k_ck_unpack(CkArgs<T> a, const void* in, 
   }
 }
 
+// 16-bit checkpoints with one scale per block of CK_BLOCK consecutive packed
+// values: q = rint(x / s) with s = max |x| / 32767 over the block (an
+// all-zero block has s = 0), restored as x = q s. The absolute error is at
+// most s / 2 (2^-16 of the block's largest value), against 2^-9 of each value
+// for bfloat16. Layout of one checkpoint: int16 [18][nI], then float scales
+// [18][ceil(nI / CK_BLOCK)] (36 nI bytes precede them, so they are 4-byte
+// aligned). One wave quantises one block (the max is a wave reduction).
+constexpr int CK_BLOCK = 64;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ck_pack_s16(CkArgs<T> a, int16_t* out) {
+  const int f = blockIdx.y;
+  const int64_t nb = (a.nI + CK_BLOCK - 1) / CK_BLOCK;
+  int16_t* q16 = out + f * a.nI;
+  float* sc = reinterpret_cast<float*>(out + 18 * a.nI) + f * nb;
+  const int lane = threadIdx.x & (CK_BLOCK - 1);
+  const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / CK_BLOCK);
+  // b is uniform over a wave (blockDim.x is a multiple of 64), so every lane joins the reduction
+  for (int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CK_BLOCK; b < nb; b += wstride) {
+    const int64_t q = b * CK_BLOCK + lane;
+    const bool ok = q < a.nI;
+    const float x = ok ? (float)a.f[f][ck_offset(a, q)] : 0.f;
+    float m = fabsf(x);
+#pragma unroll
+    for (int o = CK_BLOCK / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, CK_BLOCK));
+    const float inv = m > 0.f ? 32767.f / m : 0.f;
+    if (ok) q16[q] = (int16_t)__float2int_rn(x * inv);
+    if (lane == 0) sc[b] = m / 32767.f;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ck_unpack_s16(CkArgs<T> a, const int16_t* in) {
+  const int f = blockIdx.y;
+  const int64_t nb = (a.nI + CK_BLOCK - 1) / CK_BLOCK;
+  const int16_t* q16 = in + f * a.nI;
+  const float* sc = reinterpret_cast<const float*>(in + 18 * a.nI) + f * nb;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.nI; q += (int64_t)gridDim.x * blockDim.x)
+    a.f[f][ck_offset(a, q)] = (T)((float)q16[q] * sc[q / CK_BLOCK]);
+}
+
 // acc[i] += (double) part[i]; part[i] = 0 (FP64 time accumulation of the
 // gradient from the per-segment partial sums).
 template <typename T>

@@ -81,7 +81,7 @@ class FwiProblem:
 
     def __init__(self, axes: PaddedAxes, nts: int, dts: float, sub: int, alpha_cos: float, shots: Sequence[FwiShot],
                  precision: str = "f32", interval: int = 20, outer_interval: int = 0, bf16: bool = True,
-                 device: int = 0):
+                 device: int = 0, checkpoint=None):
         self.axes = axes
         self.nts, self.dts, self.sub = int(nts), float(dts), int(sub)
         self.dtw = self.dts / self.sub
@@ -90,7 +90,8 @@ class FwiProblem:
         self.dev = f"cuda:{device}"
         self.prop = P.Propagator(axes.nz, axes.nx, axes.ny, nts, sub, axes.dz, axes.dx, axes.dy, dts, axes.min_pad,
                                  alpha_cos, precision, device)
-        self.prop.set_gradient_checkpoints(interval, outer_interval, bf16)
+        # `checkpoint` ("bf16", "s16", "native") overrides the older bf16 flag
+        self.prop.set_gradient_checkpoints(interval, outer_interval, bf16 if checkpoint is None else checkpoint)
         self.dxyz = axes.dx * axes.dy * axes.dz
         self.last_timing = {}
 

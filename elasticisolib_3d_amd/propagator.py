@@ -31,6 +31,19 @@ def _is_torch(x) -> bool:
     return hasattr(x, "data_ptr") and hasattr(x, "is_cuda")
 
 
+CHECKPOINT_FORMATS = {"native": 0, "bf16": 1, "s16": 2}   # ELA_CK_NATIVE, ELA_CK_BF16, ELA_CK_S16
+CHECKPOINT_NAMES = {v: k for k, v in CHECKPOINT_FORMATS.items()}
+
+
+def checkpoint_format(checkpoint) -> int:
+    """ELA_CK_* code of a checkpoint format name (True / False: "bf16" / "native")."""
+    if isinstance(checkpoint, (bool, np.bool_)):
+        return CHECKPOINT_FORMATS["bf16" if checkpoint else "native"]
+    if checkpoint in CHECKPOINT_FORMATS:
+        return CHECKPOINT_FORMATS[checkpoint]
+    raise ValueError(f"checkpoint format must be one of {sorted(CHECKPOINT_FORMATS)}, got {checkpoint!r}")
+
+
 class Propagator:
     """One propagator handle (one grid, one time axis, one device)."""
 
@@ -247,17 +260,22 @@ class Propagator:
         self._keep.clear()
         return grad8, gs
 
-    def set_gradient_checkpoints(self, interval: int = 20, outer_interval: int = 0, bf16: bool = True,
+    def set_gradient_checkpoints(self, interval: int = 20, outer_interval: int = 0, checkpoint="bf16",
                                  budget_bytes: int = 0) -> None:
-        check(self.lib.ela_set_gradient_checkpoints(self.h, int(interval), int(outer_interval), int(bool(bf16)),
-                                                    int(budget_bytes)), "ela_set_gradient_checkpoints")
+        """`checkpoint`: storage of the outer checkpoints, "bf16" (bfloat16,
+        one power-of-two scale per array and snapshot), "s16" (int16, one
+        scale per 64 values) or "native" (exact); True / False mean "bf16" /
+        "native"."""
+        check(self.lib.ela_set_gradient_checkpoints(self.h, int(interval), int(outer_interval),
+                                                    checkpoint_format(checkpoint), int(budget_bytes)),
+              "ela_set_gradient_checkpoints")
 
     def gradient_plan_used(self):
-        """(interval, outer_interval, bf16, checkpoint bytes) of the last gradient forward."""
+        """(interval, outer_interval, checkpoint format name, checkpoint bytes) of the last gradient forward."""
         k, k1, b, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()
         check(self.lib.ela_gradient_plan_used(self.h, ctypes.byref(k), ctypes.byref(k1), ctypes.byref(b),
                                               ctypes.byref(nb)), "ela_gradient_plan_used")
-        return k.value, k1.value, bool(b.value), nb.value
+        return k.value, k1.value, CHECKPOINT_NAMES[b.value], nb.value
 
     def set_windows(self, y0, ny, nx) -> None:
         """Stack windows along y (see ela_set_windows); empty lists remove them."""

@@ -207,19 +207,26 @@ int ela_gradient_forward(ela_handle* h, const ela_points* src_pos, const ela_tra
 int ela_gradient_backward(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
                           const ela_points* rec_pos, const ela_traces* adj, double* grad8,
                           ela_traces* src_grad, int mem);
+/* Storage formats of the gradient's outer checkpoints (2 or 4/8 bytes per value). */
+#define ELA_CK_NATIVE 0   /* the handle's precision, exact */
+#define ELA_CK_BF16   1   /* bfloat16, one power-of-two scale per array and snapshot */
+#define ELA_CK_S16    2   /* int16, one scale per 64 consecutive values (2.06 bytes per value) */
+
 /* Checkpointing of the gradient forward: every `interval` fine steps the
  * backward pass recomputes a segment and stores its derivatives; outer
  * checkpoints are taken every `outer_interval` steps (a multiple of
  * `interval`; 0: the smallest memory-feasible choice, equal to `interval` when
- * it fits), in bfloat16 scaled per array and per snapshot by a power of two
- * (bf16 = 1) or in the handle's precision (bf16 = 0). With outer_interval >
- * interval the outer segments are recomputed once more to store exact inner
- * checkpoints. budget_bytes bounds the automatic choice (0: free device
- * memory). Defaults: 20, 0, 1, 0. */
-int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_interval, int bf16,
+ * it fits), stored in `format` (ELA_CK_*). bfloat16 keeps 8 significant bits
+ * of each value; the block-scaled int16 format keeps every value to 2^-16 of
+ * the largest one of its 64-value block. With outer_interval > interval the
+ * outer segments are recomputed once more to store exact inner checkpoints.
+ * budget_bytes bounds the automatic choice (0: free device memory).
+ * Defaults: 20, 0, ELA_CK_BF16, 0. No counterpart in the reference (which has
+ * no gradient). */
+int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_interval, int format,
                                  int64_t budget_bytes);
-/* Plan of the last gradient forward and the bytes its outer checkpoints hold. */
-int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* bf16,
+/* Plan of the last gradient forward (format = ELA_CK_*) and the bytes its outer checkpoints hold. */
+int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* format,
                            int64_t* checkpoint_bytes);
 
 /* Axes of one acquisition grid (its origin already shifted by half a cell on

@@ -25,27 +25,29 @@ def make_prop(precision):
 
 
 CASES = [
-    # precision, interval, outer, bf16, tolerance (relative L2 vs the FP64 oracle)
-    ("f64", 3, 3, False, 1e-11),      # one level, exact checkpoints
-    ("f64", 2, 6, False, 1e-11),      # two levels (outer segment recomputed)
-    ("f64", 20, 0, False, 1e-11),     # one segment longer than the run
-    ("f32", 4, 4, False, 2e-5),
-    ("f32", 2, 8, True, 5e-3),        # bfloat16 outer checkpoints
+    # precision, interval, outer, checkpoint format, tolerance (relative L2 vs the FP64 oracle)
+    ("f64", 3, 3, "native", 1e-11),   # one level, exact checkpoints
+    ("f64", 2, 6, "native", 1e-11),   # two levels (outer segment recomputed)
+    ("f64", 20, 0, "native", 1e-11),  # one segment longer than the run
+    ("f32", 4, 4, "native", 2e-5),
+    ("f32", 2, 8, "bf16", 5e-3),      # bfloat16 outer checkpoints
+    ("f32", 2, 8, "s16", 5e-5),       # block-scaled int16 outer checkpoints
+    ("f64", 3, 3, "s16", 5e-5),
 ]
 
 
-@pytest.mark.parametrize("precision,K,K1,bf16,tol", CASES)
-def test_gradient_matches_oracle(problem, precision, K, K1, bf16, tol):
+@pytest.mark.parametrize("precision,K,K1,ck,tol", CASES)
+def test_gradient_matches_oracle(problem, precision, K, K1, ck, tol):
     P, g8_ref, gsrc_ref = problem
     prop = make_prop(precision)
     prop.set_medium(P["med"])
-    prop.set_gradient_checkpoints(K, K1, bf16)
+    prop.set_gradient_checkpoints(K, K1, ck)
     rec = prop.gradient_forward(P["spos"], P["src"], P["rpos"])
     fwd = prop.nl_forward(P["spos"], P["src"], P["rpos"])
     assert all(np.array_equal(a, b) for a, b in zip(rec, fwd)), "gradient forward data differ from nl_forward"
     g8, gsrc = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
     e8, es = rel(g8, g8_ref), rel(gsrc, gsrc_ref)
-    print(f"gradient {precision} K={K} K1={K1} bf16={bf16}: g8 rel={e8:.3e} src rel={es:.3e} "
+    print(f"gradient {precision} K={K} K1={K1} {ck}: g8 rel={e8:.3e} src rel={es:.3e} "
           f"plan={prop.gradient_plan_used()}", flush=True)
     assert e8 < tol and es < tol
 
