@@ -8,7 +8,7 @@ from tests.problems import random_medium
 pytestmark = pytest.mark.gpu
 
 D = 10.0
-NTS, SUB, DTS = 6, 4, 0.004
+NTS, SUB, DTS = 6, 8, 0.004
 
 
 def _model():

@@ -15,7 +15,7 @@ def _problem(precision, bf16, interval=4, outer=0, checkpoint=None):
     from oracle import oracle as O
     from elasticisolib_3d_amd.acquisition import PaddedAxes, ShotGeometry
     from elasticisolib_3d_amd.fwi import FwiProblem, FwiShot
-    g = O.Grid(nz=30, nx=32, ny=28, nts=7, sub=4, dz=D, dx=D, dy=D, dts=0.004, minPad=5, alphaCos=0.95)
+    g = O.Grid(nz=30, nx=32, ny=28, nts=7, sub=8, dz=D, dx=D, dy=D, dts=0.004, minPad=5, alphaCos=0.95)
     pad = 5
     ax = PaddedAxes(nz=g.nz, nx=g.nx, ny=g.ny, oz=-(4 + pad) * D, ox=-(4 + pad) * D, oy=-(4 + pad) * D, dz=D, dx=D,
                     dy=D, fat=4, z_pad_minus=pad, z_pad_plus=pad, x_pad_minus=pad, x_pad_plus=pad, y_pad=pad)

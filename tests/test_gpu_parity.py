@@ -17,9 +17,9 @@ TOL = {"f64": 1e-11, "f32": 2e-5}
 
 GRIDS = {
     # odd sizes: partial z and x tiles, several y chunks
-    "odd": O.Grid(nz=29, nx=31, ny=45, nts=5, sub=4, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=5, alphaCos=0.95),
+    "odd": O.Grid(nz=29, nx=31, ny=45, nts=5, sub=8, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=5, alphaCos=0.95),
     # more than one 64-wide z tile
-    "wide": O.Grid(nz=81, nx=21, ny=19, nts=4, sub=4, dz=12.5, dx=10.0, dy=10.0, dts=0.004, minPad=4, alphaCos=0.9),
+    "wide": O.Grid(nz=81, nx=21, ny=19, nts=4, sub=8, dz=12.5, dx=10.0, dy=10.0, dts=0.004, minPad=4, alphaCos=0.9),
 }
 
 
@@ -111,7 +111,7 @@ def test_device_memory_path(problems):
     assert rel(out, ref) < 1e-6
 
 
-LONG = O.Grid(nz=29, nx=27, ny=25, nts=11, sub=4, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=5, alphaCos=0.95)
+LONG = O.Grid(nz=29, nx=27, ny=25, nts=11, sub=8, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=5, alphaCos=0.95)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])

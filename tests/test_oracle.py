@@ -26,7 +26,7 @@ import pytest
 
 from oracle import ela_numpy as N
 from oracle import oracle as O
-from tests.problems import interior_points, random_problem, rel
+from tests.problems import born_blob_problem, interior_points, random_problem, rel
 
 FAT = 4
 
@@ -195,8 +195,38 @@ def test_trilinear_weights():
     assert it2.weight.max() == pytest.approx(1.0) and np.sum(it2.weight > 0) == 1
 
 
+@pytest.mark.parametrize("courant,stable", [(0.43, True), (0.47, False)])
+def test_leapfrog_stability_limit(courant, stable):
+    """The stepper is a leapfrog over two levels, n = o + 2 dtw L c, so it is
+    stable while dtw max|omega| <= 1. The highest frequency of the 8th-order
+    staggered operator in 3-D is vp sqrt(3) 2 sum_k |c_k| / h, which puts the
+    limit at 2 vp dtw / h = 1 / (sqrt(3) sum_k |c_k|) = 0.4488 -- the quantity
+    and bound of the reference's Courant check, 2 vp dtw / min(h) <= 0.45
+    (fdParamElastic_3D.cpp:288-291). Just below it a broadband source stays
+    bounded; just above it the field grows exponentially."""
+    c = [1225 / 1024, 245 / 3072, 49 / 5120, 5 / 7168]
+    assert 1 / (math.sqrt(3) * sum(c)) == pytest.approx(0.4488, abs=1e-4)
+    h, vp, sub = 10.0, 3000.0, 8
+    dtw = courant * h / (2 * vp)
+    g = O.Grid(nz=24, nx=24, ny=24, nts=16, sub=sub, dz=h, dx=h, dy=h, dts=dtw * sub, minPad=0, alphaCos=1.0)
+    rho = np.full(g.shape, 2000.0)
+    mu = rho * (0.95 * vp / math.sqrt(3)) ** 2
+    lam = rho * vp ** 2 - 2 * mu
+    med = O.lib_scale_medium(rho, lam, mu, g.dtw)
+    rng = np.random.default_rng(4)
+    spos = [np.array([12 * 24 * 24 + 12 * 24 + 12], np.int64)] * 7
+    rpos = [np.array([10 * 24 * 24 + 14 * 24 + 11], np.int64)] * 7
+    src = [rng.standard_normal((1, g.nts)) for _ in range(9)]
+    d = np.stack(O.lib_nl_forward(g, med, src, spos, rpos))[:, 0, :]
+    early, late = np.abs(d[:, 2:6]).max(), np.abs(d[:, -4:]).max()
+    if stable:
+        assert late < 20 * early
+    else:
+        assert late > 1e6 * early
+
+
 def test_numpy_and_cpp_restatements_agree():
-    g = O.Grid(nz=22, nx=24, ny=20, nts=5, sub=4, dz=10., dx=10., dy=10., dts=0.004, minPad=4, alphaCos=0.9)
+    g = O.Grid(nz=22, nx=24, ny=20, nts=5, sub=8, dz=10., dx=10., dy=10., dts=0.004, minPad=4, alphaCos=0.9)
     P = random_problem(g, seed=7)
     a = O.lib_nl_forward(g, P["med"], P["src"], P["spos"], P["rpos"])
     b = N.nl_forward(g, P["med"], P["src"], P["spos"], P["rpos"])
@@ -209,7 +239,7 @@ class _Pos:
 
 
 def _dot_problem(seed, minPad=5, alpha=0.95):
-    g = O.Grid(nz=26, nx=25, ny=24, nts=7, sub=4, dz=10., dx=10., dy=10., dts=0.004, minPad=minPad, alphaCos=alpha)
+    g = O.Grid(nz=26, nx=25, ny=24, nts=7, sub=8, dz=10., dx=10., dy=10., dts=0.004, minPad=minPad, alphaCos=alpha)
     return g, random_problem(g, seed=seed)
 
 
@@ -268,40 +298,7 @@ def test_born_first_order_taylor(param):
     lambda perturbations away from the source (the reference's Born operator
     linearises the scheme up to the coarse-time derivative of the source
     wavefield, so agreement is to a few percent, not to rounding)."""
-    g = O.Grid(nz=40, nx=40, ny=40, nts=60, sub=8, dz=10., dx=10., dy=10., dts=0.004, minPad=6, alphaCos=0.99)
-    shape = g.shape
-    rho = np.full(shape, 2000.0)
-    vp, vs = 2500.0, 1440.0
-    mu = rho * vs ** 2
-    lam = rho * vp ** 2 - 2 * mu
-    # explosive source at the centre, receivers on a line
-    nz, nx = g.nz, g.nx
-    c = 20
-    flat = lambda iz, ix, iy: iy * nz * nx + ix * nz + iz  # noqa: E731
-    spos = [np.array([flat(c, c, c)])] + [np.zeros(0, np.int64)] * 6
-    rpos = [np.array([flat(12, x, 20) for x in range(10, 31, 2)])] * 7
-    w = np.zeros(g.nts)
-    t = np.arange(g.nts) * g.dts - 0.09
-    a2 = (math.pi * 12.0) ** 2
-    w = (1 - 2 * a2 * t * t) * np.exp(-a2 * t * t)
-    src = [np.zeros((len(spos[O.COMP_GRID[k]]), g.nts)) for k in range(9)]
-    # a smooth perturbation blob between source and receivers, zero at the source
-    iy, ix, iz = np.meshgrid(np.arange(g.ny), np.arange(g.nx), np.arange(g.nz), indexing="ij")
-    blob = np.exp(-((iz - 14) ** 2 + (ix - 20) ** 2 + (iy - 20) ** 2) / 8.0)
-    d3 = np.zeros((3,) + shape)
-    d3[param] = blob * (rho if param == 0 else lam) * 1.0
-
-    def fwd(scale):
-        r = rho + scale * d3[0]
-        lm = lam + scale * d3[1]
-        med = O.lib_scale_medium(r, lm, mu, g.dtw)
-        m8 = med.reshape(8, -1)
-        s = [np.zeros_like(x) for x in src]
-        pc = spos[0][0]
-        for k in (3, 4, 5):   # explosive: mxx = myy = mzz
-            s[k][0] = w * (m8[3][pc] + 2 * m8[4][pc]) / (g.dx * g.dy * g.dz)
-        return O.lib_nl_forward(g, med, s, spos, rpos), med, s
-
+    g, rho, lam, mu, d3, spos, rpos, fwd = born_blob_problem(param)
     eps = 1e-4
     dp, _, _ = fwd(eps)
     dn, _, _ = fwd(-eps)
@@ -312,6 +309,28 @@ def test_born_first_order_taylor(param):
     err = rel(born, fd)
     print(f"Born vs central difference, parameter {param}: relative L2 {err:.4f}")
     assert err < 0.1, err
+
+
+@pytest.mark.parametrize("param", [0, 1])
+def test_gradient_agrees_with_born_adjoint(param):
+    """The adjoint-state gradient against the reference's Born-adjoint
+    gradient: for J = <r, F(m)> with r the recorded data, the exact gradient
+    contracted with a smooth perturbation, <dJ/dM, dM>, equals <B^T r, dm>
+    up to the Born operator's coarse-time linearisation (a few percent).
+    dM, the change of the 8 scaled medium arrays, is the central difference of
+    the medium scaling; dm is the Born operator's staggered perturbation."""
+    g, rho, lam, mu, d3, spos, rpos, fwd = born_blob_problem(param)
+    r, med0, s0 = fwd(0.0)
+    g8, _ = O.lib_gradient(g, med0, s0, spos, r, rpos)
+    h = 1e-4
+    dM = (O.lib_scale_medium(rho + h * d3[0], lam + h * d3[1], mu, g.dtw)
+          - O.lib_scale_medium(rho - h * d3[0], lam - h * d3[1], mu, g.dtw)) / (2 * h)
+    lhs = float(np.sum(g8 * dM))
+    img = O.lib_born_adjoint(g, med0, s0, spos, r, rpos)
+    dm8 = O.born_stagger_perturbation(d3, med0, g.dtw, O.lib_stagger)
+    rhs = float(np.sum(np.asarray(img) * dm8))
+    print(f"gradient vs Born adjoint, parameter {param}: {lhs:.6e} {rhs:.6e} rel {abs(lhs - rhs) / abs(rhs):.4f}")
+    assert abs(lhs - rhs) <= 0.1 * abs(rhs)
 
 
 def _stokes_vz_on_axis(t, r, rho, alpha, beta, force):
@@ -366,7 +385,7 @@ def test_point_force_matches_stokes_solution():
 
 def _grad_problem(seed=21):
     from tests.problems import random_problem
-    g = O.Grid(nz=20, nx=22, ny=18, nts=5, sub=4, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=4, alphaCos=0.9)
+    g = O.Grid(nz=20, nx=22, ny=18, nts=5, sub=8, dz=10.0, dx=10.0, dy=10.0, dts=0.004, minPad=4, alphaCos=0.9)
     P = random_problem(g, seed=seed, nsrc=2, nrec=6)
     return g, P
 
