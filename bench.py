@@ -451,6 +451,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
+    ap.add_argument("--no-stream-ref", action="store_true",
+                    help="skip the streaming-kernel reference of the roofline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for counter calibration)")
     ap.add_argument("--workload", choices=("nl_forward", "born", "fp64_check", "fwi", "fwi_batch"),
@@ -528,6 +530,18 @@ def main():
     kv, ks = prop.time_step_kernels(args.kernel_reps)
     step_ms = kv + ks
     ach = BYTES_PER_CELL_F32 * w.interior_cells / (step_ms * 1e-3) / 1e9
+    # streaming reference: plain kernels with the same read / in-place write
+    # mixes over arrays of the padded grid's size (no neighbour accesses)
+    ceiling = None
+    if not args.no_stream_ref:
+        n_pad = w.axes.nz * w.axes.nx * w.axes.ny // 4 * 4
+        c_v = prop.stream_probe(12, 3, n_pad)
+        c_s = prop.stream_probe(14, 6, n_pad)
+        step_ref = BYTES_PER_CELL_F32 / (BYTES_VEL_F32 / c_v + BYTES_STRESS_F32 / c_s)
+        ceiling = {"streaming_GBs_12r_3w": round(c_v, 1), "streaming_GBs_14r_6w": round(c_s, 1),
+                   "streaming_GBs_step_mix": round(step_ref, 1), "frac_of_streaming": round(ach / step_ref, 4)}
+        log(f"[bench] streaming reference: 12r/3w {c_v:.0f} GB/s, 14r/6w {c_s:.0f} GB/s, step mix {step_ref:.0f} "
+            f"GB/s; stencil at {ach / step_ref:.1%} of it")
     finite = all(bool(torch.isfinite(r).all()) for r in rec)
     norm = float(sum(float((r.double() ** 2).sum()) for r in rec) ** 0.5)
 
@@ -570,6 +584,7 @@ def main():
             "k_stress_ms": round(ks, 4),
             "k_velocity_GBs": round(BYTES_VEL_F32 * w.interior_cells / (kv * 1e-3) / 1e9, 1),
             "k_stress_GBs": round(BYTES_STRESS_F32 * w.interior_cells / (ks * 1e-3) / 1e9, 1),
+            "streaming_reference": ceiling,
         },
     }
     if not args.no_cpu:

@@ -1646,6 +1646,38 @@ int ela_get_tiling(const ela_handle* h, int* tile_x, int* y_chunk) {
   return ELA_OK;
 }
 
+int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, double* gbs) {
+  if (!h || !gbs || reps < 1 || n < 4 || n % 4) return fail(ELA_ERR_ARG, "bad argument");
+  void (*kern)(float4* const*, int64_t) = nullptr;
+  if (reads == 1 && writes == 1) kern = ela::k_stream<1, 1>;
+  else if (reads == 12 && writes == 3) kern = ela::k_stream<12, 3>;
+  else if (reads == 14 && writes == 6) kern = ela::k_stream<14, 6>;
+  if (!kern) return fail(ELA_ERR_ARG, "stream probe mixes: (1, 1), (12, 3), (14, 6)");
+  DeviceGuard dg(h->cfg.device);
+  DevBuf buf, tab;
+  CHK(buf.alloc((size_t)reads * n * 4));
+  CHK(tab.alloc(sizeof(float4*) * 16));
+  float4* ptrs[16];
+  for (int k = 0; k < reads; ++k) ptrs[k] = reinterpret_cast<float4*>(static_cast<char*>(buf.p) + (size_t)k * n * 4);
+  HIPCHK(hipMemcpyAsync(tab.p, ptrs, sizeof(float4*) * reads, hipMemcpyHostToDevice, h->stream));
+  CHK(fill<float>(h, static_cast<float*>(buf.p), (int64_t)reads * n));
+  const int64_t n4 = n / 4;
+  // enough 256-thread blocks for about 4 vectors per thread
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n4 + 1023) / 1024, 1 << 20));
+  float4* const* t = static_cast<float4* const*>(tab.p);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  HIPCHK(hipGetLastError());
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *gbs = double(reads + writes) * 4.0 * double(n) * reps / (double(ms) * 1e-3) / 1e9;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return ELA_OK;
+}
+
 int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* ms_stress) {
   if (!h || reps < 1 || !ms_velocity || !ms_stress) return fail(ELA_ERR_ARG, "bad argument");
   if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");

@@ -956,6 +956,31 @@ __global__ void __launch_bounds__(256) k_ck_unpack(CkArgs<T> a, const void* in, 
   }
 }
 
+// Streaming reference for the roofline: R arrays of n4 16-byte vectors are
+// read and the first W of them rewritten in place (non-temporal stores), the
+// byte mix of a stencil kernel (reads of c, o and the medium, o rewritten)
+// without neighbour accesses. Grid-stride loop over vectors.
+template <int R, int W>
+__global__ void __launch_bounds__(256) k_stream(float4* const* arrays, int64_t n4) {
+  float4* a[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) a[k] = arrays[k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
+    float4 s = a[0][i];
+#pragma unroll
+    for (int k = 1; k < R; ++k) {
+      const float4 v = a[k][i];
+      s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      const f4 o = {s.x + (float)k, s.y, s.z, s.w};
+      __builtin_nontemporal_store(o, reinterpret_cast<f4*>(a[k] + i));
+    }
+  }
+}
+
 // 16-bit checkpoints with one scale per block of CK_BLOCK consecutive packed
 // values: q = rint(x / s) with s = max |x| / 32767 over the block (an
 // all-zero block has s = 0), restored as x = q s. The absolute error is at

@@ -326,6 +326,14 @@ class Propagator:
               "ela_time_step_kernels")
         return mv.value, ms.value
 
+    def stream_probe(self, reads: int, writes: int, n: int, reps: int = 20) -> float:
+        """GB/s of a plain streaming kernel with the given read / in-place
+        write mix over n FP32 values (ela_stream_probe)."""
+        g = ctypes.c_double()
+        check(self.lib.ela_stream_probe(self.h, int(reads), int(writes), int(n), int(reps), ctypes.byref(g)),
+              "ela_stream_probe")
+        return g.value
+
     def synchronize(self):
         check(self.lib.ela_synchronize(self.h), "ela_synchronize")
 

@@ -236,3 +236,20 @@ def test_graph_replay_matches_eager(problems, precision):
         assert all(np.array_equal(a, b) for a, b in zip(out_f, ref_f))
         assert all(np.array_equal(a, b) for a, b in zip(out_a, ref_a))
         assert all(np.array_equal(a, b) for a, b in zip(eager, ref_f))
+
+
+def test_stream_probe_reports_bandwidth():
+    """The streaming reference of the roofline runs its three mixes and
+    rejects others."""
+    from elasticisolib_3d_amd._lib import ElaError
+    from elasticisolib_3d_amd.propagator import Propagator
+    g = GRIDS["odd"]
+    prop = Propagator(g.nz, g.nx, g.ny, g.nts, g.sub, g.dz, g.dx, g.dy, g.dts, g.minPad, g.alphaCos, "f32")
+    for r, w in ((1, 1), (12, 3), (14, 6)):
+        gbs = prop.stream_probe(r, w, 1 << 20, reps=3)
+        print(f"stream probe {r}r/{w}w over 4 MiB arrays: {gbs:.0f} GB/s", flush=True)
+        assert gbs > 0
+    with pytest.raises(ElaError):
+        prop.stream_probe(2, 1, 1 << 20)
+    with pytest.raises(ElaError):
+        prop.stream_probe(12, 3, 1001)
