@@ -36,14 +36,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM-side bytes per fine step of the stencil pair from the committed
-    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by the
-    plain-copy calibration), or None if the summary is absent."""
+def measured_traffic(shape):
+    """HBM-side bytes per fine step of the stencil pair for launch shape
+    (tile_x, y_chunk, order, x_group) from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the plain-copy calibration), or
+    None if that shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "stencil_layered_400_pmc.md")
+    key = "traffic_bytes_per_step[{}x{}/{}/{}]:".format(*shape)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_step:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -529,6 +531,7 @@ def main():
     # per-kernel launch durations with HIP events on the propagator's stream
     kv, ks = prop.time_step_kernels(args.kernel_reps)
     step_ms = kv + ks
+    shape = (*prop.get_tiling(), *((tuned["order"], tuned["x_group"]) if tuned else (0, 1)))
     ach = BYTES_PER_CELL_F32 * w.interior_cells / (step_ms * 1e-3) / 1e9
     # streaming reference: plain kernels with the same read / in-place write
     # mixes over arrays of the padded grid's size (no neighbour accesses)
@@ -578,7 +581,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(),
+            "traffic": measured_traffic(shape),
             "bytes_per_cell": BYTES_PER_CELL_F32,
             "k_velocity_ms": round(kv, 4),
             "k_stress_ms": round(ks, 4),

@@ -108,3 +108,56 @@ def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = N
         tab[key] = best
         _store(tab)
     return dict(best, cached=False, table=[list(r) for r in table])
+
+
+def _parse_shapes(text: str):
+    """"16x8/0/1,4x8/3/4" -> [(16, 8, 0, 1), (4, 8, 3, 4)]"""
+    out = []
+    for item in text.split(","):
+        tile, order, group = item.strip().split("/")
+        tx, yc = tile.split("x")
+        out.append((int(tx), int(yc), int(order), int(group)))
+    return out
+
+
+def main(argv=None) -> int:
+    """Time launch shapes of the stencil on a synthetic medium of the given
+    padded grid: every candidate (as the autotuner does, without its cache),
+    or the listed shapes in order (--shapes), printing one line per shape."""
+    import argparse
+    import torch
+    from .propagator import Propagator
+    ap = argparse.ArgumentParser(prog="python -m elasticisolib_3d_amd.autotune")
+    ap.add_argument("--grid", default="328,472,468", help="padded nz,nx,ny")
+    ap.add_argument("--precision", default="f32", choices=("f32", "f64"))
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--shapes", default="", help="tile_x x y_chunk / order / x_group, comma separated")
+    a = ap.parse_args(argv)
+    nz, nx, ny = (int(v) for v in a.grid.split(","))
+    prop = Propagator(nz, nx, ny, 10, 8, 12.5, 12.5, 12.5, 0.004, 30, 0.99, a.precision)
+    dt = torch.float32 if a.precision == "f32" else torch.float64
+    med = torch.empty((8, ny, nx, nz), dtype=dt, device="cuda")
+    med[:3] = 1e-4
+    med[3] = 3e6
+    med[4:] = 1e6
+    prop.set_medium(med)
+    del med
+    cells = (nz - 8) * (nx - 8) * (ny - 8)
+    bpc = 140 if a.precision == "f32" else 280
+    shapes = _parse_shapes(a.shapes) if a.shapes else list(default_candidates(a.precision))
+    best = None
+    for tx, yc, order, xg in shapes:
+        prop.set_tiling(tx, yc)
+        prop.set_tile_order(order, xg)
+        kv, ks = prop.time_step_kernels(a.reps)
+        gbs = bpc * cells / ((kv + ks) * 1e-3) / 1e9
+        print(f"[autotune] shape {tx}x{yc}/{order}/{xg}: velocity {kv:.4f} ms stress {ks:.4f} ms "
+              f"step {kv + ks:.4f} ms ({gbs:.0f} GB/s algorithmic)", flush=True)
+        if best is None or kv + ks < best[0]:
+            best = (kv + ks, tx, yc, order, xg)
+    print(f"[autotune] fastest {best[1]}x{best[2]}/{best[3]}/{best[4]}: {best[0]:.4f} ms per step", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
