@@ -452,6 +452,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--launch", default="", help="fixed launch shape tile_x x y_chunk / order / x_group, "
+                                                 "e.g. 4x8/3/4 (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
     ap.add_argument("--no-stream-ref", action="store_true",
                     help="skip the streaming-kernel reference of the roofline")
@@ -503,7 +505,13 @@ def main():
     log(f"[bench] model {w.name} padded {w.axes.nz}x{w.axes.nx}x{w.axes.ny} built in {time.time() - t0:.1f}s")
     prop, spos, src, rpos, rec = gpu_setup(w)
     tuned = None
-    if not args.no_autotune:
+    if args.launch:
+        from elasticisolib_3d_amd.autotune import apply, _parse_shapes
+        tx, yc, order, xg = _parse_shapes(args.launch)[0]
+        apply(prop, {"tile_x": tx, "y_chunk": yc, "order": order, "x_group": xg})
+        tuned = {"tile_x": tx, "y_chunk": yc, "order": order, "x_group": xg, "given": True}
+        log(f"[bench] launch shape {tx}x{yc} order {order}/{xg} (given)")
+    elif not args.no_autotune:
         from elasticisolib_3d_amd.autotune import autotune
         t0 = time.time()
         tuned = autotune(prop, reps=10)
@@ -572,7 +580,7 @@ def main():
             "data_l2": norm,
             "launch": {"tile_x": prop.get_tiling()[0], "y_chunk": prop.get_tiling()[1],
                        "order": None if tuned is None else [tuned["order"], tuned["x_group"]],
-                       "autotuned": tuned is not None, "graph": not args.no_graph},
+                       "autotuned": tuned is not None and not tuned.get("given"), "graph": not args.no_graph},
         },
         "roofline": {
             "bound": "hbm",
