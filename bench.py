@@ -275,7 +275,8 @@ def bench_fwi(args):
     start = w.extra["start"]
     log(f"[bench] model {w.name} padded {ax.nz}x{ax.nx}x{ax.ny} built in {time.time() - t0:.1f}s")
     shots = _fwi_shots(w, ax)
-    prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, checkpoint=args.checkpoint)
+    prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, checkpoint=args.checkpoint,
+                      autotune=not args.no_autotune)
     t0 = time.perf_counter()
     prob.synthesize(w.rho, w.lame, w.mu)
     log(f"[bench] observed data of {len(shots)} shots synthesized in {time.perf_counter() - t0:.1f}s")
@@ -286,6 +287,7 @@ def bench_fwi(args):
         times.append(time.perf_counter() - t0)
         log(f"[bench] gradient {i + 1}: J = {J:.6e}, {times[-1]:.1f}s ({prob.last_timing})")
     plan = prob.prop.gradient_plan_used()
+    tuned = prob.tuned
     del prob
     torch.cuda.empty_cache()
     t_plain = min(times)
@@ -370,6 +372,7 @@ def bench_fwi(args):
             "gradient_s_per_call": [round(t, 2) for t in times],
             "checkpoint_plan": {"interval": plan[0], "outer_interval": plan[1], "format": plan[2],
                                 "checkpoint_GB_per_shot": round(plan[3] / 1e9, 2)},
+            "launch": None if tuned is None else {k: tuned[k] for k in ("tile_x", "y_chunk", "order", "x_group")},
             "interior_cells": w.interior_cells,
             "fine_steps_per_propagation": steps,
             "propagation_gcell_updates_per_s": round(3 * cells * steps * len(shots) / t_plain / 1e9, 2),
@@ -399,7 +402,7 @@ def bench_fwi_batch(args):
             sig[c][0] = cfg["wavelet"]
         shots.append(BatchShot(src, rec, Window(x0, y0, wx, wy), sig))
     shape = cfg["true"][0].shape
-    kw = dict(precision="f32", interval=20, checkpoint=args.checkpoint)
+    kw = dict(precision="f32", interval=20, checkpoint=args.checkpoint, autotune=not args.no_autotune)
     ref = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], shots, batch=1, **kw)
     t0 = time.perf_counter()
     ref.synthesize(cfg["true"])

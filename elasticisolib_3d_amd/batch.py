@@ -163,7 +163,7 @@ class BatchedFwi:
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
-                 bf16: bool = True, device: int = 0, tiling=None, checkpoint=None):
+                 bf16: bool = True, device: int = 0, tiling=None, checkpoint=None, autotune: bool = False):
         self.model_shape, self.d, self.pad = tuple(model_shape), float(d), int(pad)
         self.nts, self.dts, self.sub, self.alpha = int(nts), float(dts), int(sub), float(alpha_cos)
         self.dtw = self.dts / self.sub
@@ -172,6 +172,7 @@ class BatchedFwi:
         self.precision, self.interval, self.device = precision, int(interval), int(device)
         self.checkpoint = bool(bf16) if checkpoint is None else checkpoint   # "bf16", "s16", "native" or a bool
         self.tiling = tiling
+        self.autotune = bool(autotune)   # time the launch shapes of every stacked grid (cached per grid shape)
         self.timing = {}
 
     def _groups(self):
@@ -192,6 +193,9 @@ class BatchedFwi:
                 prop.set_tile_order(*self.tiling[2:])
         td = torch.float32 if self.precision == "f32" else torch.float64
         prop.set_medium(torch.tensor(med, dtype=td, device="cuda"))
+        if self.autotune and not self.tiling:
+            from .autotune import autotune
+            autotune(prop, reps=5)
         prop.set_gradient_checkpoints(self.interval, 0, self.checkpoint)
         geos = []
         for k, s in enumerate(group):

@@ -81,7 +81,7 @@ class FwiProblem:
 
     def __init__(self, axes: PaddedAxes, nts: int, dts: float, sub: int, alpha_cos: float, shots: Sequence[FwiShot],
                  precision: str = "f32", interval: int = 20, outer_interval: int = 0, bf16: bool = True,
-                 device: int = 0, checkpoint=None):
+                 device: int = 0, checkpoint=None, autotune: bool = False):
         self.axes = axes
         self.nts, self.dts, self.sub = int(nts), float(dts), int(sub)
         self.dtw = self.dts / self.sub
@@ -94,6 +94,8 @@ class FwiProblem:
         self.prop.set_gradient_checkpoints(interval, outer_interval, bf16 if checkpoint is None else checkpoint)
         self.dxyz = axes.dx * axes.dy * axes.dz
         self.last_timing = {}
+        # autotune: time the stencil launch shapes once, when the first medium is set
+        self._tune, self.tuned = bool(autotune), None
 
     # -- helpers -----------------------------------------------------------
     def _tdtype(self):
@@ -122,6 +124,9 @@ class FwiProblem:
                               self.dtw)
         torch = _torch()
         self.prop.set_medium(torch.tensor(med8, dtype=self._tdtype(), device=self.dev))
+        if self._tune and self.tuned is None:
+            from .autotune import autotune
+            self.tuned = autotune(self.prop, reps=5)
         return med8
 
     def _to_irregular(self, shot, reg):
