@@ -140,6 +140,11 @@ def bench_born(args):
     dm8_d = torch.tensor(dm8, dtype=torch.float32, device="cuda")
     del dm8
     prop.set_born_storage(args.born_segment)
+    tuned = None
+    if not args.no_autotune:
+        from elasticisolib_3d_amd.autotune import autotune
+        tuned = autotune(prop, reps=5)
+        log(f"[bench] autotune: tile {tuned['tile_x']}x{tuned['y_chunk']} order {tuned['order']}/{tuned['x_group']}")
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -195,6 +200,7 @@ def bench_born(args):
             "adjoint_gcell_updates_per_s": round(ga, 3),
             "adjoint_storage": "all coarse samples in HBM" if seg < 0 else f"exact checkpoints every {seg} coarse samples",
             "dot_product": {"lhs": lhs, "rhs": rhs, "rel_err": dot_rel, "reductions": "fp64"},
+            "launch": None if tuned is None else {k: tuned[k] for k in ("tile_x", "y_chunk", "order", "x_group")},
         },
     }
     print(json.dumps(out), flush=True)
