@@ -157,9 +157,12 @@ class BatchShot:
 
 class BatchedFwi:
     """FWI misfit and gradient over shots with shot-local windows, `batch`
-    shots per launch sequence on one GPU (FP32 stencil, bfloat16 or exact
-    checkpoints, FP64 gradient accumulation, as FwiProblem). `model_shape`
-    is the [ny, nx, nz] shape of the (unpadded) model arrays."""
+    shots per launch sequence on one GPU (FP32 stencil, bfloat16, int16
+    block-scaled or exact checkpoints, FP64 gradient accumulation, as
+    FwiProblem). `model_shape` is the [ny, nx, nz] shape of the (unpadded)
+    model arrays; `checkpoint` ("bf16", "s16", "native") overrides the older
+    `bf16` flag; `autotune` times the launch shapes of every stacked grid
+    unless `tiling` fixes them."""
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,

@@ -77,7 +77,10 @@ class FwiShot:
 
 
 class FwiProblem:
-    """Shots on one padded grid, propagated on one GPU."""
+    """Shots on one padded grid, propagated on one GPU. `checkpoint` selects
+    the storage of the gradient's outer checkpoints ("bf16", "s16" or
+    "native"; None: the `bf16` flag), `autotune` times the stencil launch
+    shapes once when the first medium is set."""
 
     def __init__(self, axes: PaddedAxes, nts: int, dts: float, sub: int, alpha_cos: float, shots: Sequence[FwiShot],
                  precision: str = "f32", interval: int = 20, outer_interval: int = 0, bf16: bool = True,

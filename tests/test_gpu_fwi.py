@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 D = 10.0
 
 
-def _problem(precision, bf16, interval=4, outer=0, checkpoint=None):
+def _problem(precision, bf16, interval=4, outer=0, checkpoint=None, autotune=False):
     from oracle import oracle as O
     from elasticisolib_3d_amd.acquisition import PaddedAxes, ShotGeometry
     from elasticisolib_3d_amd.fwi import FwiProblem, FwiShot
@@ -33,7 +33,7 @@ def _problem(precision, bf16, interval=4, outer=0, checkpoint=None):
         sig[2][0] = 0.3e6 * w
         shots.append(FwiShot(src, rec, sig))
     prob = FwiProblem(ax, g.nts, g.dts, g.sub, g.alphaCos, shots, precision=precision, interval=interval,
-                      outer_interval=outer, bf16=bf16, checkpoint=checkpoint)
+                      outer_interval=outer, bf16=bf16, checkpoint=checkpoint, autotune=autotune)
     true = random_medium(g, np.random.default_rng(8), contrast=0.1)
     prob.synthesize(*true)
     start = [m * (1 + 0.03 * np.random.default_rng(9 + k).standard_normal(m.shape)) for k, m in enumerate(true)]
@@ -75,7 +75,7 @@ def test_fp32_bf16_gradient_close_to_fp64():
     J32, g32 = p32.misfit_and_gradient(*m0)
     p32e, _ = _problem("f32", bf16=False, interval=4)
     J32e, g32e = p32e.misfit_and_gradient(*m0)
-    p32s, _ = _problem("f32", bf16=True, interval=4, outer=8, checkpoint="s16")
+    p32s, _ = _problem("f32", bf16=True, interval=4, outer=8, checkpoint="s16", autotune=True)
     J32s, g32s = p32s.misfit_and_gradient(*m0)
     e = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
     ee = float(np.linalg.norm(g32e - g64) / np.linalg.norm(g64))
@@ -88,6 +88,7 @@ def test_fp32_bf16_gradient_close_to_fp64():
     assert es < 1e-4 and es < e / 10
     assert ej < 1e-4
     assert p32s.prop.gradient_plan_used()[2] == "s16"
+    assert p32s.tuned is not None   # launch shape tuned on the first medium (results do not depend on it)
 
 
 def test_autograd_front_end():
