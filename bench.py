@@ -215,13 +215,21 @@ def bench_fp64_check(args):
 
     w = layered_400(nts=args.nts)
     res = {}
+    shapes = {}
     for prec in ("f64", "f32"):
         prop, spos, src, rpos, rec = gpu_setup(w, prec)
+        if not args.no_autotune:
+            from elasticisolib_3d_amd.autotune import autotune
+            t = autotune(prop, reps=5)
+            shapes[prec] = f"{t['tile_x']}x{t['y_chunk']}/{t['order']}/{t['x_group']}"
+        prop.set_graph(not args.no_graph)
+        prop.nl_forward(spos, src, rpos, rec_out=rec)   # warm-up (code objects, graph capture)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         prop.nl_forward(spos, src, rpos, rec_out=rec)
         torch.cuda.synchronize()
         res[prec] = ([r.double().cpu() for r in rec], time.perf_counter() - t0)
-        log(f"[bench] {prec} forward {res[prec][1]:.1f} s")
+        log(f"[bench] {prec} forward {res[prec][1]:.1f} s (launch shape {shapes.get(prec, 'default')})")
         del prop, spos, src, rpos, rec
         torch.cuda.empty_cache()
     num = sum(float(((a - b) ** 2).sum()) for a, b in zip(res["f32"][0], res["f64"][0]))
@@ -246,6 +254,8 @@ def bench_fp64_check(args):
             "per_component_rel_err": [round(e, 9) for e in per],
             "f64_gcell_updates_per_s": round(cells / res["f64"][1] / 1e9, 3),
             "f32_gcell_updates_per_s": round(cells / res["f32"][1] / 1e9, 3),
+            "f64_GBs_algorithmic": round(cells * 280 / res["f64"][1] / 1e9, 1),
+            "launch": shapes or None,
         },
     }
     print(json.dumps(out), flush=True)
