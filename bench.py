@@ -228,7 +228,8 @@ def bench_fp64_check(args):
         t0 = time.perf_counter()
         prop.nl_forward(spos, src, rpos, rec_out=rec)
         torch.cuda.synchronize()
-        res[prec] = ([r.double().cpu() for r in rec], time.perf_counter() - t0)
+        elapsed = time.perf_counter() - t0   # the traces stay on the GPU until here
+        res[prec] = ([r.double().cpu() for r in rec], elapsed)
         log(f"[bench] {prec} forward {res[prec][1]:.1f} s (launch shape {shapes.get(prec, 'default')})")
         del prop, spos, src, rpos, rec
         torch.cuda.empty_cache()
