@@ -132,18 +132,28 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", default="f32", choices=("f32", "f64"))
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--shapes", default="", help="tile_x x y_chunk / order / x_group, comma separated")
+    ap.add_argument("--medium", default="scaled", choices=("scaled", "elastic"),
+                    help="8 scaled medium arrays (ela_set_medium) or rho, lambda, mu (ela_set_elastic)")
     a = ap.parse_args(argv)
     nz, nx, ny = (int(v) for v in a.grid.split(","))
     prop = Propagator(nz, nx, ny, 10, 8, 12.5, 12.5, 12.5, 0.004, 30, 0.99, a.precision)
     dt = torch.float32 if a.precision == "f32" else torch.float64
-    med = torch.empty((8, ny, nx, nz), dtype=dt, device="cuda")
-    med[:3] = 1e-4
-    med[3] = 3e6
-    med[4:] = 1e6
-    prop.set_medium(med)
+    if a.medium == "scaled":
+        med = torch.empty((8, ny, nx, nz), dtype=dt, device="cuda")
+        med[:3] = 1e-4
+        med[3] = 3e6
+        med[4:] = 1e6
+        prop.set_medium(med)
+    else:
+        med = torch.empty((3, ny, nx, nz), dtype=dt, device="cuda")
+        med[0] = 2000.0
+        med[1] = 6e9
+        med[2] = 4e9
+        prop.set_elastic(med, 0.0005)
     del med
     cells = (nz - 8) * (nx - 8) * (ny - 8)
-    bpc = 140 if a.precision == "f32" else 280
+    # algorithmic bytes per cell-update: 8 medium reads, or 3 when the stepper forms the medium
+    bpc = (140 if a.medium == "scaled" else 120) * (1 if a.precision == "f32" else 2)
     shapes = _parse_shapes(a.shapes) if a.shapes else list(default_candidates(a.precision))
     best = None
     for tx, yc, order, xg in shapes:

@@ -120,6 +120,9 @@ struct ela_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf wave;              // 2 x 9 arrays (p0, p1)
   DevBuf med;               // 8 arrays
+  DevBuf elastic;           // rho, lambda, mu (ela_set_elastic)
+  bool have_elastic = false;  // the steppers form the medium from `elastic` (F_MED3)
+  double tdt = 0.0;         // 2 dtw of ela_set_elastic
   DevBuf taper;             // min_pad values
   DevBuf scratch_tm;        // time-major trace buffer
   DevBuf wave2;             // Born scattered / adjoint wavefield, 2 x 9 arrays
@@ -248,6 +251,10 @@ ela::StepArgs<T> base_args(ela_handle* h, T* o[9], T* c[9]) {
   std::memset(&a, 0, sizeof(a));
   for (int k = 0; k < 9; ++k) { a.o[k] = o[k]; a.c[k] = c[k]; }
   for (int k = 0; k < 8; ++k) a.m[k] = arr<T>(h->med, h->stride, k);
+  if (h->have_elastic) {
+    for (int k = 0; k < 3; ++k) a.ep[k] = arr<T>(h->elastic, h->stride, k);
+    a.tdt = (T)h->tdt;
+  }
   set_coefs<T>(h, a.cx, a.cy, a.cz);
   a.taper = reinterpret_cast<const T*>(h->taper.p);
   a.min_pad = (int)h->cfg.min_pad;
@@ -283,6 +290,17 @@ int launch_step(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
     case 8: return launch_step_bx<T, 8, MODE, FV, FS>(h, a, which);
     default: return launch_step_bx<T, 16, MODE, FV, FS>(h, a, which);
   }
+}
+
+// Forward and nonlinear-adjoint steps form the medium from (rho, lambda, mu)
+// when the handle holds them (ela_set_elastic): 3 medium reads per cell
+// instead of 8. The other modes read the 8 scaled arrays.
+template <typename T, int MODE, int FV, int FS>
+int launch_step_m(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
+  if constexpr (MODE == ela::MODE_FWD || MODE == ela::MODE_ADJ) {
+    if (h->have_elastic) return launch_step<T, MODE, FV | ela::F_MED3, FS | ela::F_MED3>(h, a, which);
+  }
+  return launch_step<T, MODE, FV, FS>(h, a, which);
 }
 
 template <typename T>
@@ -586,7 +604,7 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
     if (!adjoint) {
       for (int64_t it2 = 1; it2 <= sub; ++it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-        CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a)));
+        CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
         ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
         ela::PointArgs<T> R = make_points<T>(h, rp, w.p0, tmv, true, nrec_tot);
         S.its_dev = R.its_dev = its_dev;
@@ -597,7 +615,7 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
     } else {
       for (int64_t it2 = sub - 1; it2 >= 0; --it2) {
         ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-        CHK((launch_step<T, ela::MODE_ADJ, 0, 0>(h, a)));
+        CHK((launch_step_m<T, ela::MODE_ADJ, 0, 0>(h, a)));
         ela::PointArgs<T> R = make_points<T>(h, rp, w.p0, tmv, true, nrec_tot);
         ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
         S.its_dev = R.its_dev = its_dev;
@@ -695,7 +713,7 @@ int bg_coarse_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_
     ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
     for (int c = 0; c < 3; ++c) { a.accl[c] = VL + c * np; a.accr[c] = VR + c * np; }
     a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-    CHK((launch_step<T, ela::MODE_FWD, ela::F_ACC, 0>(h, a)));
+    CHK((launch_step_m<T, ela::MODE_FWD, ela::F_ACC, 0>(h, a)));
     ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
     for (int c = 0; c < 3; ++c) { S.accl[c] = a.accl[c]; S.accr[c] = a.accr[c]; }
     S.accfl = a.accfl; S.accfr = a.accfr;
@@ -825,7 +843,7 @@ int born_forward_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
       ela::StepArgs<T> a = base_args<T>(h, sc.p0, sc.p1);
       for (int k = 0; k < 9; ++k) { a.ssl[k] = ssL[k]; a.ssr[k] = ssR[k]; }
       a.ssfl = T(f[it2 - 1]); a.ssfr = T(f[hh + it2 - 1]);
-      CHK((launch_step<T, ela::MODE_FWD, ela::F_SS, ela::F_SS>(h, a)));
+      CHK((launch_step_m<T, ela::MODE_FWD, ela::F_SS, ela::F_SS>(h, a)));
       launch_record<T>(h, make_points<T>(h, rp, sc.p0, TmViews<T>(tm).v, true, nrec_tot), (int)its, f[it2],
                        f[hh + it2]);
       sc.swap();
@@ -1148,9 +1166,9 @@ int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_t
   ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
   if (STORE) {
     for (int c = 0; c < 9; ++c) a.dst[c] = dst + c * np;
-    CHK((launch_step<T, ela::MODE_FWD, ela::F_STORE, ela::F_STORE>(h, a)));
+    CHK((launch_step_m<T, ela::MODE_FWD, ela::F_STORE, ela::F_STORE>(h, a)));
   } else {
-    CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a)));
+    CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
   }
   launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), its, f[i2], f[hh + i2]);
   w.swap();
@@ -1195,7 +1213,7 @@ int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in
     }
     const int its = (int)(k / sub), i2 = (int)(k % sub);
     ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-    CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a)));
+    CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
     launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), its, f[i2], f[hh + i2]);
     launch_record<T>(h, make_points<T>(h, rp, w.p0, TmViews<T>(tm).v, true, nrec_tot), its, f[i2 + 1],
                      f[hh + i2 + 1]);
@@ -1358,14 +1376,14 @@ int time_kernels(ela_handle* h, int reps, double* ms_v, double* ms_s) {
   Wave<T> w = wave_ptrs<T>(h->wave, h->stride);
   ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
   float tv = 0.f, ts = 0.f;
-  CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a, 3)));  // warm caches and code
+  CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a, 3)));  // warm caches and code
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a, 1)));
+  for (int r = 0; r < reps; ++r) CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a, 1)));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   HIPCHK(hipEventSynchronize(h->ev1));
   HIPCHK(hipEventElapsedTime(&tv, h->ev0, h->ev1));
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  for (int r = 0; r < reps; ++r) CHK((launch_step<T, ela::MODE_FWD, 0, 0>(h, a, 2)));
+  for (int r = 0; r < reps; ++r) CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a, 2)));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   HIPCHK(hipEventSynchronize(h->ev1));
   HIPCHK(hipEventElapsedTime(&ts, h->ev0, h->ev1));
@@ -1388,6 +1406,28 @@ struct DeviceGuard {
 };
 
 }  // namespace
+
+template <typename T>
+int set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem) {
+  CHK(h->elastic.alloc((size_t)3 * h->stride * h->esize));
+  CHK(fill<T>(h, h->elastic.p, 3 * h->stride));
+  CHK(copy_in_3d(h, h->elastic.p, rlm3, 3, mem));
+  ela::MedArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  for (int k = 0; k < 3; ++k) a.ep[k] = arr<T>(h->elastic, h->stride, k);
+  for (int k = 0; k < 8; ++k) a.m[k] = arr<T>(h->med, h->stride, k);
+  a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
+  a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
+  a.ys = h->cfg.nx * h->nzi;
+  a.tdt = (T)(2.0 * dtw);
+  const int64_t n = h->cfg.nz * h->cfg.nx * h->cfg.ny;
+  hipLaunchKernelGGL((ela::k_scale_medium<T>), dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20)), dim3(256),
+                     0, h->stream, a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->tdt = 2.0 * dtw;
+  return ELA_OK;
+}
 
 extern "C" {
 
@@ -1492,6 +1532,18 @@ int ela_set_medium(ela_handle* h, const void* med8, int mem) {
   CHK(copy_in_3d(h, h->med.p, med8, 8, mem));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->have_medium = true;
+  h->have_elastic = false;   // the steppers read these 8 arrays
+  h->grad_ready = false;
+  return ELA_OK;
+}
+
+int ela_set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem) {
+  if (!h || !rlm3) return fail(ELA_ERR_ARG, "null argument");
+  if (!(dtw > 0)) return fail(ELA_ERR_ARG, "dtw must be positive");
+  DeviceGuard dg(h->cfg.device);
+  h->have_medium = h->have_elastic = false;
+  CHK(h->esize == 4 ? set_elastic<float>(h, rlm3, dtw, mem) : set_elastic<double>(h, rlm3, dtw, mem));
+  h->have_medium = h->have_elastic = true;
   h->grad_ready = false;
   return ELA_OK;
 }

@@ -48,7 +48,8 @@ enum { MODE_FWD = 0,       // n = w(w o + M D c)        stepFwdGpu_3D
 enum { F_SS = 1,     // add the secondary sources ssl/ssr (Born forward)
        F_ACC = 2,    // accumulate the new level into accl/accr (coarse-time interpolation)
        F_STORE = 4,  // write the step's derivative combinations to dst (gradient recomputation)
-       F_GRAD = 8    // gradient imaging: grad[m] += new level x stored derivatives drv
+       F_GRAD = 8,   // gradient imaging: grad[m] += new level x stored derivatives drv
+       F_MED3 = 16   // MODE_FWD / MODE_ADJ: form the 8 staggered, scaled medium values from (rho, lambda, mu)
 };
 
 template <typename T>
@@ -65,6 +66,8 @@ struct StepArgs {
   T* grad[8];           // F_GRAD: gradient accumulators, one per medium array
   const T* q[9];        // MODE_BORN_ADJ_PM: M c of the current level
   T* qn[9];             // MODE_BORN_ADJ_PM: M n of the new level (written)
+  const T* ep[3];       // F_MED3: rho, lambda, mu
+  T tdt;                // F_MED3: 2 dtw
   T ssfl, ssfr;         // filter weights for the secondary source
   T accfl, accfr;       // filter weights for the accumulation
   T cx[4], cy[4], cz[4];
@@ -340,12 +343,27 @@ struct VelStage {
   T hxx0, hxx1, hxy0, hxy1, hxz0, hxz1;  // x-halo rows (low / high)
   T hzz, hxz, hyz;              // z-halo column
   T nxy, nyy, nyz;              // new queue values (plane + 4, + 3, + 4)
+  T rho, rho_hx, rho_hz;        // F_MED3: density of the column, of row x0 - 1 and of z0 - 1
 };
+
+// F_MED3: the staggered, scaled medium formed in the stepper from (rho,
+// lambda, mu) with the interior rule of the host scaling (a cell's staggered
+// value averages it with its lower neighbour along the axis; the FD interior
+// never meets the edge rules): rhox = (rho + rho[x-1]) / 2, rhoxDtw = 2 dtw /
+// rhox (likewise y, z); lamb2MuDtw = 2 dtw (lambda + 2 mu), lambDtw = 2 dtw
+// lambda; muxzDtw = 2 dtw * Sz(Sx(mu)) with S the same half-cell average, muxy
+// = Sy(Sx(mu)), muyz = Sz(Sy(mu)) -- the operations of ela_scale_medium in the
+// same order, so an FP64 stepper reproduces the 8-array stepper exactly.
+template <typename T>
+__device__ __forceinline__ T stag_buoy(T tdt, T c, T m) { return tdt / (T(0.5) * (c + m)); }
 
 template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
-  constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0;
+  constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
+  static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
   __shared__ Plane<T, BX> l_xx[2], l_zz[2], l_xy[2], l_xz[2], l_yz[2];
+  // F_MED3: density planes with the low x row and low z column
+  __shared__ T l_rho[2][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
@@ -367,6 +385,12 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
     if (t.zh) {
       v.hzz = SZ.szz(pl + t.czh); v.hxz = SZ.sxz(pl + t.czh); v.hyz = SZ.syz(pl + t.czh);
     }
+    if (MED3) {
+      // the column's own cell and its x - 1 / z - 1 neighbours (inside the grid whenever the column is)
+      v.rho = t.okc ? *at(a.ep[0], pl + t.col) : T(0);
+      if (t.ty == 0) v.rho_hx = t.okc ? *at(a.ep[0], pl + t.col - (uint32_t)a.nzi) : T(0);
+      if (t.tz == 0) v.rho_hz = t.okc ? *at(a.ep[0], pl + t.col - 1u) : T(0);
+    }
   };
   const int r = t.ty + FAT, q = t.tz + FAT;
   auto stage = [&](int b, const VelStage<T>& v, T cxy, T cyz) {
@@ -377,7 +401,14 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
       l_xz[b][t.ty][q] = v.hxz0; l_xz[b][r + BX][q] = v.hxz1;
     }
     if (t.zh) { l_zz[b][r][t.colh] = v.hzz; l_xz[b][r][t.colh] = v.hxz; l_yz[b][r][t.colh] = v.hyz; }
+    if (MED3) {
+      l_rho[b][t.ty + 1][t.tz + 1] = v.rho;
+      if (t.ty == 0) l_rho[b][0][t.tz + 1] = v.rho_hx;
+      if (t.tz == 0) l_rho[b][t.ty + 1][0] = v.rho_hz;
+    }
   };
+  // F_MED3: density of the column on the previous plane (y - 1)
+  T rho_prev = (MED3 && t.okc) ? *at(a.ep[0], (uint32_t)(t.y0 - 1) * ys + t.col) : T(0);
 
   // queues: sxy (D+y, q[j] = f[iy-3+j]), syy (D-y, f[iy-4+j]), syz (D+y)
   T qxy[8], qyy[8], qyz[8];
@@ -406,7 +437,13 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
         o3[f] = ld_nt(a.o[f], i);
-        if (MODE != MODE_BORN_ADJ) m3[f] = ld_nt(a.m[f], i);
+        if (!MED3 && MODE != MODE_BORN_ADJ) m3[f] = ld_nt(a.m[f], i);
+      }
+      if (MED3) {
+        const T rc = l_rho[b][t.ty + 1][t.tz + 1];
+        m3[0] = stag_buoy(a.tdt, rc, l_rho[b][t.ty][t.tz + 1]);
+        m3[1] = stag_buoy(a.tdt, rc, rho_prev);
+        m3[2] = stag_buoy(a.tdt, rc, l_rho[b][t.ty + 1][t.tz]);
       }
       const T sx = dmx<T, BX>(l_xx[b], r, q, cx) + dq(qxy, cy) + dpz<T, BX>(l_xz[b], r, q, cz);
       const T sy = dpx<T, BX>(l_xy[b], r, q, cx) + dq(qyy, cy) + dpz<T, BX>(l_yz[b], r, q, cz);
@@ -434,6 +471,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
         if (F & F_GRAD) *at(a.grad[f], i) += u * ld_nt(a.drv[f], i);  // BX_/BY_/BZ_ == VX/VY/VZ
       }
     }
+    if (MED3) rho_prev = l_rho[b][t.ty + 1][t.tz + 1];
 #pragma unroll
     for (int j = 0; j < 7; ++j) { qxy[j] = qxy[j + 1]; qyy[j] = qyy[j + 1]; qyz[j] = qyz[j + 1]; }
     if (more) {
@@ -452,12 +490,16 @@ template <typename T>
 struct StressStage {
   T hlo[3], hhi[3], hz[3];   // x-halo rows and z-halo column of vx, vy, vz
   T nx, ny, nz;              // new queue values (vx: plane+3, vy: +4, vz: +3)
+  T mu, mu_hx, mu_hz, mu_hc; // F_MED3: shear modulus of the column, row x0 - 1, z0 - 1 and both
 };
 
 template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
-  constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0;
+  constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
+  static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
   __shared__ Plane<T, BX> l_vx[2], l_vy[2], l_vz[2];
+  // F_MED3: shear-modulus planes y - 1, y, y + 1 (rotating) with the low x row and low z column
+  __shared__ T l_mu[MED3 ? 3 : 1][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
@@ -476,6 +518,20 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
 #pragma unroll
       for (int f = 0; f < 3; ++f) v.hz[f] = Vzh.v(f, pl + t.czh);
     }
+  };
+  // F_MED3: shear modulus of plane p around the column (inside the grid whenever the column is)
+  auto fetch_mu = [&](StressStage<T>& v, int p) {
+    const uint32_t pl = (uint32_t)p * ys, nzi = (uint32_t)a.nzi;
+    v.mu = t.okc ? *at(a.ep[2], pl + t.col) : T(0);
+    if (t.ty == 0) v.mu_hx = t.okc ? *at(a.ep[2], pl + t.col - nzi) : T(0);
+    if (t.tz == 0) v.mu_hz = t.okc ? *at(a.ep[2], pl + t.col - 1u) : T(0);
+    if (t.ty == 0 && t.tz == 0) v.mu_hc = t.okc ? *at(a.ep[2], pl + t.col - nzi - 1u) : T(0);
+  };
+  auto stage_mu = [&](int m, const StressStage<T>& v) {
+    l_mu[m][t.ty + 1][t.tz + 1] = v.mu;
+    if (t.ty == 0) l_mu[m][0][t.tz + 1] = v.mu_hx;
+    if (t.tz == 0) l_mu[m][t.ty + 1][0] = v.mu_hz;
+    if (t.ty == 0 && t.tz == 0) l_mu[m][0][0] = v.mu_hc;
   };
   const int r = t.ty + FAT, q = t.tz + FAT;
   auto stage = [&](int b, const StressStage<T>& v, T cvx, T cvy, T cvz) {
@@ -496,18 +552,30 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
     qy[j] = V.v(VY, (uint32_t)(t.y0 - 3 + j) * ys + t.col);
     qz[j] = V.v(VZ, (uint32_t)(t.y0 - 4 + j) * ys + t.col);
   }
+  // F_MED3: mu planes rotate through l_mu[0..2]; mc holds plane iy, (mc + 2) % 3 plane iy - 1
+  int mc = 1;
   {
     StressStage<T> v0;
     fetch(v0, t.y0);
     qx[7] = v0.nx; qy[7] = v0.ny; qz[7] = v0.nz;
     stage(0, v0, qx[4], qy[3], qz[4]);
+    if (MED3) {
+      StressStage<T> vm;
+      fetch_mu(vm, t.y0 - 1);
+      stage_mu(0, vm);
+      fetch_mu(v0, t.y0);
+      stage_mu(1, v0);
+    }
   }
   __syncthreads();
   int b = 0;
   for (int iy = t.y0; iy < t.y1; ++iy) {
     const bool more = iy + 1 < t.y1;
     StressStage<T> nx_;
-    if (more) fetch(nx_, iy + 1);
+    if (more) {
+      fetch(nx_, iy + 1);
+      if (MED3) fetch_mu(nx_, iy + 1);
+    }
     if (t.active) {
       const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
@@ -515,7 +583,20 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) o6[k] = ld_nt(a.o[SXX + k], i);
       T l2m = 0, lam = 0, mxz = 0, mxy = 0, myz = 0;
-      if (MODE != MODE_BORN_ADJ) {
+      if (MED3) {
+        const int rr = t.ty + 1, qq = t.tz + 1, mp = (mc + 2) % 3;
+        const T mu_c = l_mu[mc][rr][qq], mu_xm = l_mu[mc][rr - 1][qq], mu_zm = l_mu[mc][rr][qq - 1];
+        const T mu_xzm = l_mu[mc][rr - 1][qq - 1];
+        const T mu_yc = l_mu[mp][rr][qq], mu_yxm = l_mu[mp][rr - 1][qq], mu_yzm = l_mu[mp][rr][qq - 1];
+        const T lr = ld_nt(a.ep[1], i);
+        l2m = a.tdt * (lr + T(2) * mu_c);
+        lam = a.tdt * lr;
+        const T sx_c = T(0.5) * (mu_c + mu_xm), sx_zm = T(0.5) * (mu_zm + mu_xzm), sx_ym = T(0.5) * (mu_yc + mu_yxm);
+        const T sy_c = T(0.5) * (mu_c + mu_yc), sy_zm = T(0.5) * (mu_zm + mu_yzm);
+        mxz = (T(0.5) * (sx_c + sx_zm)) * a.tdt;
+        mxy = (T(0.5) * (sx_c + sx_ym)) * a.tdt;
+        myz = (T(0.5) * (sy_c + sy_zm)) * a.tdt;
+      } else if (MODE != MODE_BORN_ADJ) {
         l2m = ld_nt(a.m[L2M], i); lam = ld_nt(a.m[LAM], i);
         mxz = ld_nt(a.m[MXZ], i); mxy = ld_nt(a.m[MXY], i); myz = ld_nt(a.m[MYZ], i);
       }
@@ -585,9 +666,12 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
     if (more) {
       qx[7] = nx_.nx; qy[7] = nx_.ny; qz[7] = nx_.nz;
       stage(b ^ 1, nx_, qx[4], qy[3], qz[4]);
+      // plane iy + 1 replaces plane iy - 2, whose last reader was the previous iteration
+      if (MED3) stage_mu((mc + 1) % 3, nx_);
     }
     __syncthreads();
     b ^= 1;
+    mc = (mc + 1) % 3;
   }
 }
 
@@ -953,6 +1037,56 @@ __global__ void __launch_bounds__(256) k_ck_unpack(CkArgs<T> a, const void* in, 
     if (BF16) x = (T)(bf16_to_f32(static_cast<const uint16_t*>(in)[f * a.nI + q]) * s);
     else x = static_cast<const T*>(in)[f * a.nI + q];
     a.f[f][ck_offset(a, q)] = x;
+  }
+}
+
+// The 8 scaled medium arrays from (rho, lambda, mu) on every cell of the
+// padded grid with the host scaling's edge rules (the half-cell average
+// halves the first sample and adds half the neighbour to the last one): the
+// device twin of ela_scale_medium, same operations in the same order.
+template <typename T>
+struct MedArgs {
+  const T* ep[3];        // rho, lambda, mu (internal layout)
+  T* m[8];               // the 8 scaled arrays (internal layout)
+  int nz, nx, ny, nzi, zoff;
+  int64_t ys;
+  T tdt;
+};
+
+template <typename T>
+__device__ __forceinline__ T half_cell(const T* f, int64_t i, int64_t s, int ia, int n) {
+  return ia == 0 ? T(0.5) * f[i] : (ia < n - 1 ? T(0.5) * (f[i] + f[i - s]) : f[i] + T(0.5) * f[i - s]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scale_medium(MedArgs<T> a) {
+  const int64_t n = (int64_t)a.nz * a.nx * a.ny;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(c % a.nz);
+    const int64_t rest = c / a.nz;
+    const int ix = (int)(rest % a.nx), iy = (int)(rest / a.nx);
+    const int64_t i = iy * a.ys + (int64_t)ix * a.nzi + a.zoff + iz;
+    const int64_t sx = a.nzi, sy = a.ys;
+    const T* rho = a.ep[0];
+    const T* lam = a.ep[1];
+    const T* mu = a.ep[2];
+    a.m[BX_][i] = a.tdt / half_cell(rho, i, sx, ix, a.nx);
+    a.m[BY_][i] = a.tdt / half_cell(rho, i, sy, iy, a.ny);
+    a.m[BZ_][i] = a.tdt / half_cell(rho, i, (int64_t)1, iz, a.nz);
+    a.m[L2M][i] = a.tdt * (lam[i] + T(2) * mu[i]);
+    a.m[LAM][i] = a.tdt * lam[i];
+    // mu staggered along x, then along z (muxz) or y (muxy); along y, then z (muyz)
+    const T tx = half_cell(mu, i, sx, ix, a.nx);
+    const T tx_z = iz > 0 ? half_cell(mu, i - 1, sx, ix, a.nx) : T(0);
+    const T tx_y = iy > 0 ? half_cell(mu, i - sy, sx, ix, a.nx) : T(0);
+    const T ty = half_cell(mu, i, sy, iy, a.ny);
+    const T ty_z = iz > 0 ? half_cell(mu, i - 1, sy, iy, a.ny) : T(0);
+    auto outer = [](T v, T vm, int ia, int na) {
+      return ia == 0 ? T(0.5) * v : (ia < na - 1 ? T(0.5) * (v + vm) : v + T(0.5) * vm);
+    };
+    a.m[MXZ][i] = outer(tx, tx_z, iz, a.nz) * a.tdt;
+    a.m[MXY][i] = outer(tx, tx_y, iy, a.ny) * a.tdt;
+    a.m[MYZ][i] = outer(ty, ty_z, iz, a.nz) * a.tdt;
   }
 }
 

@@ -159,6 +159,18 @@ class Propagator:
             med8 = np.ascontiguousarray(med8, dtype=self.dtype)
         check(self.lib.ela_set_medium(self.h, self._ptr(med8, self.dtype), mem), "ela_set_medium")
 
+    def set_elastic(self, rlm3, dtw: float) -> None:
+        """Elastic parameters (rho, lambda, mu) [3, ny, nx, nz] instead of the
+        8 scaled arrays (ela_set_elastic): the 8 arrays are derived on the GPU
+        with the staggering and scaling of scale_medium(rho, lambda, mu, dtw),
+        and the forward / nonlinear-adjoint steps form them on the fly."""
+        if tuple(rlm3.shape) != (3,) + self.shape:
+            raise ValueError(f"elastic parameters must have shape {(3,) + self.shape}")
+        mem = self._mem_of(rlm3)
+        if mem == _lib.ELA_MEM_HOST:
+            rlm3 = np.ascontiguousarray(rlm3, dtype=self.dtype)
+        check(self.lib.ela_set_elastic(self.h, self._ptr(rlm3, self.dtype), float(dtw), mem), "ela_set_elastic")
+
     def nl_forward(self, src_pos, src_traces, rec_pos, rec_out=None):
         mem = self._mem_of(*src_traces, *(rec_out or []))
         sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)

@@ -120,6 +120,19 @@ int ela_synchronize(ela_handle* h);
  * the handle precision). */
 int ela_set_medium(ela_handle* h, const void* med8, int mem);
 
+/* Upload the elastic parameters (rho, lambda, mu) [3][ny][nx][nz] (element
+ * type of the handle precision) instead of the 8 scaled arrays: the library
+ * derives the 8 arrays on the device with the staggering and scaling of
+ * ela_scale_medium (same operations and order; dtw = dts / sub as given there),
+ * and its forward and nonlinear-adjoint steps form the staggered, scaled
+ * values from the three parameters as they go, reading 3 medium arrays per
+ * cell instead of 8 (120 instead of 140 bytes per FP32 cell-update). The other
+ * steps read the derived 8 arrays. Replaces the medium of ela_set_medium (and
+ * is replaced by a later ela_set_medium). No counterpart in the reference,
+ * whose allocate functions take the 8 scaled arrays
+ * (nonlinearPropElasticGpuFunctions_3D.h:10). */
+int ela_set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem);
+
 /* Host-side staggering and scaling of (rho, lambda, mu) into the 8 arrays of
  * ela_set_medium (fdParamElastic_3D.cpp:130-183). Double precision in and out;
  * dtw = dts / sub. */

@@ -253,3 +253,36 @@ def test_stream_probe_reports_bandwidth():
         prop.stream_probe(2, 1, 1 << 20)
     with pytest.raises(ElaError):
         prop.stream_probe(12, 3, 1001)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("name", ["odd", "wide"])
+def test_elastic_parameters_match_scaled_medium(problems, precision, name):
+    """ela_set_elastic (rho, lambda, mu; medium formed in the stepper and the
+    8 arrays derived on the GPU) against ela_set_medium with the host-scaled
+    8 arrays: bit-identical in FP64 (same operations in the same order), FP32
+    to rounding, for the nonlinear forward and adjoint and the Born forward
+    and adjoint."""
+    from elasticisolib_3d_amd.propagator import Propagator, scale_medium
+    g, P = GRIDS[name], problems[name]
+    rlm = np.stack([P["rho"], P["lam"], P["mu"]])
+    med8 = scale_medium(P["rho"], P["lam"], P["mu"], g.dtw)   # the library's host scaling
+    out = {}
+    for kind in ("med8", "elastic"):
+        prop = Propagator(g.nz, g.nx, g.ny, g.nts, g.sub, g.dz, g.dx, g.dy, g.dts, g.minPad, g.alphaCos, precision)
+        if kind == "med8":
+            prop.set_medium(med8)
+        else:
+            prop.set_elastic(rlm, g.dtw)
+        out[kind] = (prop.nl_forward(P["spos"], P["src"], P["rpos"]),
+                     prop.nl_adjoint(P["spos"], P["rec"], P["rpos"]),
+                     prop.born_forward(P["spos"], P["src"], P["dm"], P["rpos"]),
+                     [prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])])
+    for k, what in enumerate(("nl_forward", "nl_adjoint", "born_forward", "born_adjoint")):
+        a, b = out["elastic"][k], out["med8"][k]
+        err = rel(a, b)
+        print(f"elastic vs med8 {precision} {name} {what}: {err:.2e}", flush=True)
+        if precision == "f64":
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), what
+        else:
+            assert err < 2e-6, (what, err)
