@@ -26,9 +26,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_F32 = 140       # 9 c + 9 o reads, 9 n writes, 8 medium reads (x4 B)
-BYTES_VEL_F32 = 60             # velocity kernel: 6 c + 3 o + 3 medium reads, 3 writes
-BYTES_STRESS_F32 = 80          # stress kernel: 3 c + 6 o + 5 medium reads, 6 writes
+# algorithmic bytes per FP32 cell-update of the two stencil kernels:
+# reads of the other level (c), of the own level (o) and of the medium, own level written
+BYTES_F32 = {
+    # 8 scaled medium arrays: velocity 6 c + 3 o + 3 medium reads, 3 writes; stress 3 c + 6 o + 5 medium, 6
+    "scaled": {"vel": 60, "stress": 80, "mix_vel": (12, 3), "mix_stress": (14, 6)},
+    # medium formed in the steppers from rho (velocity) and lambda, mu (stress)
+    "elastic": {"vel": 52, "stress": 68, "mix_vel": (10, 3), "mix_stress": (11, 6)},
+}
 METRIC = "Gcell-updates/s & %HBM peak; FWI-gradient shots/s batched vs plain, 1 GPU"
 
 
@@ -36,13 +41,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(shape):
+def measured_traffic(shape, medium="scaled"):
     """HBM-side bytes per fine step of the stencil pair for launch shape
-    (tile_x, y_chunk, order, x_group) from the committed rocprofv3 PMC summary
+    (tile_x, y_chunk, order, x_group) and medium kind from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE runs scaled by the plain-copy calibration), or
     None if that shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "stencil_layered_400_pmc.md")
-    key = "traffic_bytes_per_step[{}x{}/{}/{}]:".format(*shape)
+    key = "traffic_bytes_per_step[{}{}x{}/{}/{}]:".format("" if medium == "scaled" else medium + " ", *shape)
     try:
         for line in open(path):
             if line.startswith(key):
@@ -52,7 +57,7 @@ def measured_traffic(shape):
     return None
 
 
-def gpu_setup(workload, precision="f32"):
+def gpu_setup(workload, precision="f32", medium="scaled"):
     import torch
     from elasticisolib_3d_amd import propagator as P
     from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
@@ -69,7 +74,10 @@ def gpu_setup(workload, precision="f32"):
     prop = P.Propagator(a.nz, a.nx, a.ny, w.nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos, precision)
     dev = "cuda"
     dt = torch.float32 if precision == "f32" else torch.float64
-    prop.set_medium(torch.tensor(med, dtype=dt, device=dev))
+    if medium == "elastic":   # (rho, lambda, mu): the steppers form the 8 scaled values
+        prop.set_elastic(torch.tensor(np.stack([w.rho, w.lame, w.mu]), dtype=dt, device=dev), w.dtw)
+    else:
+        prop.set_medium(torch.tensor(med, dtype=dt, device=dev))
     del med
     spos = [torch.tensor(p, device=dev) for p in src_geom.positions()]
     rpos = [torch.tensor(p, device=dev) for p in rec_geom.positions()]
@@ -475,6 +483,9 @@ def main():
     ap.add_argument("--launch", default="", help="fixed launch shape tile_x x y_chunk / order / x_group, "
                                                  "e.g. 4x8/3/4 (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
+    ap.add_argument("--medium", choices=("elastic", "scaled"), default="elastic",
+                    help="nl_forward: rho, lambda, mu with the medium formed in the steppers (120 B per cell-update), "
+                         "or the 8 scaled arrays of the reference's interface (140 B)")
     ap.add_argument("--no-stream-ref", action="store_true",
                     help="skip the streaming-kernel reference of the roofline")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -523,7 +534,9 @@ def main():
     t0 = time.time()
     w = layered_400(nts=args.nts)
     log(f"[bench] model {w.name} padded {w.axes.nz}x{w.axes.nx}x{w.axes.ny} built in {time.time() - t0:.1f}s")
-    prop, spos, src, rpos, rec = gpu_setup(w)
+    prop, spos, src, rpos, rec = gpu_setup(w, "f32", args.medium)
+    by = BYTES_F32[args.medium]
+    bpc = by["vel"] + by["stress"]
     tuned = None
     if args.launch:
         from elasticisolib_3d_amd.autotune import apply, _parse_shapes
@@ -560,19 +573,20 @@ def main():
     kv, ks = prop.time_step_kernels(args.kernel_reps)
     step_ms = kv + ks
     shape = (*prop.get_tiling(), *((tuned["order"], tuned["x_group"]) if tuned else (0, 1)))
-    ach = BYTES_PER_CELL_F32 * w.interior_cells / (step_ms * 1e-3) / 1e9
+    ach = bpc * w.interior_cells / (step_ms * 1e-3) / 1e9
     # streaming reference: plain kernels with the same read / in-place write
     # mixes over arrays of the padded grid's size (no neighbour accesses)
     ceiling = None
     if not args.no_stream_ref:
         n_pad = w.axes.nz * w.axes.nx * w.axes.ny // 4 * 4
-        c_v = prop.stream_probe(12, 3, n_pad)
-        c_s = prop.stream_probe(14, 6, n_pad)
-        step_ref = BYTES_PER_CELL_F32 / (BYTES_VEL_F32 / c_v + BYTES_STRESS_F32 / c_s)
-        ceiling = {"streaming_GBs_12r_3w": round(c_v, 1), "streaming_GBs_14r_6w": round(c_s, 1),
+        (rv, wv), (rs, wsn) = by["mix_vel"], by["mix_stress"]
+        c_v = prop.stream_probe(rv, wv, n_pad)
+        c_s = prop.stream_probe(rs, wsn, n_pad)
+        step_ref = bpc / (by["vel"] / c_v + by["stress"] / c_s)
+        ceiling = {f"streaming_GBs_{rv}r_{wv}w": round(c_v, 1), f"streaming_GBs_{rs}r_{wsn}w": round(c_s, 1),
                    "streaming_GBs_step_mix": round(step_ref, 1), "frac_of_streaming": round(ach / step_ref, 4)}
-        log(f"[bench] streaming reference: 12r/3w {c_v:.0f} GB/s, 14r/6w {c_s:.0f} GB/s, step mix {step_ref:.0f} "
-            f"GB/s; stencil at {ach / step_ref:.1%} of it")
+        log(f"[bench] streaming reference: {rv}r/{wv}w {c_v:.0f} GB/s, {rs}r/{wsn}w {c_s:.0f} GB/s, step mix "
+            f"{step_ref:.0f} GB/s; stencil at {ach / step_ref:.1%} of it")
     finite = all(bool(torch.isfinite(r).all()) for r in rec)
     norm = float(sum(float((r.double() ** 2).sum()) for r in rec) ** 0.5)
 
@@ -594,6 +608,8 @@ def main():
                         "cells (padded 328x472x468, 30-cell sponge), d=12.5 m, explosive 10 Hz Ricker, "
                         f"{w.nts} coarse samples x {w.sub} fine steps, 200x200 surface receivers, 9 components",
             "interior_cells": w.interior_cells,
+            "medium": ("rho, lambda, mu; the 8 staggered, scaled values formed in the steppers" if args.medium == "elastic"
+                       else "8 scaled medium arrays"),
             "fine_steps": w.fine_steps,
             "event_loop_ms_per_step": round(ev_ms / args.steps, 2),
             "data_finite": finite,
@@ -609,12 +625,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(shape),
-            "bytes_per_cell": BYTES_PER_CELL_F32,
+            "traffic": measured_traffic(shape, args.medium),
+            "bytes_per_cell": bpc,
+            "medium": args.medium,
             "k_velocity_ms": round(kv, 4),
             "k_stress_ms": round(ks, 4),
-            "k_velocity_GBs": round(BYTES_VEL_F32 * w.interior_cells / (kv * 1e-3) / 1e9, 1),
-            "k_stress_GBs": round(BYTES_STRESS_F32 * w.interior_cells / (ks * 1e-3) / 1e9, 1),
+            "k_velocity_GBs": round(by["vel"] * w.interior_cells / (kv * 1e-3) / 1e9, 1),
+            "k_stress_GBs": round(by["stress"] * w.interior_cells / (ks * 1e-3) / 1e9, 1),
             "streaming_reference": ceiling,
         },
     }

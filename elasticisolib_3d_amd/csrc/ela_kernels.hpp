@@ -355,7 +355,14 @@ struct VelStage {
 // = Sy(Sx(mu)), muyz = Sz(Sy(mu)) -- the operations of ela_scale_medium in the
 // same order, so an FP64 stepper reproduces the 8-array stepper exactly.
 template <typename T>
-__device__ __forceinline__ T stag_buoy(T tdt, T c, T m) { return tdt / (T(0.5) * (c + m)); }
+__device__ __forceinline__ T stag_buoy(T tdt, T c, T m) {
+  if constexpr (sizeof(T) == 4) {
+    // FP32: one hardware reciprocal (1 ulp) instead of the correctly rounded division
+    return tdt * __builtin_amdgcn_rcpf(T(0.5) * (c + m));
+  } else {
+    return tdt / (T(0.5) * (c + m));
+  }
+}
 
 template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
@@ -498,8 +505,8 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
   static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
   __shared__ Plane<T, BX> l_vx[2], l_vy[2], l_vz[2];
-  // F_MED3: shear-modulus planes y - 1, y, y + 1 (rotating) with the low x row and low z column
-  __shared__ T l_mu[MED3 ? 3 : 1][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
+  // F_MED3: shear-modulus planes (double-buffered like the velocity planes) with the low x row and low z column
+  __shared__ T l_mu[2][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
@@ -552,19 +559,24 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
     qy[j] = V.v(VY, (uint32_t)(t.y0 - 3 + j) * ys + t.col);
     qz[j] = V.v(VZ, (uint32_t)(t.y0 - 4 + j) * ys + t.col);
   }
-  // F_MED3: mu planes rotate through l_mu[0..2]; mc holds plane iy, (mc + 2) % 3 plane iy - 1
-  int mc = 1;
+  // F_MED3: the previous plane's terms carried in registers -- Sx(mu) of the
+  // column, mu of the column and of its z - 1 neighbour -- which are the
+  // current plane's values one iteration later (same operands, same order)
+  T sx_prev = T(0), mu_prev = T(0), muz_prev = T(0);
+  if (MED3 && t.okc) {
+    const uint32_t pl = (uint32_t)(t.y0 - 1) * ys;
+    mu_prev = *at(a.ep[2], pl + t.col);
+    muz_prev = *at(a.ep[2], pl + t.col - 1u);
+    sx_prev = T(0.5) * (mu_prev + *at(a.ep[2], pl + t.col - (uint32_t)a.nzi));
+  }
   {
     StressStage<T> v0;
     fetch(v0, t.y0);
     qx[7] = v0.nx; qy[7] = v0.ny; qz[7] = v0.nz;
     stage(0, v0, qx[4], qy[3], qz[4]);
     if (MED3) {
-      StressStage<T> vm;
-      fetch_mu(vm, t.y0 - 1);
-      stage_mu(0, vm);
       fetch_mu(v0, t.y0);
-      stage_mu(1, v0);
+      stage_mu(0, v0);
     }
   }
   __syncthreads();
@@ -584,17 +596,16 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       for (int k = 0; k < 6; ++k) o6[k] = ld_nt(a.o[SXX + k], i);
       T l2m = 0, lam = 0, mxz = 0, mxy = 0, myz = 0;
       if (MED3) {
-        const int rr = t.ty + 1, qq = t.tz + 1, mp = (mc + 2) % 3;
-        const T mu_c = l_mu[mc][rr][qq], mu_xm = l_mu[mc][rr - 1][qq], mu_zm = l_mu[mc][rr][qq - 1];
-        const T mu_xzm = l_mu[mc][rr - 1][qq - 1];
-        const T mu_yc = l_mu[mp][rr][qq], mu_yxm = l_mu[mp][rr - 1][qq], mu_yzm = l_mu[mp][rr][qq - 1];
+        const int rr = t.ty + 1, qq = t.tz + 1;
+        const T mu_c = l_mu[b][rr][qq], mu_xm = l_mu[b][rr - 1][qq], mu_zm = l_mu[b][rr][qq - 1];
+        const T mu_xzm = l_mu[b][rr - 1][qq - 1];
         const T lr = ld_nt(a.ep[1], i);
         l2m = a.tdt * (lr + T(2) * mu_c);
         lam = a.tdt * lr;
-        const T sx_c = T(0.5) * (mu_c + mu_xm), sx_zm = T(0.5) * (mu_zm + mu_xzm), sx_ym = T(0.5) * (mu_yc + mu_yxm);
-        const T sy_c = T(0.5) * (mu_c + mu_yc), sy_zm = T(0.5) * (mu_zm + mu_yzm);
+        const T sx_c = T(0.5) * (mu_c + mu_xm), sx_zm = T(0.5) * (mu_zm + mu_xzm);
+        const T sy_c = T(0.5) * (mu_c + mu_prev), sy_zm = T(0.5) * (mu_zm + muz_prev);
         mxz = (T(0.5) * (sx_c + sx_zm)) * a.tdt;
-        mxy = (T(0.5) * (sx_c + sx_ym)) * a.tdt;
+        mxy = (T(0.5) * (sx_c + sx_prev)) * a.tdt;
         myz = (T(0.5) * (sy_c + sy_zm)) * a.tdt;
       } else if (MODE != MODE_BORN_ADJ) {
         l2m = ld_nt(a.m[L2M], i); lam = ld_nt(a.m[LAM], i);
@@ -663,15 +674,20 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
     }
 #pragma unroll
     for (int j = 0; j < 7; ++j) { qx[j] = qx[j + 1]; qy[j] = qy[j + 1]; qz[j] = qz[j + 1]; }
+    if (MED3) {
+      // this plane's terms become the previous plane's (every thread, active or not)
+      const int rr = t.ty + 1, qq = t.tz + 1;
+      mu_prev = l_mu[b][rr][qq];
+      muz_prev = l_mu[b][rr][qq - 1];
+      sx_prev = T(0.5) * (mu_prev + l_mu[b][rr - 1][qq]);
+    }
     if (more) {
       qx[7] = nx_.nx; qy[7] = nx_.ny; qz[7] = nx_.nz;
       stage(b ^ 1, nx_, qx[4], qy[3], qz[4]);
-      // plane iy + 1 replaces plane iy - 2, whose last reader was the previous iteration
-      if (MED3) stage_mu((mc + 1) % 3, nx_);
+      if (MED3) stage_mu(b ^ 1, nx_);
     }
     __syncthreads();
     b ^= 1;
-    mc = (mc + 1) % 3;
   }
 }
 

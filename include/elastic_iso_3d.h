@@ -334,10 +334,11 @@ int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* 
 /* Diagnostic streaming reference for the roofline: a plain kernel that reads
  * `reads` arrays of n FP32 values (16-byte accesses) and rewrites the first
  * `writes` of them in place -- the byte mix of the velocity (12, 3) and stress
- * (14, 6) kernels without neighbour accesses -- timed over `reps` launches on
- * the handle's stream; *gbs = (reads + writes) * 4 * n bytes per launch / time.
- * Mixes: (1, 1), (12, 3), (14, 6); n a multiple of 4. Allocates and frees
- * reads * n * 4 bytes. No counterpart in the reference. */
+ * (14, 6) kernels without neighbour accesses, or (10, 3) and (11, 6) when they
+ * form the medium from (rho, lambda, mu) -- timed over `reps` launches on the
+ * handle's stream; *gbs = (reads + writes) * 4 * n bytes per launch / time.
+ * Mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6); n a multiple of 4.
+ * Allocates and frees reads * n * 4 bytes. No counterpart in the reference. */
 int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, double* gbs);
 
 #ifdef __cplusplus

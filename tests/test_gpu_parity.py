@@ -239,13 +239,13 @@ def test_graph_replay_matches_eager(problems, precision):
 
 
 def test_stream_probe_reports_bandwidth():
-    """The streaming reference of the roofline runs its three mixes and
+    """The streaming reference of the roofline runs its five mixes and
     rejects others."""
     from elasticisolib_3d_amd._lib import ElaError
     from elasticisolib_3d_amd.propagator import Propagator
     g = GRIDS["odd"]
     prop = Propagator(g.nz, g.nx, g.ny, g.nts, g.sub, g.dz, g.dx, g.dy, g.dts, g.minPad, g.alphaCos, "f32")
-    for r, w in ((1, 1), (12, 3), (14, 6)):
+    for r, w in ((1, 1), (12, 3), (14, 6), (10, 3), (11, 6)):
         gbs = prop.stream_probe(r, w, 1 << 20, reps=3)
         print(f"stream probe {r}r/{w}w over 4 MiB arrays: {gbs:.0f} GB/s", flush=True)
         assert gbs > 0
@@ -261,8 +261,8 @@ def test_elastic_parameters_match_scaled_medium(problems, precision, name):
     """ela_set_elastic (rho, lambda, mu; medium formed in the stepper and the
     8 arrays derived on the GPU) against ela_set_medium with the host-scaled
     8 arrays: bit-identical in FP64 (same operations in the same order), FP32
-    to rounding, for the nonlinear forward and adjoint and the Born forward
-    and adjoint."""
+    to rounding, for the nonlinear forward and adjoint, the Born forward and
+    adjoint and the adjoint-state gradient."""
     from elasticisolib_3d_amd.propagator import Propagator, scale_medium
     g, P = GRIDS[name], problems[name]
     rlm = np.stack([P["rho"], P["lam"], P["mu"]])
@@ -274,11 +274,15 @@ def test_elastic_parameters_match_scaled_medium(problems, precision, name):
             prop.set_medium(med8)
         else:
             prop.set_elastic(rlm, g.dtw)
+        prop.set_gradient_checkpoints(3, 0, "native")
+        prop.gradient_forward(P["spos"], P["src"], P["rpos"])
+        g8, gs = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
         out[kind] = (prop.nl_forward(P["spos"], P["src"], P["rpos"]),
                      prop.nl_adjoint(P["spos"], P["rec"], P["rpos"]),
                      prop.born_forward(P["spos"], P["src"], P["dm"], P["rpos"]),
-                     [prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])])
-    for k, what in enumerate(("nl_forward", "nl_adjoint", "born_forward", "born_adjoint")):
+                     [prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])],
+                     [g8] + list(gs))
+    for k, what in enumerate(("nl_forward", "nl_adjoint", "born_forward", "born_adjoint", "gradient")):
         a, b = out["elastic"][k], out["med8"][k]
         err = rel(a, b)
         print(f"elastic vs med8 {precision} {name} {what}: {err:.2e}", flush=True)
