@@ -141,7 +141,7 @@ def bench_born(args):
     t0 = time.time()
     w = born_layered_400(nts=args.nts if args.nts != 626 else 501)
     log(f"[bench] model {w.name} padded {w.axes.nz}x{w.axes.nx}x{w.axes.ny} built in {time.time() - t0:.1f}s")
-    prop, spos, src, rpos, rec = gpu_setup(w)
+    prop, spos, src, rpos, rec = gpu_setup(w, "f32", args.medium)
     med = P.scale_medium(w.rho, w.lame, w.mu, w.dtw)
     dm3 = w.extra["dm3"]
     dm8 = P.born_stagger(dm3, med, w.dtw)
@@ -301,7 +301,7 @@ def bench_fwi(args):
     log(f"[bench] model {w.name} padded {ax.nz}x{ax.nx}x{ax.ny} built in {time.time() - t0:.1f}s")
     shots = _fwi_shots(w, ax)
     prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, checkpoint=args.checkpoint,
-                      autotune=not args.no_autotune)
+                      autotune=not args.no_autotune, medium=args.medium)
     t0 = time.perf_counter()
     prob.synthesize(w.rho, w.lame, w.mu)
     log(f"[bench] observed data of {len(shots)} shots synthesized in {time.perf_counter() - t0:.1f}s")
@@ -484,7 +484,7 @@ def main():
                                                  "e.g. 4x8/3/4 (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
     ap.add_argument("--medium", choices=("elastic", "scaled"), default="elastic",
-                    help="nl_forward: rho, lambda, mu with the medium formed in the steppers (120 B per cell-update), "
+                    help="nl_forward, born, fwi: rho, lambda, mu with the medium formed in the steppers (120 B per cell-update), "
                          "or the 8 scaled arrays of the reference's interface (140 B)")
     ap.add_argument("--no-stream-ref", action="store_true",
                     help="skip the streaming-kernel reference of the roofline")

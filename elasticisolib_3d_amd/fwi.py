@@ -80,11 +80,13 @@ class FwiProblem:
     """Shots on one padded grid, propagated on one GPU. `checkpoint` selects
     the storage of the gradient's outer checkpoints ("bf16", "s16" or
     "native"; None: the `bf16` flag), `autotune` times the stencil launch
-    shapes once when the first medium is set."""
+    shapes once when the first medium is set, `medium` chooses between
+    handing (rho, lambda, mu) to the library ("elastic") and the 8 host-scaled
+    arrays ("scaled"); the two agree exactly in FP64."""
 
     def __init__(self, axes: PaddedAxes, nts: int, dts: float, sub: int, alpha_cos: float, shots: Sequence[FwiShot],
                  precision: str = "f32", interval: int = 20, outer_interval: int = 0, bf16: bool = True,
-                 device: int = 0, checkpoint=None, autotune: bool = False):
+                 device: int = 0, checkpoint=None, autotune: bool = False, medium: str = "elastic"):
         self.axes = axes
         self.nts, self.dts, self.sub = int(nts), float(dts), int(sub)
         self.dtw = self.dts / self.sub
@@ -99,6 +101,11 @@ class FwiProblem:
         self.last_timing = {}
         # autotune: time the stencil launch shapes once, when the first medium is set
         self._tune, self.tuned = bool(autotune), None
+        # "elastic": hand (rho, lambda, mu) to the library, whose forward steps form the scaled medium
+        # (fewer bytes per step); "scaled": the 8 host-scaled arrays
+        if medium not in ("elastic", "scaled"):
+            raise ValueError("medium must be 'elastic' or 'scaled'")
+        self.medium = medium
 
     # -- helpers -----------------------------------------------------------
     def _tdtype(self):
@@ -126,7 +133,11 @@ class FwiProblem:
         med8 = P.scale_medium(np.asarray(rho, np.float64), np.asarray(lam, np.float64), np.asarray(mu, np.float64),
                               self.dtw)
         torch = _torch()
-        self.prop.set_medium(torch.tensor(med8, dtype=self._tdtype(), device=self.dev))
+        if self.medium == "elastic":
+            rlm = np.stack([np.asarray(rho, np.float64), np.asarray(lam, np.float64), np.asarray(mu, np.float64)])
+            self.prop.set_elastic(torch.tensor(rlm, dtype=self._tdtype(), device=self.dev), self.dtw)
+        else:
+            self.prop.set_medium(torch.tensor(med8, dtype=self._tdtype(), device=self.dev))
         if self._tune and self.tuned is None:
             from .autotune import autotune
             self.tuned = autotune(self.prop, reps=5)
