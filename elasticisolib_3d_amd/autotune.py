@@ -12,8 +12,8 @@ planes), so it is measured: every candidate
 is timed with HIP events on the propagator's own stream
 (ela_time_step_kernels: a velocity and a stress launch of a forward step on
 the handle's current wavefields, which it overwrites), and the fastest is
-applied. Results are cached per device name, precision and grid shape in a
-JSON file (ELA_TUNE_CACHE, default ~/.cache/elasticisolib_3d_amd/tuning.json).
+applied. Results are cached per device name, precision, medium kind and grid
+shape in a JSON file (ELA_TUNE_CACHE, default ~/.cache/elasticisolib_3d_amd/tuning.json).
 """
 from __future__ import annotations
 
@@ -48,7 +48,8 @@ def _device_name() -> str:
 
 def _key(prop) -> str:
     ny, nx, nz = prop.shape
-    return f"{_device_name()}|{prop.precision}|{nz}x{nx}x{ny}"
+    kind = getattr(prop, "medium_kind", "scaled")   # the steppers differ with the medium kind
+    return f"{_device_name()}|{prop.precision}|{kind}|{nz}x{nx}x{ny}"
 
 
 def _load() -> Dict[str, dict]:

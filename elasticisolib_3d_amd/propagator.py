@@ -63,6 +63,7 @@ class Propagator:
         check(self.lib.ela_create(ctypes.byref(self.cfg), ctypes.byref(h)), "ela_create")
         self.h = h
         self._keep = []
+        self.medium_kind = None   # "scaled" (set_medium) or "elastic" (set_elastic)
 
     def close(self):
         if getattr(self, "h", None):
@@ -158,6 +159,7 @@ class Propagator:
         if mem == _lib.ELA_MEM_HOST:
             med8 = np.ascontiguousarray(med8, dtype=self.dtype)
         check(self.lib.ela_set_medium(self.h, self._ptr(med8, self.dtype), mem), "ela_set_medium")
+        self.medium_kind = "scaled"
 
     def set_elastic(self, rlm3, dtw: float) -> None:
         """Elastic parameters (rho, lambda, mu) [3, ny, nx, nz] instead of the
@@ -170,6 +172,7 @@ class Propagator:
         if mem == _lib.ELA_MEM_HOST:
             rlm3 = np.ascontiguousarray(rlm3, dtype=self.dtype)
         check(self.lib.ela_set_elastic(self.h, self._ptr(rlm3, self.dtype), float(dtw), mem), "ela_set_elastic")
+        self.medium_kind = "elastic"
 
     def nl_forward(self, src_pos, src_traces, rec_pos, rec_out=None):
         mem = self._mem_of(*src_traces, *(rec_out or []))
