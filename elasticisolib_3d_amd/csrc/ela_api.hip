@@ -113,7 +113,7 @@ struct ela_handle {
   int64_t nzi = 0;          // internal row pitch
   int64_t zoff = 0;         // internal z origin
   int64_t n_int = 0;        // ny*nx*nzi
-  int64_t stride = 0;       // per-array stride in the library's buffers (n_int rounded up)
+  int64_t stride = 0;       // per-array stride in the library's buffers (n_int + guard, rounded up)
   int esize = 4;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
@@ -1452,7 +1452,8 @@ int ela_create(const ela_config* cfg, ela_handle** out) {
   h->zoff = line - ela::FAT;
   h->nzi = round_up(h->zoff + cfg->nz, line);
   h->n_int = h->nzi * cfg->nx * cfg->ny;
-  h->stride = round_up(h->n_int, 64);
+  // each array carries a guard past its last plane for the stencils' unpredicated loads
+  h->stride = round_up(h->n_int + ela::GUARD_ROWS * h->nzi + ela::GUARD_EXTRA, 64);
   auto bail = [&](int rc) {
     delete h;
     return rc;

@@ -148,6 +148,10 @@ __device__ __forceinline__ T dmx(const Plane<T, BX>& p, int r, int q, const T* c
          c[2] * (p[r + 2][q] - p[r - 3][q]) + c[3] * (p[r + 3][q] - p[r - 4][q]);
 }
 
+// Guard past the end of every library array (unpredicated stencil loads).
+constexpr int GUARD_ROWS = 16;      // >= the largest tile_x
+constexpr int GUARD_EXTRA = 2 * TZ;  // >= one tile height plus its z halo
+
 // 32-bit element offsets from a uniform base pointer: the compiler can use
 // the scalar-base + 32-bit vector-offset addressing mode, which saves the
 // 64-bit per-lane address arithmetic. The host guarantees that every array
@@ -192,45 +196,46 @@ __device__ __forceinline__ void st_nt(T* base, uint32_t i, T v) {
 // position, kernelsElaGpu_3D.cu:864-1040); MODE_BORN_ADJ_PM reads the same
 // products from the premultiplied level that the previous step wrote, so its
 // halo and queue loads are one array per field as in the forward stepper.
+//
+// The loads are not predicated: a tile's columns, halo rows and halo column
+// may lie outside the grid (past nz in the row, past nx in the plane), where
+// no interior cell's stencil reaches, so those values are never used; every
+// library array ends with a guard of GUARD_ROWS rows plus GUARD_EXTRA elements
+// past its last plane, which holds the farthest such position (at most
+// (tile_x - 1) rows plus one tile height past the array's end), so the loads
+// stay inside the allocation.
 template <typename T, int MODE>
 struct StressVal {
   const StepArgs<T>& a;
-  bool ok;
-  __device__ __forceinline__ T pm(int f, uint32_t i) const { return ok ? *at(a.q[f], i) : T(0); }
+  __device__ __forceinline__ T pm(int f, uint32_t i) const { return *at(a.q[f], i); }
   __device__ __forceinline__ T sxx(uint32_t i) const {
     if (MODE == MODE_BORN_ADJ_PM) return pm(SXX, i);
-    if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
       return *at(a.c[SXX], i) * *at(a.m[L2M], i) + (*at(a.c[SYY], i) + *at(a.c[SZZ], i)) * *at(a.m[LAM], i);
     return *at(a.c[SXX], i);
   }
   __device__ __forceinline__ T syy(uint32_t i) const {
     if (MODE == MODE_BORN_ADJ_PM) return pm(SYY, i);
-    if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
       return *at(a.c[SYY], i) * *at(a.m[L2M], i) + (*at(a.c[SXX], i) + *at(a.c[SZZ], i)) * *at(a.m[LAM], i);
     return *at(a.c[SYY], i);
   }
   __device__ __forceinline__ T szz(uint32_t i) const {
     if (MODE == MODE_BORN_ADJ_PM) return pm(SZZ, i);
-    if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ)
       return *at(a.c[SZZ], i) * *at(a.m[L2M], i) + (*at(a.c[SXX], i) + *at(a.c[SYY], i)) * *at(a.m[LAM], i);
     return *at(a.c[SZZ], i);
   }
   __device__ __forceinline__ T sxz(uint32_t i) const {
     if (MODE == MODE_BORN_ADJ_PM) return pm(SXZ, i);
-    if (!ok) return T(0);
     return MODE == MODE_BORN_ADJ ? *at(a.c[SXZ], i) * *at(a.m[MXZ], i) : *at(a.c[SXZ], i);
   }
   __device__ __forceinline__ T sxy(uint32_t i) const {
     if (MODE == MODE_BORN_ADJ_PM) return pm(SXY, i);
-    if (!ok) return T(0);
     return MODE == MODE_BORN_ADJ ? *at(a.c[SXY], i) * *at(a.m[MXY], i) : *at(a.c[SXY], i);
   }
   __device__ __forceinline__ T syz(uint32_t i) const {
     if (MODE == MODE_BORN_ADJ_PM) return pm(SYZ, i);
-    if (!ok) return T(0);
     return MODE == MODE_BORN_ADJ ? *at(a.c[SYZ], i) * *at(a.m[MYZ], i) : *at(a.c[SYZ], i);
   }
 };
@@ -238,9 +243,7 @@ struct StressVal {
 template <typename T, int MODE>
 struct VelVal {
   const StepArgs<T>& a;
-  bool ok;
   __device__ __forceinline__ T v(int f, uint32_t i) const {
-    if (!ok) return T(0);
     if (MODE == MODE_BORN_ADJ_PM) return *at(a.q[f], i);
     return MODE == MODE_BORN_ADJ ? *at(a.c[f], i) * *at(a.m[f], i) : *at(a.c[f], i);  // BX_/BY_/BZ_ == VX/VY/VZ
   }
@@ -374,7 +377,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
-  const StressVal<T, MODE> S{a, t.okc}, SXl{a, t.okxl}, SXh{a, t.okxh}, SZ{a, t.okzh};
+  const StressVal<T, MODE> S{a}, SXl{a}, SXh{a}, SZ{a};
 
   auto fetch = [&](VelStage<T>& v, int p) {
     const uint32_t pl = (uint32_t)p * ys;
@@ -393,10 +396,10 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
       v.hzz = SZ.szz(pl + t.czh); v.hxz = SZ.sxz(pl + t.czh); v.hyz = SZ.syz(pl + t.czh);
     }
     if (MED3) {
-      // the column's own cell and its x - 1 / z - 1 neighbours (inside the grid whenever the column is)
-      v.rho = t.okc ? *at(a.ep[0], pl + t.col) : T(0);
-      if (t.ty == 0) v.rho_hx = t.okc ? *at(a.ep[0], pl + t.col - (uint32_t)a.nzi) : T(0);
-      if (t.tz == 0) v.rho_hz = t.okc ? *at(a.ep[0], pl + t.col - 1u) : T(0);
+      // the column's own cell and its x - 1 / z - 1 neighbours
+      v.rho = *at(a.ep[0], pl + t.col);
+      if (t.ty == 0) v.rho_hx = *at(a.ep[0], pl + t.col - (uint32_t)a.nzi);
+      if (t.tz == 0) v.rho_hz = *at(a.ep[0], pl + t.col - 1u);
     }
   };
   const int r = t.ty + FAT, q = t.tz + FAT;
@@ -415,7 +418,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
     }
   };
   // F_MED3: density of the column on the previous plane (y - 1)
-  T rho_prev = (MED3 && t.okc) ? *at(a.ep[0], (uint32_t)(t.y0 - 1) * ys + t.col) : T(0);
+  T rho_prev = MED3 ? *at(a.ep[0], (uint32_t)(t.y0 - 1) * ys + t.col) : T(0);
 
   // queues: sxy (D+y, q[j] = f[iy-3+j]), syy (D-y, f[iy-4+j]), syz (D+y)
   T qxy[8], qyy[8], qyz[8];
@@ -510,7 +513,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
-  const VelVal<T, MODE> V{a, t.okc}, Vlo{a, t.okxl}, Vhi{a, t.okxh}, Vzh{a, t.okzh};
+  const VelVal<T, MODE> V{a}, Vlo{a}, Vhi{a}, Vzh{a};
 
   auto fetch = [&](StressStage<T>& v, int p) {
     const uint32_t pl = (uint32_t)p * ys;
@@ -526,13 +529,13 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       for (int f = 0; f < 3; ++f) v.hz[f] = Vzh.v(f, pl + t.czh);
     }
   };
-  // F_MED3: shear modulus of plane p around the column (inside the grid whenever the column is)
+  // F_MED3: shear modulus of plane p around the column
   auto fetch_mu = [&](StressStage<T>& v, int p) {
     const uint32_t pl = (uint32_t)p * ys, nzi = (uint32_t)a.nzi;
-    v.mu = t.okc ? *at(a.ep[2], pl + t.col) : T(0);
-    if (t.ty == 0) v.mu_hx = t.okc ? *at(a.ep[2], pl + t.col - nzi) : T(0);
-    if (t.tz == 0) v.mu_hz = t.okc ? *at(a.ep[2], pl + t.col - 1u) : T(0);
-    if (t.ty == 0 && t.tz == 0) v.mu_hc = t.okc ? *at(a.ep[2], pl + t.col - nzi - 1u) : T(0);
+    v.mu = *at(a.ep[2], pl + t.col);
+    if (t.ty == 0) v.mu_hx = *at(a.ep[2], pl + t.col - nzi);
+    if (t.tz == 0) v.mu_hz = *at(a.ep[2], pl + t.col - 1u);
+    if (t.ty == 0 && t.tz == 0) v.mu_hc = *at(a.ep[2], pl + t.col - nzi - 1u);
   };
   auto stage_mu = [&](int m, const StressStage<T>& v) {
     l_mu[m][t.ty + 1][t.tz + 1] = v.mu;
@@ -563,7 +566,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   // column, mu of the column and of its z - 1 neighbour -- which are the
   // current plane's values one iteration later (same operands, same order)
   T sx_prev = T(0), mu_prev = T(0), muz_prev = T(0);
-  if (MED3 && t.okc) {
+  if (MED3) {
     const uint32_t pl = (uint32_t)(t.y0 - 1) * ys;
     mu_prev = *at(a.ep[2], pl + t.col);
     muz_prev = *at(a.ep[2], pl + t.col - 1u);
