@@ -191,6 +191,14 @@ __device__ __forceinline__ void st_nt(T* base, uint32_t i, T v) {
 #endif
 }
 
+// Direct global -> LDS load of one row of a tile (FP32): lane l of the wave
+// writes row[l], so a 64-lane wave fills TZ consecutive LDS values without
+// passing them through registers. Completion is covered by the wait that
+// precedes the block's next barrier.
+__device__ __forceinline__ void row_to_lds(const float* base, uint32_t i, float* row) {
+  __builtin_amdgcn_global_load_lds(at(base, i), (__attribute__((address_space(3))) void*)row, 4, 0, 0);
+}
+
 // Values that the derivative operators see. In MODE_BORN_ADJ they are the
 // medium-multiplied fields (stepAdjGpuBorn_3D multiplies at every loaded
 // position, kernelsElaGpu_3D.cu:864-1040); MODE_BORN_ADJ_PM reads the same
@@ -340,6 +348,14 @@ __device__ __forceinline__ TileCoord tile_coord(int nbz, int nbx, int nby, int y
 // land in the second LDS buffer, so one barrier per plane suffices and the
 // HBM latency overlaps the stencil arithmetic.
 // ---------------------------------------------------------------------------
+// The velocity stepper's LDS: double-buffered planes of sxx, szz, sxy, sxz,
+// syz and (F_MED3) density planes with the low x row and low z column.
+template <typename T, int BX, bool MED3>
+struct VelLds {
+  Plane<T, BX> xx[2], zz[2], xy[2], xz[2], yz[2];
+  T rho[2][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
+};
+
 template <typename T>
 struct VelStage {
   T xx, zz, xz;                 // LDS centres of the non-queue fields
@@ -371,49 +387,79 @@ template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
   static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
-  __shared__ Plane<T, BX> l_xx[2], l_zz[2], l_xy[2], l_xz[2], l_yz[2];
-  // F_MED3: density planes with the low x row and low z column
-  __shared__ T l_rho[2][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
+  // one LDS object: the compiler's wait analysis then tells the rows the
+  // direct loads fill (buffer b ^ 1) from the rows the stencil reads (buffer b)
+  __shared__ VelLds<T, BX, MED3> L;
+  auto& l_xx = L.xx; auto& l_zz = L.zz; auto& l_xy = L.xy; auto& l_xz = L.xz; auto& l_yz = L.yz;
+  auto& l_rho = L.rho;
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
   const StressVal<T, MODE> S{a}, SXl{a}, SXh{a}, SZ{a};
+  // FP32 rows that are plain array values go straight from HBM to LDS
+  // (MODE_BORN_ADJ multiplies them by the medium first, in registers)
+  constexpr bool DMA = sizeof(T) == 4 && MODE != MODE_BORN_ADJ;
+  auto src = [&](int f) -> const T* { return MODE == MODE_BORN_ADJ_PM ? a.q[f] : a.c[f]; };
+  const int r = t.ty + FAT, q = t.tz + FAT;
+  // a wave is one row of the tile (blockDim.x == TZ == wavefront size): its row index is wave-uniform
+  const int wy = __builtin_amdgcn_readfirstlane(t.ty), wr = wy + FAT;
 
-  auto fetch = [&](VelStage<T>& v, int p) {
+  // plane p's values: LDS rows of buffer nb directly (DMA), the rest into v
+  auto fetch = [&](VelStage<T>& v, int p, int nb) {
     const uint32_t pl = (uint32_t)p * ys;
-    v.xx = S.sxx(pl + t.col);
-    v.zz = S.szz(pl + t.col);
-    v.xz = S.sxz(pl + t.col);
+    if constexpr (DMA) {
+      row_to_lds(src(SXX), pl + t.col, &l_xx[nb][wr][FAT]);
+      row_to_lds(src(SZZ), pl + t.col, &l_zz[nb][wr][FAT]);
+      row_to_lds(src(SXZ), pl + t.col, &l_xz[nb][wr][FAT]);
+    } else {
+      v.xx = S.sxx(pl + t.col);
+      v.zz = S.szz(pl + t.col);
+      v.xz = S.sxz(pl + t.col);
+    }
     v.nxy = S.sxy(pl + 4 * ys + t.col);
     v.nyy = S.syy(pl + 3 * ys + t.col);
     v.nyz = S.syz(pl + 4 * ys + t.col);
-    if (t.xh) {
-      v.hxx0 = SXl.sxx(pl + t.cxl); v.hxx1 = SXh.sxx(pl + t.cxh);
-      v.hxy0 = SXl.sxy(pl + t.cxl); v.hxy1 = SXh.sxy(pl + t.cxh);
-      v.hxz0 = SXl.sxz(pl + t.cxl); v.hxz1 = SXh.sxz(pl + t.cxh);
+    if (wy < FAT) {  // the x-halo loaders (t.xh) are whole waves
+      if constexpr (DMA) {
+        row_to_lds(src(SXX), pl + t.cxl, &l_xx[nb][wy][FAT]); row_to_lds(src(SXX), pl + t.cxh, &l_xx[nb][wr + BX][FAT]);
+        row_to_lds(src(SXY), pl + t.cxl, &l_xy[nb][wy][FAT]); row_to_lds(src(SXY), pl + t.cxh, &l_xy[nb][wr + BX][FAT]);
+        row_to_lds(src(SXZ), pl + t.cxl, &l_xz[nb][wy][FAT]); row_to_lds(src(SXZ), pl + t.cxh, &l_xz[nb][wr + BX][FAT]);
+      } else {
+        v.hxx0 = SXl.sxx(pl + t.cxl); v.hxx1 = SXh.sxx(pl + t.cxh);
+        v.hxy0 = SXl.sxy(pl + t.cxl); v.hxy1 = SXh.sxy(pl + t.cxh);
+        v.hxz0 = SXl.sxz(pl + t.cxl); v.hxz1 = SXh.sxz(pl + t.cxh);
+      }
     }
     if (t.zh) {
       v.hzz = SZ.szz(pl + t.czh); v.hxz = SZ.sxz(pl + t.czh); v.hyz = SZ.syz(pl + t.czh);
     }
     if (MED3) {
       // the column's own cell and its x - 1 / z - 1 neighbours
-      v.rho = *at(a.ep[0], pl + t.col);
-      if (t.ty == 0) v.rho_hx = *at(a.ep[0], pl + t.col - (uint32_t)a.nzi);
+      if constexpr (DMA) {
+        row_to_lds(a.ep[0], pl + t.col, &l_rho[nb][wy + 1][1]);
+        if (wy == 0) row_to_lds(a.ep[0], pl + t.col - (uint32_t)a.nzi, &l_rho[nb][0][1]);
+      } else {
+        v.rho = *at(a.ep[0], pl + t.col);
+        if (t.ty == 0) v.rho_hx = *at(a.ep[0], pl + t.col - (uint32_t)a.nzi);
+      }
       if (t.tz == 0) v.rho_hz = *at(a.ep[0], pl + t.col - 1u);
     }
   };
-  const int r = t.ty + FAT, q = t.tz + FAT;
+  // the values fetch kept in registers, and the queue centres
   auto stage = [&](int b, const VelStage<T>& v, T cxy, T cyz) {
-    l_xx[b][r][q] = v.xx; l_zz[b][r][q] = v.zz; l_xz[b][r][q] = v.xz; l_xy[b][r][q] = cxy; l_yz[b][r][q] = cyz;
-    if (t.xh) {
+    if constexpr (!DMA) { l_xx[b][r][q] = v.xx; l_zz[b][r][q] = v.zz; l_xz[b][r][q] = v.xz; }
+    l_xy[b][r][q] = cxy; l_yz[b][r][q] = cyz;
+    if (!DMA && t.xh) {
       l_xx[b][t.ty][q] = v.hxx0; l_xx[b][r + BX][q] = v.hxx1;
       l_xy[b][t.ty][q] = v.hxy0; l_xy[b][r + BX][q] = v.hxy1;
       l_xz[b][t.ty][q] = v.hxz0; l_xz[b][r + BX][q] = v.hxz1;
     }
     if (t.zh) { l_zz[b][r][t.colh] = v.hzz; l_xz[b][r][t.colh] = v.hxz; l_yz[b][r][t.colh] = v.hyz; }
     if (MED3) {
-      l_rho[b][t.ty + 1][t.tz + 1] = v.rho;
-      if (t.ty == 0) l_rho[b][0][t.tz + 1] = v.rho_hx;
+      if constexpr (!DMA) {
+        l_rho[b][t.ty + 1][t.tz + 1] = v.rho;
+        if (t.ty == 0) l_rho[b][0][t.tz + 1] = v.rho_hx;
+      }
       if (t.tz == 0) l_rho[b][t.ty + 1][0] = v.rho_hz;
     }
   };
@@ -430,7 +476,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   }
   {
     VelStage<T> v0;
-    fetch(v0, t.y0);
+    fetch(v0, t.y0, 0);
     qxy[7] = v0.nxy; qyy[7] = v0.nyy; qyz[7] = v0.nyz;
     stage(0, v0, qxy[3], qyz[3]);
   }
@@ -439,7 +485,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   for (int iy = t.y0; iy < t.y1; ++iy) {
     const bool more = iy + 1 < t.y1;
     VelStage<T> nx_;
-    if (more) fetch(nx_, iy + 1);
+    if (more) fetch(nx_, iy + 1, b ^ 1);  // buffer b ^ 1 is free: every wave passed the last barrier
     if (t.active) {
       const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
@@ -496,6 +542,14 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Stress update: six stresses from vx, vy, vz (same pipeline).
 // ---------------------------------------------------------------------------
+// The stress stepper's LDS: double-buffered planes of vx, vy, vz and (F_MED3)
+// shear-modulus planes with the low x row and low z column.
+template <typename T, int BX, bool MED3>
+struct StressLds {
+  Plane<T, BX> vx[2], vy[2], vz[2];
+  T mu[2][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
+};
+
 template <typename T>
 struct StressStage {
   T hlo[3], hhi[3], hz[3];   // x-halo rows and z-halo column of vx, vy, vz
@@ -507,22 +561,33 @@ template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
   static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
-  __shared__ Plane<T, BX> l_vx[2], l_vy[2], l_vz[2];
-  // F_MED3: shear-modulus planes (double-buffered like the velocity planes) with the low x row and low z column
-  __shared__ T l_mu[2][MED3 ? BX + 1 : 1][MED3 ? TZ + 1 : 1];
+  __shared__ StressLds<T, BX, MED3> L;  // one LDS object (see k_velocity)
+  auto& l_vx = L.vx; auto& l_vy = L.vy; auto& l_vz = L.vz; auto& l_mu = L.mu;
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
   const uint32_t ys = (uint32_t)a.ys;
   const VelVal<T, MODE> V{a}, Vlo{a}, Vhi{a}, Vzh{a};
+  // FP32 rows that are plain array values go straight from HBM to LDS (see k_velocity)
+  constexpr bool DMA = sizeof(T) == 4 && MODE != MODE_BORN_ADJ;
+  auto src = [&](int f) -> const T* { return MODE == MODE_BORN_ADJ_PM ? a.q[f] : a.c[f]; };
+  const int r = t.ty + FAT, q = t.tz + FAT;
+  const int wy = __builtin_amdgcn_readfirstlane(t.ty), wr = wy + FAT;  // wave-uniform row
 
-  auto fetch = [&](StressStage<T>& v, int p) {
+  // plane p's values: LDS rows of buffer nb directly (DMA), the rest into v
+  auto fetch = [&](StressStage<T>& v, int p, int nb) {
     const uint32_t pl = (uint32_t)p * ys;
     v.nx = V.v(VX, pl + 3 * ys + t.col);
     v.ny = V.v(VY, pl + 4 * ys + t.col);
     v.nz = V.v(VZ, pl + 3 * ys + t.col);
-    if (t.xh) {
+    if (wy < FAT) {  // the x-halo loaders (t.xh) are whole waves
+      if constexpr (DMA) {
+        row_to_lds(src(VX), pl + t.cxl, &l_vx[nb][wy][FAT]); row_to_lds(src(VX), pl + t.cxh, &l_vx[nb][wr + BX][FAT]);
+        row_to_lds(src(VY), pl + t.cxl, &l_vy[nb][wy][FAT]); row_to_lds(src(VY), pl + t.cxh, &l_vy[nb][wr + BX][FAT]);
+        row_to_lds(src(VZ), pl + t.cxl, &l_vz[nb][wy][FAT]); row_to_lds(src(VZ), pl + t.cxh, &l_vz[nb][wr + BX][FAT]);
+      } else {
 #pragma unroll
-      for (int f = 0; f < 3; ++f) { v.hlo[f] = Vlo.v(f, pl + t.cxl); v.hhi[f] = Vhi.v(f, pl + t.cxh); }
+        for (int f = 0; f < 3; ++f) { v.hlo[f] = Vlo.v(f, pl + t.cxl); v.hhi[f] = Vhi.v(f, pl + t.cxh); }
+      }
     }
     if (t.zh) {
 #pragma unroll
@@ -530,23 +595,30 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
     }
   };
   // F_MED3: shear modulus of plane p around the column
-  auto fetch_mu = [&](StressStage<T>& v, int p) {
+  auto fetch_mu = [&](StressStage<T>& v, int p, int nb) {
     const uint32_t pl = (uint32_t)p * ys, nzi = (uint32_t)a.nzi;
-    v.mu = *at(a.ep[2], pl + t.col);
-    if (t.ty == 0) v.mu_hx = *at(a.ep[2], pl + t.col - nzi);
+    if constexpr (DMA) {
+      row_to_lds(a.ep[2], pl + t.col, &l_mu[nb][wy + 1][1]);
+      if (wy == 0) row_to_lds(a.ep[2], pl + t.col - nzi, &l_mu[nb][0][1]);
+    } else {
+      v.mu = *at(a.ep[2], pl + t.col);
+      if (t.ty == 0) v.mu_hx = *at(a.ep[2], pl + t.col - nzi);
+    }
     if (t.tz == 0) v.mu_hz = *at(a.ep[2], pl + t.col - 1u);
     if (t.ty == 0 && t.tz == 0) v.mu_hc = *at(a.ep[2], pl + t.col - nzi - 1u);
   };
   auto stage_mu = [&](int m, const StressStage<T>& v) {
-    l_mu[m][t.ty + 1][t.tz + 1] = v.mu;
-    if (t.ty == 0) l_mu[m][0][t.tz + 1] = v.mu_hx;
+    if constexpr (!DMA) {
+      l_mu[m][t.ty + 1][t.tz + 1] = v.mu;
+      if (t.ty == 0) l_mu[m][0][t.tz + 1] = v.mu_hx;
+    }
     if (t.tz == 0) l_mu[m][t.ty + 1][0] = v.mu_hz;
     if (t.ty == 0 && t.tz == 0) l_mu[m][0][0] = v.mu_hc;
   };
-  const int r = t.ty + FAT, q = t.tz + FAT;
+  // the values fetch kept in registers, and the queue centres
   auto stage = [&](int b, const StressStage<T>& v, T cvx, T cvy, T cvz) {
     l_vx[b][r][q] = cvx; l_vy[b][r][q] = cvy; l_vz[b][r][q] = cvz;
-    if (t.xh) {
+    if (!DMA && t.xh) {
       l_vx[b][t.ty][q] = v.hlo[0]; l_vx[b][r + BX][q] = v.hhi[0];
       l_vy[b][t.ty][q] = v.hlo[1]; l_vy[b][r + BX][q] = v.hhi[1];
       l_vz[b][t.ty][q] = v.hlo[2]; l_vz[b][r + BX][q] = v.hhi[2];
@@ -574,11 +646,11 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   }
   {
     StressStage<T> v0;
-    fetch(v0, t.y0);
+    fetch(v0, t.y0, 0);
     qx[7] = v0.nx; qy[7] = v0.ny; qz[7] = v0.nz;
     stage(0, v0, qx[4], qy[3], qz[4]);
     if (MED3) {
-      fetch_mu(v0, t.y0);
+      fetch_mu(v0, t.y0, 0);
       stage_mu(0, v0);
     }
   }
@@ -587,9 +659,9 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   for (int iy = t.y0; iy < t.y1; ++iy) {
     const bool more = iy + 1 < t.y1;
     StressStage<T> nx_;
-    if (more) {
-      fetch(nx_, iy + 1);
-      if (MED3) fetch_mu(nx_, iy + 1);
+    if (more) {  // buffer b ^ 1 is free: every wave passed the last barrier
+      fetch(nx_, iy + 1, b ^ 1);
+      if (MED3) fetch_mu(nx_, iy + 1, b ^ 1);
     }
     if (t.active) {
       const uint32_t i = (uint32_t)iy * ys + t.col;
