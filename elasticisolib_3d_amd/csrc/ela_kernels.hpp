@@ -191,6 +191,11 @@ __device__ __forceinline__ void st_nt(T* base, uint32_t i, T v) {
 #endif
 }
 
+// FP32 LDS rows loaded straight from HBM (1) or staged through registers (0).
+#ifndef ELA_LDS_DMA
+#define ELA_LDS_DMA 1
+#endif
+
 // Direct global -> LDS load of one row of a tile (FP32): lane l of the wave
 // writes row[l], so a 64-lane wave fills TZ consecutive LDS values without
 // passing them through registers. Completion is covered by the wait that
@@ -373,14 +378,23 @@ struct VelStage {
 // lambda; muxzDtw = 2 dtw * Sz(Sx(mu)) with S the same half-cell average, muxy
 // = Sy(Sx(mu)), muyz = Sz(Sy(mu)) -- the operations of ela_scale_medium in the
 // same order, so an FP64 stepper reproduces the 8-array stepper exactly.
+//
+// buoy: 2 dtw / rho_h for a staggered density rho_h; FP32 uses one hardware
+// reciprocal (1 ulp) instead of the correctly rounded division, in the
+// steppers and in k_scale_medium alike, so that every FP32 mode of one handle
+// sees the same buoyancies.
+template <typename T>
+__device__ __forceinline__ T buoy(T tdt, T rho_h) {
+  if constexpr (sizeof(T) == 4) {
+    return tdt * __builtin_amdgcn_rcpf(rho_h);
+  } else {
+    return tdt / rho_h;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T stag_buoy(T tdt, T c, T m) {
-  if constexpr (sizeof(T) == 4) {
-    // FP32: one hardware reciprocal (1 ulp) instead of the correctly rounded division
-    return tdt * __builtin_amdgcn_rcpf(T(0.5) * (c + m));
-  } else {
-    return tdt / (T(0.5) * (c + m));
-  }
+  return buoy(tdt, T(0.5) * (c + m));
 }
 
 template <typename T, int BX, int MODE, int F>
@@ -398,7 +412,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   const StressVal<T, MODE> S{a}, SXl{a}, SXh{a}, SZ{a};
   // FP32 rows that are plain array values go straight from HBM to LDS
   // (MODE_BORN_ADJ multiplies them by the medium first, in registers)
-  constexpr bool DMA = sizeof(T) == 4 && MODE != MODE_BORN_ADJ;
+  constexpr bool DMA = ELA_LDS_DMA && sizeof(T) == 4 && MODE != MODE_BORN_ADJ;
   auto src = [&](int f) -> const T* { return MODE == MODE_BORN_ADJ_PM ? a.q[f] : a.c[f]; };
   const int r = t.ty + FAT, q = t.tz + FAT;
   // a wave is one row of the tile (blockDim.x == TZ == wavefront size): its row index is wave-uniform
@@ -568,7 +582,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   const uint32_t ys = (uint32_t)a.ys;
   const VelVal<T, MODE> V{a}, Vlo{a}, Vhi{a}, Vzh{a};
   // FP32 rows that are plain array values go straight from HBM to LDS (see k_velocity)
-  constexpr bool DMA = sizeof(T) == 4 && MODE != MODE_BORN_ADJ;
+  constexpr bool DMA = ELA_LDS_DMA && sizeof(T) == 4 && MODE != MODE_BORN_ADJ;
   auto src = [&](int f) -> const T* { return MODE == MODE_BORN_ADJ_PM ? a.q[f] : a.c[f]; };
   const int r = t.ty + FAT, q = t.tz + FAT;
   const int wy = __builtin_amdgcn_readfirstlane(t.ty), wr = wy + FAT;  // wave-uniform row
@@ -1134,7 +1148,8 @@ __global__ void __launch_bounds__(256) k_ck_unpack(CkArgs<T> a, const void* in, 
 // The 8 scaled medium arrays from (rho, lambda, mu) on every cell of the
 // padded grid with the host scaling's edge rules (the half-cell average
 // halves the first sample and adds half the neighbour to the last one): the
-// device twin of ela_scale_medium, same operations in the same order.
+// device twin of ela_scale_medium, same operations in the same order (FP32
+// buoyancies with the steppers' reciprocal, see buoy).
 template <typename T>
 struct MedArgs {
   const T* ep[3];        // rho, lambda, mu (internal layout)
@@ -1161,9 +1176,9 @@ __global__ void __launch_bounds__(256) k_scale_medium(MedArgs<T> a) {
     const T* rho = a.ep[0];
     const T* lam = a.ep[1];
     const T* mu = a.ep[2];
-    a.m[BX_][i] = a.tdt / half_cell(rho, i, sx, ix, a.nx);
-    a.m[BY_][i] = a.tdt / half_cell(rho, i, sy, iy, a.ny);
-    a.m[BZ_][i] = a.tdt / half_cell(rho, i, (int64_t)1, iz, a.nz);
+    a.m[BX_][i] = buoy(a.tdt, half_cell(rho, i, sx, ix, a.nx));
+    a.m[BY_][i] = buoy(a.tdt, half_cell(rho, i, sy, iy, a.ny));
+    a.m[BZ_][i] = buoy(a.tdt, half_cell(rho, i, (int64_t)1, iz, a.nz));
     a.m[L2M][i] = a.tdt * (lam[i] + T(2) * mu[i]);
     a.m[LAM][i] = a.tdt * lam[i];
     // mu staggered along x, then along z (muxz) or y (muxy); along y, then z (muyz)
