@@ -217,6 +217,32 @@ def test_tile_orders_give_identical_results(problems, order, xgroup):
         assert all(np.array_equal(a, b) for a, b in zip(out, ref)), (tx, yc)
 
 
+@pytest.mark.parametrize("medium", ["scaled", "elastic"])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_tilings_give_identical_results(problems, precision, medium):
+    """Every cell's update uses the same operands in the same order whatever
+    the tile width and y chunk, so all launch shapes give bit-identical
+    results: the tile rows and halo rows that FP32 loads straight into LDS
+    are filled by whole waves, one wave per row, and a tile 16 wide has 12
+    rows (16 waves) that take no x halo. Nonlinear forward (forward stepper)
+    and Born adjoint (premultiplied transposed stepper), both medium kinds."""
+    g, P = GRIDS["wide"], problems["wide"]
+    prop = make_prop(g, precision)
+    if medium == "scaled":
+        prop.set_medium(P["med"])
+    else:
+        prop.set_elastic(np.stack([P["rho"], P["lam"], P["mu"]]), g.dtw)
+    out = {}
+    for tx, yc in ((4, 5), (8, 3), (16, 8), (16, 1)):
+        prop.set_tiling(tx, yc)
+        out[(tx, yc)] = (prop.nl_forward(P["spos"], P["src"], P["rpos"]),
+                         [prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])])
+    ref = out[(4, 5)]
+    for k, v in out.items():
+        for a, b in zip(v, ref):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), (k, precision, medium)
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_graph_replay_matches_eager(problems, precision):
     """The coarse step replayed as a HIP graph (nts - 1 replays per call, the
