@@ -111,11 +111,11 @@ class WindowStack:
         g = _unpad(g, -3, FAT + self.pad, w.ny)
         out[..., w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :] += g
 
-    def stack(self, per_window: Sequence[np.ndarray], dtype=None) -> np.ndarray:
+    def stack(self, per_window: Sequence[np.ndarray], dtype=None, fill: float = 0.0) -> np.ndarray:
         """[..., ny_w, nx_w, nz] arrays -> one stacked [..., ny, nx, nz] array
-        (zeros beyond a narrower window)."""
+        (`fill` beyond a narrower window)."""
         lead = per_window[0].shape[:-3]
-        out = np.zeros(lead + (self.ny, self.nx, self.nz), dtype or per_window[0].dtype)
+        out = np.full(lead + (self.ny, self.nx, self.nz), fill, dtype or per_window[0].dtype)
         for k, a in enumerate(per_window):
             out[..., self.y_off[k]:self.y_off[k] + self.ny_w[k], :self.nx_w[k], :] = a
         return out
@@ -129,6 +129,13 @@ class WindowStack:
         meds = [P.scale_medium(self.window_model(k, rho), self.window_model(k, lam), self.window_model(k, mu), dtw)
                 for k in range(len(self.windows))]
         return self.stack(meds), meds
+
+    def stacked_elastic(self, rho, lam, mu) -> np.ndarray:
+        """Stacked [3, ny, nx, nz] (rho, lambda, mu) of the padded windows, for
+        a propagator that forms the scaled medium itself. Cells beyond a
+        narrower window (never within reach of an interior stencil) hold 1."""
+        return self.stack([np.stack([self.window_model(k, rho), self.window_model(k, lam), self.window_model(k, mu)])
+                           for k in range(len(self.windows))], fill=1.0)
 
     # -- points -----------------------------------------------------------
     def stacked_positions(self, k: int, pos: Sequence[np.ndarray]) -> List[np.ndarray]:
@@ -162,11 +169,13 @@ class BatchedFwi:
     FwiProblem). `model_shape` is the [ny, nx, nz] shape of the (unpadded)
     model arrays; `checkpoint` ("bf16", "s16", "native") overrides the older
     `bf16` flag; `autotune` times the launch shapes of every stacked grid
-    unless `tiling` fixes them."""
+    unless `tiling` fixes them; `medium` ("elastic" or "scaled") chooses how
+    the stacked model reaches the library, as in FwiProblem."""
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
-                 bf16: bool = True, device: int = 0, tiling=None, checkpoint=None, autotune: bool = False):
+                 bf16: bool = True, device: int = 0, tiling=None, checkpoint=None, autotune: bool = False,
+                 medium: str = "elastic"):
         self.model_shape, self.d, self.pad = tuple(model_shape), float(d), int(pad)
         self.nts, self.dts, self.sub, self.alpha = int(nts), float(dts), int(sub), float(alpha_cos)
         self.dtw = self.dts / self.sub
@@ -176,6 +185,11 @@ class BatchedFwi:
         self.checkpoint = bool(bf16) if checkpoint is None else checkpoint   # "bf16", "s16", "native" or a bool
         self.tiling = tiling
         self.autotune = bool(autotune)   # time the launch shapes of every stacked grid (cached per grid shape)
+        # "elastic": hand the stacked (rho, lambda, mu) to the library (its forward steps form the
+        # scaled medium); "scaled": the stacked 8 scaled arrays (ela_set_medium)
+        if medium not in ("elastic", "scaled"):
+            raise ValueError("medium must be 'elastic' or 'scaled'")
+        self.medium = medium
         self.timing = {}
 
     def _groups(self):
@@ -195,7 +209,11 @@ class BatchedFwi:
             if len(self.tiling) > 2:
                 prop.set_tile_order(*self.tiling[2:])
         td = torch.float32 if self.precision == "f32" else torch.float64
-        prop.set_medium(torch.tensor(med, dtype=td, device="cuda"))
+        if self.medium == "elastic":
+            rlm = ws.stacked_elastic(model[0], model[1], model[2])
+            prop.set_elastic(torch.tensor(rlm, dtype=td, device="cuda"), self.dtw)
+        else:
+            prop.set_medium(torch.tensor(med, dtype=td, device="cuda"))
         if self.autotune and not self.tiling:
             from .autotune import autotune
             autotune(prop, reps=5)
