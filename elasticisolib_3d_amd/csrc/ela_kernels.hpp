@@ -400,6 +400,7 @@ __device__ __forceinline__ T stag_buoy(T tdt, T c, T m) {
 template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
+  constexpr bool HOIST = sizeof(T) == 4;  // per-cell operands loaded ahead of the stores
   static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
   // one LDS object: the compiler's wait analysis then tells the rows the
   // direct loads fill (buffer b ^ 1) from the rows the stencil reads (buffer b)
@@ -503,11 +504,21 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
     if (t.active) {
       const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
-      T o3[3], m3[3];
+      // FP32: every per-cell operand is loaded before the first store (the
+      // stores could alias them as far as the compiler knows, and a load
+      // issued after a store waits for it); FP64 loads them where they are
+      // used, which holds fewer registers (2 waves per SIMD)
+      T o3[3], m3[3], ss3[3], al3[3], ar3[3], g3[3], d3[3];
+      auto load_ss = [&](int f) { return *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr; };
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
         o3[f] = ld_nt(a.o[f], i);
         if (!MED3 && MODE != MODE_BORN_ADJ) m3[f] = ld_nt(a.m[f], i);
+        if constexpr (HOIST) {
+          if (SS) ss3[f] = load_ss(f);
+          if (ACC) { al3[f] = *at(a.accl[f], i); ar3[f] = *at(a.accr[f], i); }
+          if (F & F_GRAD) { g3[f] = *at(a.grad[f], i); d3[f] = ld_nt(a.drv[f], i); }  // BX_/BY_/BZ_ == VX/VY/VZ
+        }
       }
       if (MED3) {
         const T rc = l_rho[b][t.ty + 1][t.tz + 1];
@@ -530,15 +541,20 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
         if (MODE == MODE_FWD) u = w * o3[f] + m3[f] * s3[f];
         else if (MODE == MODE_ADJ) u = w * o3[f] - m3[f] * s3[f];
         else u = w * o3[f] - s3[f];
-        if (SS) u += *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr;
+        if constexpr (!HOIST) {
+          if (SS) ss3[f] = load_ss(f);
+          if (ACC) { al3[f] = *at(a.accl[f], i); ar3[f] = *at(a.accr[f], i); }
+          if (F & F_GRAD) { g3[f] = *at(a.grad[f], i); d3[f] = ld_nt(a.drv[f], i); }
+        }
+        if (SS) u += ss3[f];
         u *= w;
         st_nt(a.o[f], i, u);
         if (MODE == MODE_BORN_ADJ_PM) st_nt(a.qn[f], i, u * m3[f]);
         if (ACC) {
-          *at(a.accl[f], i) += u * a.accfl;
-          *at(a.accr[f], i) += u * a.accfr;
+          *at(a.accl[f], i) = al3[f] + u * a.accfl;
+          *at(a.accr[f], i) = ar3[f] + u * a.accfr;
         }
-        if (F & F_GRAD) *at(a.grad[f], i) += u * ld_nt(a.drv[f], i);  // BX_/BY_/BZ_ == VX/VY/VZ
+        if (F & F_GRAD) *at(a.grad[f], i) = g3[f] + u * d3[f];
       }
     }
     if (MED3) rho_prev = l_rho[b][t.ty + 1][t.tz + 1];
@@ -574,6 +590,7 @@ struct StressStage {
 template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
+  constexpr bool HOIST = sizeof(T) == 4;  // per-cell operands loaded ahead of the stores
   static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
   __shared__ StressLds<T, BX, MED3> L;  // one LDS object (see k_velocity)
   auto& l_vx = L.vx; auto& l_vy = L.vy; auto& l_vz = L.vz; auto& l_mu = L.mu;
@@ -680,9 +697,24 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
     if (t.active) {
       const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
-      T o6[6];
+      // FP32: every per-cell operand is loaded before the first store (see k_velocity)
+      T o6[6], ss6[6], al6[6], ar6[6], g5[5], d6[6];
+      auto load_ss = [&](int f) { return *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr; };
+      auto load_grad = [&]() {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) o6[k] = ld_nt(a.o[SXX + k], i);
+        for (int k = 0; k < 6; ++k) d6[k] = ld_nt(a.drv[SXX + k], i);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) g5[m] = *at(a.grad[L2M + m], i);
+      };
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        o6[k] = ld_nt(a.o[SXX + k], i);
+        if constexpr (HOIST) {
+          if (SS) ss6[k] = load_ss(SXX + k);
+          if (ACC) { al6[k] = *at(a.accl[SXX + k], i); ar6[k] = *at(a.accr[SXX + k], i); }
+        }
+      }
+      if constexpr (HOIST && (F & F_GRAD) != 0) load_grad();
       T l2m = 0, lam = 0, mxz = 0, mxy = 0, myz = 0;
       if (MED3) {
         const int rr = t.ty + 1, qq = t.tz + 1;
@@ -731,14 +763,18 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         const int f = SXX + k;
+        if constexpr (!HOIST) {
+          if (SS) ss6[k] = load_ss(f);
+          if (ACC) { al6[k] = *at(a.accl[f], i); ar6[k] = *at(a.accr[f], i); }
+        }
         T v = u[k];
-        if (SS) v += *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr;
+        if (SS) v += ss6[k];
         v *= w;
         u[k] = v;
         st_nt(a.o[f], i, v);
         if (ACC) {
-          *at(a.accl[f], i) += v * a.accfl;
-          *at(a.accr[f], i) += v * a.accfr;
+          *at(a.accl[f], i) = al6[k] + v * a.accfl;
+          *at(a.accr[f], i) = ar6[k] + v * a.accfr;
         }
       }
       if (MODE == MODE_BORN_ADJ_PM) {
@@ -752,13 +788,14 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       if (F & F_GRAD) {
         // lamb2MuDtw multiplies exx, eyy, ezz in sxx, syy, szz; lambDtw the
         // other two normal strains; the shear moduli one shear strain each
-        const T dxx = ld_nt(a.drv[SXX], i), dyy = ld_nt(a.drv[SYY], i), dzz = ld_nt(a.drv[SZZ], i);
+        if constexpr (!HOIST) load_grad();
+        const T dxx = d6[0], dyy = d6[1], dzz = d6[2];
         const T l2 = u[0] * dxx + u[1] * dyy + u[2] * dzz;
-        *at(a.grad[L2M], i) += l2;
-        *at(a.grad[LAM], i) += (u[0] + u[1] + u[2]) * (dxx + dyy + dzz) - l2;
-        *at(a.grad[MXZ], i) += u[3] * ld_nt(a.drv[SXZ], i);
-        *at(a.grad[MXY], i) += u[4] * ld_nt(a.drv[SXY], i);
-        *at(a.grad[MYZ], i) += u[5] * ld_nt(a.drv[SYZ], i);
+        *at(a.grad[L2M], i) = g5[0] + l2;
+        *at(a.grad[LAM], i) = g5[1] + ((u[0] + u[1] + u[2]) * (dxx + dyy + dzz) - l2);
+        *at(a.grad[MXZ], i) = g5[2] + u[3] * d6[3];
+        *at(a.grad[MXY], i) = g5[3] + u[4] * d6[4];
+        *at(a.grad[MYZ], i) = g5[4] + u[5] * d6[5];
       }
     }
 #pragma unroll
