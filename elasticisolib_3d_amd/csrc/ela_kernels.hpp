@@ -1036,26 +1036,36 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
         else if (a.its == a.nts - 1) dv[k] = a.o[k][i];
         else dv[k] = a.o[k][i] - a.n[k][i];
       }
+      // all loads of the cell ahead of its stores (a load behind a store that
+      // might alias it waits for that store)
       if (!ADJ) {
+        T dm[8];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.ss[k][i] = a.dm[k][i] * dv[k] * a.dts_inv;
+        for (int k = 0; k < 8; ++k) dm[k] = a.dm[k][i];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.ss[k][i] = dm[k] * dv[k] * a.dts_inv;
         const T div = exx + eyy + ezz;
-        a.ss[SXX][i] = a.dm[3][i] * div + T(2.0) * a.dm[4][i] * exx;
-        a.ss[SYY][i] = a.dm[3][i] * div + T(2.0) * a.dm[4][i] * eyy;
-        a.ss[SZZ][i] = a.dm[3][i] * div + T(2.0) * a.dm[4][i] * ezz;
-        a.ss[SXZ][i] = a.dm[5][i] * gxz;
-        a.ss[SXY][i] = a.dm[6][i] * gxy;
+        a.ss[SXX][i] = dm[3] * div + T(2.0) * dm[4] * exx;
+        a.ss[SYY][i] = dm[3] * div + T(2.0) * dm[4] * eyy;
+        a.ss[SZZ][i] = dm[3] * div + T(2.0) * dm[4] * ezz;
+        a.ss[SXZ][i] = dm[5] * gxz;
+        a.ss[SXY][i] = dm[6] * gxy;
         // reference behaviour: the yz term uses the xz derivative pattern
-        a.ss[SYZ][i] = a.dm[7][i] * gxz;
+        a.ss[SYZ][i] = dm[7] * gxz;
       } else {
+        T ss[9], dmo[8];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.dmo[k][i] += a.ss[k][i] * dv[k] * a.dts_inv;
-        const T sxx = a.ss[SXX][i], syy = a.ss[SYY][i], szz = a.ss[SZZ][i];
-        a.dmo[3][i] += (sxx + syy + szz) * (exx + eyy + ezz);
-        a.dmo[4][i] += T(2.0) * (exx * sxx + eyy * syy + ezz * szz);
-        a.dmo[5][i] += a.ss[SXZ][i] * gxz;
-        a.dmo[6][i] += a.ss[SXY][i] * gxy;
-        a.dmo[7][i] += a.ss[SYZ][i] * gxz;
+        for (int k = 0; k < 9; ++k) ss[k] = a.ss[k][i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dmo[k] = a.dmo[k][i];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.dmo[k][i] = dmo[k] + ss[k] * dv[k] * a.dts_inv;
+        const T sxx = ss[SXX], syy = ss[SYY], szz = ss[SZZ];
+        a.dmo[3][i] = dmo[3] + (sxx + syy + szz) * (exx + eyy + ezz);
+        a.dmo[4][i] = dmo[4] + T(2.0) * (exx * sxx + eyy * syy + ezz * szz);
+        a.dmo[5][i] = dmo[5] + ss[SXZ] * gxz;
+        a.dmo[6][i] = dmo[6] + ss[SXY] * gxy;
+        a.dmo[7][i] = dmo[7] + ss[SYZ] * gxz;
       }
     }
 #pragma unroll
