@@ -130,6 +130,7 @@ struct ela_handle {
   DevBuf ckpt;              // Born adjoint checkpoints [segment][21][stride]
   DevBuf ss;                // Born secondary sources 2 x 9 arrays
   DevBuf dmod;              // Born perturbations / images, 8 arrays
+  DevBuf dmod64;            // FP32 handles: the Born adjoint's FP64 image accumulator, 8 arrays
   DevBuf pos;               // converted positions (int32) and an error flag
   DevBuf staging;           // device copies of host-memory arguments
   bool have_medium = false;
@@ -292,12 +293,17 @@ int launch_step(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
   }
 }
 
-// Forward and nonlinear-adjoint steps form the medium from (rho, lambda, mu)
-// when the handle holds them (ela_set_elastic): 3 medium reads per cell
-// instead of 8. The other modes read the 8 scaled arrays.
+// Forward, nonlinear-adjoint and premultiplied transposed steps form the
+// medium from (rho, lambda, mu) when the handle holds them (ela_set_elastic):
+// 3 medium reads per cell instead of 8. MODE_BORN_ADJ and the injections read
+// the 8 scaled arrays, which k_scale_medium derives with the steppers'
+// operations. The Born adjoint keeps its transposed step on the 8 arrays too:
+// formed in the step, its FP32 dot test against the Born forward rose from
+// 3e-7 to 0.5-1.3e-6 (the 1e-6 bar), while the gradient's transposed step,
+// judged at 1e-3 against FP64, gains 3% from the 3 reads.
 template <typename T, int MODE, int FV, int FS>
 int launch_step_m(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
-  if constexpr (MODE == ela::MODE_FWD || MODE == ela::MODE_ADJ) {
+  if constexpr (MODE == ela::MODE_FWD || MODE == ela::MODE_ADJ || MODE == ela::MODE_BORN_ADJ_PM) {
     if (h->have_elastic) return launch_step<T, MODE, FV | ela::F_MED3, FS | ela::F_MED3>(h, a, which);
   }
   return launch_step<T, MODE, FV, FS>(h, a, which);
@@ -728,7 +734,7 @@ int bg_coarse_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_
 // V[its+1] (null outside 0..nts-1).
 template <typename T>
 ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* Vo, const T* Vc, const T* Vn, int64_t its, const T* dm,
-                                 T* dmo, T* const ss[9]) {
+                                 double* dmo, T* const ss[9]) {
   ela::ImagingArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   const int64_t np = h->stride;
@@ -776,8 +782,9 @@ struct BornPlan {
 
 // Arrays (of h->stride elements) the Born adjoint needs besides V and the
 // checkpoints: medium 8, background wavefield 18, adjoint wavefield with its
-// premultiplied levels 36, secondary sources 18, images 8.
-constexpr int64_t kBornFixedArrays = 8 + 18 + 36 + 18 + 8;
+// premultiplied levels 36, secondary sources 18, images 8 (FP32 handles: and
+// their FP64 accumulator, 16 more).
+int64_t born_fixed_arrays(const ela_handle* h) { return 8 + 18 + 36 + 18 + 8 + (h->esize == 4 ? 16 : 0); }
 
 int born_plan(ela_handle* h, BornPlan& p) {
   const int64_t nts = h->cfg.nts, ncoarse = nts - 1;
@@ -800,10 +807,11 @@ int born_plan(ela_handle* h, BornPlan& p) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     budget = double(fr) + double(h->coarse.bytes) + double(h->ckpt.bytes) + double(h->wave2.bytes) +
-             double(h->ss.bytes) + double(h->dmod.bytes) + double(h->wave.bytes) + double(h->med.bytes) -
+             double(h->ss.bytes) + double(h->dmod.bytes) + double(h->dmod64.bytes) + double(h->wave.bytes) +
+             double(h->med.bytes) -
              double(2ull << 30);  // 2 GiB margin for traces and the runtime
   }
-  const double avail = budget / arr_b - double(kBornFixedArrays);
+  const double avail = budget / arr_b - double(born_fixed_arrays(h));
   if (double(3 * nts) <= avail) { p.full = true; p.slots = nts; return ELA_OK; }
   for (int64_t K = ncoarse; K >= 1; --K) {
     const int64_t S = (ncoarse + K - 1) / K;
@@ -865,7 +873,7 @@ int born_forward_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
 
 template <typename T>
 int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], const DevPoints& rp, T* tm,
-                     int64_t nrec_tot, T* dm) {
+                     int64_t nrec_tot, double* dm) {
   const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1, np = h->stride;
   BornPlan plan;
   CHK(born_plan(h, plan));
@@ -928,7 +936,7 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
       set_pm<T>(h, a, ad);
       for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
       a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-      CHK((launch_step<T, ela::MODE_BORN_ADJ_PM, ela::F_ACC, ela::F_ACC>(h, a)));
+      CHK((launch_step<T, ela::MODE_BORN_ADJ_PM, ela::F_ACC, ela::F_ACC>(h, a)));  // 8 arrays: see launch_step_m
       ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm).v, true, nrec_tot);
       set_pm<T>(h, R, ad);
       for (int k = 0; k < 9; ++k) { R.accl[k] = ssL[k]; R.accr[k] = ssR[k]; }
@@ -1004,10 +1012,27 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
       off += n;
     }
   }
+  // the adjoint sums its image over every coarse sample in FP64 (FP32
+  // handles: in an FP64 copy of the incoming image, narrowed at the end)
+  double* dm64 = nullptr;
+  if (adjoint) {
+    if constexpr (sizeof(T) == 8) {
+      dm64 = reinterpret_cast<double*>(dm);
+    } else {
+      CHK(h->dmod64.alloc(8 * np * sizeof(double)));
+      dm64 = (double*)h->dmod64.p;
+      CHK(fill<double>(h, dm64, 8 * np));
+      hipLaunchKernelGGL((ela::k_fold<T>), dim3((unsigned)std::min<int64_t>((8 * np + 255) / 256, 65536)), dim3(256),
+                         0, h->stream, dm, dm64, 8 * np);
+    }
+  }
   HIPCHK(hipEventRecord(h->ev0, h->stream));
   if (!adjoint) CHK(born_forward_run<T>(h, sp, src_tr, rp, tm, nrec_tot, dm));
-  else CHK(born_adjoint_run<T>(h, sp, src_tr, rp, tm, nrec_tot, dm));
+  else CHK(born_adjoint_run<T>(h, sp, src_tr, rp, tm, nrec_tot, dm64));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
+  if (adjoint && sizeof(T) == 4)
+    hipLaunchKernelGGL((ela::k_narrow<T>), dim3((unsigned)std::min<int64_t>((8 * np + 255) / 256, 65536)), dim3(256),
+                       0, h->stream, dm64, dm, 8 * np);
   HIPCHK(hipGetLastError());
 
   if (!adjoint) {
@@ -1312,7 +1337,7 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
     set_pm<T>(h, a, ad);
     for (int c = 0; c < 9; ++c) a.drv[c] = d + c * np;
     for (int m = 0; m < 8; ++m) a.grad[m] = g32 + m * np;
-    CHK((launch_step<T, ela::MODE_BORN_ADJ_PM, ela::F_GRAD, ela::F_GRAD>(h, a)));
+    CHK((launch_step_m<T, ela::MODE_BORN_ADJ_PM, ela::F_GRAD, ela::F_GRAD>(h, a)));
     ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm_adj).v, true, nrec_tot);
     set_pm<T>(h, R, ad);
     for (int c = 0; c < 9; ++c) R.drv[c] = a.drv[c];

@@ -401,7 +401,8 @@ template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
   constexpr bool HOIST = sizeof(T) == 4;  // per-cell operands loaded ahead of the stores
-  static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
+  static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ || MODE == MODE_BORN_ADJ_PM,
+                "F_MED3 needs MODE_FWD, MODE_ADJ or MODE_BORN_ADJ_PM");
   // one LDS object: the compiler's wait analysis then tells the rows the
   // direct loads fill (buffer b ^ 1) from the rows the stencil reads (buffer b)
   __shared__ VelLds<T, BX, MED3> L;
@@ -591,7 +592,8 @@ template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
   constexpr bool HOIST = sizeof(T) == 4;  // per-cell operands loaded ahead of the stores
-  static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ, "F_MED3 needs MODE_FWD or MODE_ADJ");
+  static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ || MODE == MODE_BORN_ADJ_PM,
+                "F_MED3 needs MODE_FWD, MODE_ADJ or MODE_BORN_ADJ_PM");
   __shared__ StressLds<T, BX, MED3> L;  // one LDS object (see k_velocity)
   auto& l_vx = L.vx; auto& l_vy = L.vy; auto& l_vz = L.vz; auto& l_mu = L.mu;
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
@@ -882,39 +884,48 @@ __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   const int32_t p = a.pos[c][k];
   if (p < 0) return;
   const int its = a.its_dev ? *a.its_dev : a.its;
-  const T* tr = a.tr[c] + k * a.tr_stride_pt;
-  const T s = tr[(int64_t)its * a.tr_stride_t] * a.fa + tr[(int64_t)(its + 1) * a.tr_stride_t] * a.fb;
-  const T v = point_taper(a, p) * s;
+  const T taper = point_taper(a, p);
+  // the injected value of component cc at this point (the three normal
+  // stresses share their point list, so index k is the same point for them)
+  auto injected = [&](int cc) {
+    const T* tr = a.tr[cc] + k * a.tr_stride_pt;
+    return taper * (tr[(int64_t)its * a.tr_stride_t] * a.fa + tr[(int64_t)(its + 1) * a.tr_stride_t] * a.fb);
+  };
+  const T v = injected(c);
   a.field[c][p] += v;
   if (ACC && a.accl[c]) {
     a.accl[c][p] += v * a.accfl;
     a.accr[c][p] += v * a.accfr;
   }
+  // Points are unique within a component's list, so every update below has
+  // one writer; the normal stresses, coupled through lamb2MuDtw / lambDtw,
+  // are updated together by the sxx thread, with the stepper's formulas.
+  T vn[3] = {T(0), T(0), T(0)};
+  if ((PM || GRAD) && c == SXX) { vn[0] = v; vn[1] = injected(SYY); vn[2] = injected(SZZ); }
   if (PM) {
-    // keep q = M n: the normal stresses share lamb2MuDtw / lambDtw (atomics)
+    // keep q = M n
     if (c < 3) {
-      atomicAdd(&a.q[c][p], v * a.m[c][p]);
-    } else if (c < 6) {
+      a.q[c][p] += v * a.m[c][p];
+    } else if (c == SXX) {
       const T l2 = a.m[L2M][p], la = a.m[LAM][p];
-      atomicAdd(&a.q[SXX][p], v * (c == SXX ? l2 : la));
-      atomicAdd(&a.q[SYY][p], v * (c == SYY ? l2 : la));
-      atomicAdd(&a.q[SZZ][p], v * (c == SZZ ? l2 : la));
-    } else {
-      atomicAdd(&a.q[c][p], v * a.m[MXZ + (c - SXZ)][p]);
+      a.q[SXX][p] += vn[0] * l2 + (vn[1] + vn[2]) * la;
+      a.q[SYY][p] += vn[1] * l2 + (vn[0] + vn[2]) * la;
+      a.q[SZZ][p] += vn[2] * l2 + (vn[0] + vn[1]) * la;
+    } else if (c >= SXZ) {
+      a.q[c][p] += v * a.m[MXZ + (c - SXZ)][p];
     }
   }
   if (GRAD) {
-    // the stepper imaged the level before this injection; add the injected
-    // part (the three normal stresses share lamb2MuDtw / lambDtw: atomics)
+    // the stepper imaged the level before this injection; add the injected part
     if (c < 3) {
-      atomicAdd(&a.grad[c][p], v * a.drv[c][p]);
-    } else if (c < 6) {
+      a.grad[c][p] += v * a.drv[c][p];
+    } else if (c == SXX) {
       const T dxx = a.drv[SXX][p], dyy = a.drv[SYY][p], dzz = a.drv[SZZ][p];
-      const T own = c == SXX ? dxx : (c == SYY ? dyy : dzz);
-      atomicAdd(&a.grad[L2M][p], v * own);
-      atomicAdd(&a.grad[LAM][p], v * (dxx + dyy + dzz - own));
-    } else {
-      atomicAdd(&a.grad[MXZ + (c - SXZ)][p], v * a.drv[c][p]);
+      const T l2 = vn[0] * dxx + vn[1] * dyy + vn[2] * dzz;
+      a.grad[L2M][p] += l2;
+      a.grad[LAM][p] += (vn[0] + vn[1] + vn[2]) * (dxx + dyy + dzz) - l2;
+    } else if (c >= SXZ) {
+      a.grad[MXZ + (c - SXZ)][p] += v * a.drv[c][p];
     }
   }
 }
@@ -964,7 +975,7 @@ struct ImagingArgs {
   const T* c[3];
   const T* n[3];
   const T* dm[8];   // scaled perturbations (drhox drhoy drhoz dlame dmu dmuxz dmuxy dmuyz)
-  T* dmo[8];        // adjoint: accumulated images
+  double* dmo[8];   // adjoint: accumulated images (FP64: summed over every coarse sample)
   T* ss[9];         // forward: secondary sources out; adjoint: adjoint coarse slices in
   T cx[4], cy[4], cz[4];
   T dts_inv;
@@ -1053,19 +1064,21 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
         // reference behaviour: the yz term uses the xz derivative pattern
         a.ss[SYZ][i] = dm[7] * gxz;
       } else {
-        T ss[9], dmo[8];
+        // each coarse sample's term in T, the sum over samples in FP64
+        T ss[9];
+        double dmo[8];
 #pragma unroll
         for (int k = 0; k < 9; ++k) ss[k] = a.ss[k][i];
 #pragma unroll
         for (int k = 0; k < 8; ++k) dmo[k] = a.dmo[k][i];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.dmo[k][i] = dmo[k] + ss[k] * dv[k] * a.dts_inv;
+        for (int k = 0; k < 3; ++k) a.dmo[k][i] = dmo[k] + double(ss[k] * dv[k] * a.dts_inv);
         const T sxx = ss[SXX], syy = ss[SYY], szz = ss[SZZ];
-        a.dmo[3][i] = dmo[3] + (sxx + syy + szz) * (exx + eyy + ezz);
-        a.dmo[4][i] = dmo[4] + T(2.0) * (exx * sxx + eyy * syy + ezz * szz);
-        a.dmo[5][i] = dmo[5] + ss[SXZ] * gxz;
-        a.dmo[6][i] = dmo[6] + ss[SXY] * gxy;
-        a.dmo[7][i] = dmo[7] + ss[SYZ] * gxz;
+        a.dmo[3][i] = dmo[3] + double((sxx + syy + szz) * (exx + eyy + ezz));
+        a.dmo[4][i] = dmo[4] + double(T(2.0) * (exx * sxx + eyy * syy + ezz * szz));
+        a.dmo[5][i] = dmo[5] + double(ss[SXZ] * gxz);
+        a.dmo[6][i] = dmo[6] + double(ss[SXY] * gxy);
+        a.dmo[7][i] = dmo[7] + double(ss[SYZ] * gxz);
       }
     }
 #pragma unroll
@@ -1307,6 +1320,13 @@ __global__ void __launch_bounds__(256) k_ck_unpack_s16(CkArgs<T> a, const int16_
   const float* sc = reinterpret_cast<const float*>(in + 18 * a.nI) + f * nb;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.nI; q += (int64_t)gridDim.x * blockDim.x)
     a.f[f][ck_offset(a, q)] = (T)((float)q16[q] * sc[q / CK_BLOCK]);
+}
+
+// out[i] = (T) in[i] (the Born adjoint's FP64 image handed back in T).
+template <typename T>
+__global__ void __launch_bounds__(256) k_narrow(const double* in, T* out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = T(in[i]);
 }
 
 // acc[i] += (double) part[i]; part[i] = 0 (FP64 time accumulation of the
