@@ -1136,12 +1136,15 @@ struct CkArgs {
   int64_t nI;            // nzI * nxI * nyI
 };
 
+// Library offset of packed interior index q. 32-bit division: nI and every
+// offset fit in 32 bits (arrays < 4 GiB), and a 64-bit division costs the
+// pack / unpack kernels more than their memory traffic.
 template <typename T>
 __device__ __forceinline__ int64_t ck_offset(const CkArgs<T>& a, int64_t q) {
-  const int64_t iz = q % a.nzI;
-  const int64_t r = q / a.nzI;
-  const int64_t ix = r % a.nxI, iy = r / a.nxI;
-  return (iy + FAT) * a.ys + (ix + FAT) * a.nzi + a.zoff + FAT + iz;
+  const uint32_t q32 = (uint32_t)q, nz = (uint32_t)a.nzI, nx = (uint32_t)a.nxI;
+  const uint32_t r = q32 / nz, iz = q32 - r * nz;
+  const uint32_t iy = r / nx, ix = r - iy * nx;
+  return (int64_t)((iy + FAT) * (uint32_t)a.ys + (ix + FAT) * (uint32_t)a.nzi + (uint32_t)(a.zoff + FAT) + iz);
 }
 
 __device__ __forceinline__ uint16_t f32_to_bf16(float x) {
