@@ -218,3 +218,42 @@ def test_positions_follow_the_memory_kind_of_the_call():
     assert all(isinstance(a, np.ndarray) and a.dtype == np.int64 for a in prop._keep)
     first = np.ctypeslib.as_array(ctypes.cast(p.pos[0], ctypes.POINTER(ctypes.c_int64)), shape=(3,))
     assert list(first) == [0, 1, 2]
+
+
+def _kernel_constant(name):
+    src = open(os.path.join(os.path.dirname(__file__), "..", "elasticisolib_3d_amd", "csrc", "ela_kernels.hpp")).read()
+    m = re.search(r"constexpr int %s = ([^;]+);" % name, src)
+    expr = m.group(1).split("//")[0].strip()
+    return eval(expr, {"TZ": 64})   # the expressions are integer literals or multiples of TZ
+
+
+@pytest.mark.parametrize("esize", [4, 8])
+def test_stencil_loads_stay_inside_the_guard(esize):
+    """The stencils load their neighbours without bounds predicates; the
+    farthest element any load reaches (last plane, last x tile plus its high x
+    halo, last z tile plus its high z halo) must lie inside the array's
+    stride, which carries GUARD_ROWS rows + GUARD_EXTRA elements past the
+    grid (the stride rule of ela_create). Grids with partial tiles in x and z,
+    every tile width, FP32 and FP64 row pitches."""
+    FAT, TZ = 4, 64
+    guard_rows, guard_extra = _kernel_constant("GUARD_ROWS"), _kernel_constant("GUARD_EXTRA")
+    assert guard_rows >= 16   # the widest tile
+    line = 128 // esize
+    rup = lambda v, m: (v + m - 1) // m * m   # noqa: E731
+    rng = np.random.default_rng(0)
+    for nz, nx, ny in [(29, 31, 45), (81, 21, 19), (328, 472, 468), (232, 328, 324)] + \
+            [tuple(rng.integers(9, 400, 3)) for _ in range(40)]:
+        zoff = line - FAT
+        nzi = rup(zoff + nz, line)
+        n_int = nzi * nx * ny
+        stride = rup(n_int + guard_rows * nzi + guard_extra, 64)
+        for bx in (4, 8, 16):
+            nbz = (nz - 2 * FAT + TZ - 1) // TZ
+            nbx = (nx - 2 * FAT + bx - 1) // bx
+            z0 = zoff + FAT + (nbz - 1) * TZ        # last z tile
+            x0 = FAT + (nbx - 1) * bx               # last x tile
+            iz_max = max(z0 + TZ - 1, z0 + TZ + FAT - 1)          # own column / high z halo
+            ix_max = max(x0 + bx - 1, x0 + bx + FAT - 1)          # own column / high x halo
+            last_plane = ny - 1                     # queue loads reach y1 + 3 <= ny - 1
+            far = last_plane * nx * nzi + ix_max * nzi + iz_max
+            assert far < stride, (nz, nx, ny, bx, esize, far - stride)
