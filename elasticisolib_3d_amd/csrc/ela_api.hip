@@ -168,6 +168,10 @@ struct ela_handle {
   bool use_graph = false;
   DevBuf counter;           // device coarse-step counter (one int)
   hipStream_t cap_stream = nullptr;
+  // the last run's executable graph, destroyed once its launches have run
+  // (nl_exec_done is recorded after them)
+  hipGraphExec_t nl_exec = nullptr;
+  hipEvent_t nl_exec_done = nullptr;
 };
 
 namespace {
@@ -233,6 +237,16 @@ int ensure_chunks(ela_handle* h) {
   HIPCHK(hipMemcpy(h->chunks.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
   h->nchunks = (int)(tab.size() / 8);
   h->chunks_yc = h->ychunk;
+  return ELA_OK;
+}
+
+// Destroy the previous run's executable graph after its in-flight launches
+// have completed (the launches may still reference it until they run).
+int release_exec(ela_handle* h) {
+  if (!h->nl_exec) return ELA_OK;
+  HIPCHK(hipEventSynchronize(h->nl_exec_done));
+  HIPCHK(hipGraphExecDestroy(h->nl_exec));
+  h->nl_exec = nullptr;
   return ELA_OK;
 }
 
@@ -635,6 +649,8 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
   const bool graph = h->use_graph && sub % 2 == 0;
   hipGraphExec_t exec = nullptr;
   if (graph) {
+    CHK(release_exec(h));
+    if (!h->nl_exec_done) HIPCHK(hipEventCreateWithFlags(&h->nl_exec_done, hipEventDisableTiming));
     CHK(ensure_chunks(h));  // no synchronous upload inside the capture
     CHK(h->counter.alloc(256));
     int* cnt = (int*)h->counter.p;
@@ -659,8 +675,11 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
   if (graph) {
     hipError_t e = hipSuccess;
     for (int64_t k = 0; k < nts - 1 && e == hipSuccess; ++k) e = hipGraphLaunch(exec, h->stream);
-    (void)hipGraphExecDestroy(exec);
+    // keep the executable graph until its launches have run (release_exec)
+    h->nl_exec = exec;
+    const hipError_t er = hipEventRecord(h->nl_exec_done, h->stream);
     if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
+    if (er != hipSuccess) return fail(ELA_ERR_HIP, "event record failed: %s", hipGetErrorString(er));
   } else if (!adjoint) {
     for (int64_t its = 0; its < nts - 1; ++its) CHK(coarse_step(its, nullptr));
   } else {
@@ -1528,6 +1547,8 @@ int ela_destroy(ela_handle* h) {
   if (!h) return ELA_OK;
   DeviceGuard dg(h->cfg.device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  (void)release_exec(h);
+  if (h->nl_exec_done) (void)hipEventDestroy(h->nl_exec_done);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
