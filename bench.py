@@ -225,7 +225,7 @@ def bench_fp64_check(args):
     res = {}
     shapes = {}
     for prec in ("f64", "f32"):
-        prop, spos, src, rpos, rec = gpu_setup(w, prec)
+        prop, spos, src, rpos, rec = gpu_setup(w, prec, args.medium)
         if not args.no_autotune:
             from elasticisolib_3d_amd.autotune import autotune
             t = autotune(prop, reps=5)
@@ -260,6 +260,7 @@ def bench_fp64_check(args):
         "config": {
             "workload": "nonlinear forward, 400x400x250 layered model (padded 328x472x468), "
                         f"{w.nts} coarse samples x {w.sub} fine steps, 200x200 receivers, 9 components",
+            "medium": args.medium,
             "per_component_rel_err": [round(e, 9) for e in per],
             "f64_gcell_updates_per_s": round(cells / res["f64"][1] / 1e9, 3),
             "f32_gcell_updates_per_s": round(cells / res["f32"][1] / 1e9, 3),
@@ -484,7 +485,7 @@ def main():
                                                  "e.g. 4x8/3/4 (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
     ap.add_argument("--medium", choices=("elastic", "scaled"), default="elastic",
-                    help="nl_forward, born, fwi: rho, lambda, mu with the medium formed in the steppers (120 B per cell-update), "
+                    help="nl_forward, born, fwi, fp64_check: rho, lambda, mu with the medium formed in the steppers (120 B per cell-update), "
                          "or the 8 scaled arrays of the reference's interface (140 B)")
     ap.add_argument("--no-stream-ref", action="store_true",
                     help="skip the streaming-kernel reference of the roofline")

@@ -269,6 +269,7 @@ ela::StepArgs<T> base_args(ela_handle* h, T* o[9], T* c[9]) {
   if (h->have_elastic) {
     for (int k = 0; k < 3; ++k) a.ep[k] = arr<T>(h->elastic, h->stride, k);
     a.tdt = (T)h->tdt;
+    a.tdt64 = h->tdt;
   }
   set_coefs<T>(h, a.cx, a.cy, a.cz);
   a.taper = reinterpret_cast<const T*>(h->taper.p);
@@ -1461,6 +1462,7 @@ int set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem) {
   a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
   a.ys = h->cfg.nx * h->nzi;
   a.tdt = (T)(2.0 * dtw);
+  a.tdt64 = 2.0 * dtw;
   const int64_t n = h->cfg.nz * h->cfg.nx * h->cfg.ny;
   hipLaunchKernelGGL((ela::k_scale_medium<T>), dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20)), dim3(256),
                      0, h->stream, a);

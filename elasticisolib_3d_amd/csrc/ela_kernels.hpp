@@ -68,6 +68,7 @@ struct StepArgs {
   T* qn[9];             // MODE_BORN_ADJ_PM: M n of the new level (written)
   const T* ep[3];       // F_MED3: rho, lambda, mu
   T tdt;                // F_MED3: 2 dtw
+  double tdt64;         // F_MED3: 2 dtw in FP64 (FP32 steppers form buoyancies and normal moduli in FP64)
   T ssfl, ssfr;         // filter weights for the secondary source
   T accfl, accfr;       // filter weights for the accumulation
   T cx[4], cy[4], cz[4];
@@ -379,22 +380,38 @@ struct VelStage {
 // = Sy(Sx(mu)), muyz = Sz(Sy(mu)) -- the operations of ela_scale_medium in the
 // same order, so an FP64 stepper reproduces the 8-array stepper exactly.
 //
-// buoy: 2 dtw / rho_h for a staggered density rho_h; FP32 uses one hardware
-// reciprocal (1 ulp) instead of the correctly rounded division, in the
-// steppers and in k_scale_medium alike, so that every FP32 mode of one handle
-// sees the same buoyancies.
+// Buoyancy 2 dtw / rho_h of a staggered density rho_h. FP64: the host
+// scaling's division. FP32: formed in FP64 from the FP32 densities (hardware
+// FP64 reciprocal, far inside FP32 rounding) and rounded once, as the host
+// rounds its FP64 scaling -- formed in FP32 arithmetic, the buoyancies and
+// moduli carried one or two extra roundings whose bias more than doubled the
+// FP32 data's distance from FP64 over a 5000-step run. k_scale_medium uses the
+// same functions, so every mode of one handle sees the same medium.
 template <typename T>
-__device__ __forceinline__ T buoy(T tdt, T rho_h) {
+__device__ __forceinline__ T buoy_h(T tdt, double tdt64, double rho_h) {
   if constexpr (sizeof(T) == 4) {
-    return tdt * __builtin_amdgcn_rcpf(rho_h);
+    return T(tdt64 * __builtin_amdgcn_rcp(rho_h));
   } else {
-    return tdt / rho_h;
+    return tdt / T(rho_h);
   }
 }
 
 template <typename T>
-__device__ __forceinline__ T stag_buoy(T tdt, T c, T m) {
-  return buoy(tdt, T(0.5) * (c + m));
+__device__ __forceinline__ T stag_buoy(const StepArgs<T>& a, T c, T m) {
+  if constexpr (sizeof(T) == 4) return buoy_h<T>(a.tdt, a.tdt64, 0.5 * (double(c) + double(m)));
+  else return a.tdt / (T(0.5) * (c + m));
+}
+
+// lamb2MuDtw and lambDtw of a cell (FP32: formed in FP64, rounded once).
+template <typename T>
+__device__ __forceinline__ void normal_moduli(T tdt, double tdt64, T lam, T mu, T& l2m, T& la) {
+  if constexpr (sizeof(T) == 4) {
+    l2m = T(tdt64 * (double(lam) + 2.0 * double(mu)));
+    la = T(tdt64 * double(lam));
+  } else {
+    l2m = tdt * (lam + T(2) * mu);
+    la = tdt * lam;
+  }
 }
 
 template <typename T, int BX, int MODE, int F>
@@ -523,9 +540,9 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
       }
       if (MED3) {
         const T rc = l_rho[b][t.ty + 1][t.tz + 1];
-        m3[0] = stag_buoy(a.tdt, rc, l_rho[b][t.ty][t.tz + 1]);
-        m3[1] = stag_buoy(a.tdt, rc, rho_prev);
-        m3[2] = stag_buoy(a.tdt, rc, l_rho[b][t.ty + 1][t.tz]);
+        m3[0] = stag_buoy(a, rc, l_rho[b][t.ty][t.tz + 1]);
+        m3[1] = stag_buoy(a, rc, rho_prev);
+        m3[2] = stag_buoy(a, rc, l_rho[b][t.ty + 1][t.tz]);
       }
       const T sx = dmx<T, BX>(l_xx[b], r, q, cx) + dq(qxy, cy) + dpz<T, BX>(l_xz[b], r, q, cz);
       const T sy = dpx<T, BX>(l_xy[b], r, q, cx) + dq(qyy, cy) + dpz<T, BX>(l_yz[b], r, q, cz);
@@ -723,8 +740,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
         const T mu_c = l_mu[b][rr][qq], mu_xm = l_mu[b][rr - 1][qq], mu_zm = l_mu[b][rr][qq - 1];
         const T mu_xzm = l_mu[b][rr - 1][qq - 1];
         const T lr = ld_nt(a.ep[1], i);
-        l2m = a.tdt * (lr + T(2) * mu_c);
-        lam = a.tdt * lr;
+        normal_moduli(a.tdt, a.tdt64, lr, mu_c, l2m, lam);
         const T sx_c = T(0.5) * (mu_c + mu_xm), sx_zm = T(0.5) * (mu_zm + mu_xzm);
         const T sy_c = T(0.5) * (mu_c + mu_prev), sy_zm = T(0.5) * (mu_zm + muz_prev);
         mxz = (T(0.5) * (sx_c + sx_zm)) * a.tdt;
@@ -1212,7 +1228,8 @@ __global__ void __launch_bounds__(256) k_ck_unpack(CkArgs<T> a, const void* in, 
 // padded grid with the host scaling's edge rules (the half-cell average
 // halves the first sample and adds half the neighbour to the last one): the
 // device twin of ela_scale_medium, same operations in the same order (FP32
-// buoyancies with the steppers' reciprocal, see buoy).
+// buoyancies and normal moduli formed in FP64 as in the steppers: buoy_h,
+// normal_moduli).
 template <typename T>
 struct MedArgs {
   const T* ep[3];        // rho, lambda, mu (internal layout)
@@ -1220,11 +1237,18 @@ struct MedArgs {
   int nz, nx, ny, nzi, zoff;
   int64_t ys;
   T tdt;
+  double tdt64;
 };
 
 template <typename T>
 __device__ __forceinline__ T half_cell(const T* f, int64_t i, int64_t s, int ia, int n) {
   return ia == 0 ? T(0.5) * f[i] : (ia < n - 1 ? T(0.5) * (f[i] + f[i - s]) : f[i] + T(0.5) * f[i - s]);
+}
+// the same in FP64 from the array's values (the FP32 buoyancies)
+template <typename T>
+__device__ __forceinline__ double half_cell64(const T* f, int64_t i, int64_t s, int ia, int n) {
+  const double c = double(f[i]);
+  return ia == 0 ? 0.5 * c : (ia < n - 1 ? 0.5 * (c + double(f[i - s])) : c + 0.5 * double(f[i - s]));
 }
 
 template <typename T>
@@ -1239,11 +1263,16 @@ __global__ void __launch_bounds__(256) k_scale_medium(MedArgs<T> a) {
     const T* rho = a.ep[0];
     const T* lam = a.ep[1];
     const T* mu = a.ep[2];
-    a.m[BX_][i] = buoy(a.tdt, half_cell(rho, i, sx, ix, a.nx));
-    a.m[BY_][i] = buoy(a.tdt, half_cell(rho, i, sy, iy, a.ny));
-    a.m[BZ_][i] = buoy(a.tdt, half_cell(rho, i, (int64_t)1, iz, a.nz));
-    a.m[L2M][i] = a.tdt * (lam[i] + T(2) * mu[i]);
-    a.m[LAM][i] = a.tdt * lam[i];
+    if constexpr (sizeof(T) == 4) {
+      a.m[BX_][i] = buoy_h<T>(a.tdt, a.tdt64, half_cell64(rho, i, sx, ix, a.nx));
+      a.m[BY_][i] = buoy_h<T>(a.tdt, a.tdt64, half_cell64(rho, i, sy, iy, a.ny));
+      a.m[BZ_][i] = buoy_h<T>(a.tdt, a.tdt64, half_cell64(rho, i, (int64_t)1, iz, a.nz));
+    } else {
+      a.m[BX_][i] = a.tdt / half_cell(rho, i, sx, ix, a.nx);
+      a.m[BY_][i] = a.tdt / half_cell(rho, i, sy, iy, a.ny);
+      a.m[BZ_][i] = a.tdt / half_cell(rho, i, (int64_t)1, iz, a.nz);
+    }
+    normal_moduli(a.tdt, a.tdt64, lam[i], mu[i], a.m[L2M][i], a.m[LAM][i]);
     // mu staggered along x, then along z (muxz) or y (muxy); along y, then z (muyz)
     const T tx = half_cell(mu, i, sx, ix, a.nx);
     const T tx_z = iz > 0 ? half_cell(mu, i - 1, sx, ix, a.nx) : T(0);
