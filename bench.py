@@ -264,7 +264,9 @@ def bench_fp64_check(args):
             "per_component_rel_err": [round(e, 9) for e in per],
             "f64_gcell_updates_per_s": round(cells / res["f64"][1] / 1e9, 3),
             "f32_gcell_updates_per_s": round(cells / res["f32"][1] / 1e9, 3),
-            "f64_GBs_algorithmic": round(cells * 280 / res["f64"][1] / 1e9, 1),
+            "f64_bytes_per_cell": 2 * (BYTES_F32[args.medium]["vel"] + BYTES_F32[args.medium]["stress"]),
+            "f64_GBs_algorithmic": round(cells * 2 * (BYTES_F32[args.medium]["vel"] + BYTES_F32[args.medium]["stress"])
+                                         / res["f64"][1] / 1e9, 1),
             "launch": shapes or None,
         },
     }
