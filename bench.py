@@ -445,6 +445,24 @@ def bench_fwi_batch(args):
     gp, gb = res["plain"][1], res["batched"][1]
     diff = float(np.linalg.norm(gb - gp) / np.linalg.norm(gp))
     tp, tb = res["plain"][2]["total_s"], res["batched"][2]["total_s"]
+    full = {}
+    if args.full_shots > 0:
+        # the reference's layout: every shot propagated over the whole padded model, one shot per
+        # launch sequence (timing only: the observed traces are the windowed ones)
+        fshots = [BatchShot(s.src_xyz, s.rec_xyz, Window(0, 0, shape[1], shape[0]), s.signals, s.observed)
+                  for s in shots[:args.full_shots]]
+        op = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], fshots,
+                        batch=1, **kw)
+        op.misfit_and_gradient(cfg["start"])
+        tf = op.timing["total_s"]
+        log(f"[bench] full grid (batch 1, {len(fshots)} shots): {op.timing}")
+        full = {"full_grid_shots": len(fshots),
+                "full_grid_shots_per_s": round(len(fshots) / tf, 4),
+                "full_grid_s": round(tf, 2),
+                "full_grid_propagation_s": round(op.timing["propagation_s"], 2),
+                "speedup_batched_over_full_grid": round((tf / len(fshots)) / (tb / len(shots)), 3)}
+        del op
+        torch.cuda.empty_cache()
     wins = [s.window for s in shots]
     out = {
         "metric": "FWI-gradient shots/s, batched vs plain (shot-local windows)",
@@ -469,6 +487,7 @@ def bench_fwi_batch(args):
             "plain_propagation_s": round(res["plain"][2]["propagation_s"], 2),
             "batched_propagation_s": round(res["batched"][2]["propagation_s"], 2),
             "gradient_rel_diff_batched_vs_plain": diff,
+            **full,
         },
     }
     print(json.dumps(out), flush=True)
@@ -500,6 +519,9 @@ def main():
                          "and gradient accuracy")
     ap.add_argument("--shots", type=int, default=4, help="fwi_batch: number of shots")
     ap.add_argument("--batch", type=int, default=2, help="fwi_batch: shots per batched launch sequence")
+    ap.add_argument("--full-shots", type=int, default=1,
+                    help="fwi_batch: shots also timed over the whole padded model, one per launch sequence "
+                         "(the reference's layout); 0 skips them")
     ap.add_argument("--fd-cells", type=int, default=16,
                     help="fwi: cells checked by central differences (-1: skip the accuracy part)")
     ap.add_argument("--checkpoint", choices=("bf16", "s16", "native"), default="bf16",
