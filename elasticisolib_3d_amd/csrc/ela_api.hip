@@ -1752,7 +1752,8 @@ int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, 
   else if (reads == 14 && writes == 6) kern = ela::k_stream<14, 6>;
   else if (reads == 10 && writes == 3) kern = ela::k_stream<10, 3>;
   else if (reads == 11 && writes == 6) kern = ela::k_stream<11, 6>;
-  if (!kern) return fail(ELA_ERR_ARG, "stream probe mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6)");
+  else if (reads == 12 && writes == 9) kern = ela::k_stream<12, 9>;
+  if (!kern) return fail(ELA_ERR_ARG, "stream probe mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9)");
   DeviceGuard dg(h->cfg.device);
   DevBuf buf, tab;
   CHK(buf.alloc((size_t)reads * n * 4));

@@ -337,7 +337,10 @@ int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* 
  * (14, 6) kernels without neighbour accesses, or (10, 3) and (11, 6) when they
  * form the medium from (rho, lambda, mu) -- timed over `reps` launches on the
  * handle's stream; *gbs = (reads + writes) * 4 * n bytes per launch / time.
- * Mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6); n a multiple of 4.
+ * Mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9); n a multiple of 4.
+ * (12, 9) is the mix of a step that advances one time sub-lattice of the
+ * leapfrog by a velocity and a stress half-step in one pass (v_old, stress,
+ * rho, lambda, mu read; velocity and stress rewritten).
  * Allocates and frees reads * n * 4 bytes. No counterpart in the reference. */
 int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, double* gbs);
 

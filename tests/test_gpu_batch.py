@@ -76,8 +76,7 @@ def test_batched_media_kinds_agree_in_fp64():
     scaled arrays give bit-identical FP64 data and misfits: the steppers form
     the same values in the same order, and the windows' edge planes, where the
     two stackings differ, lie outside every interior stencil. The gradients
-    agree to 1e-12: the injected residual's imaging adds into shared cells
-    with atomics, whose order varies from run to run."""
+    are held to 1e-12."""
     from elasticisolib_3d_amd.batch import BatchedFwi
     m = _model()
     start = [x * (1 + 0.02 * np.random.default_rng(5 + k).standard_normal(x.shape)) for k, x in enumerate(m)]
@@ -95,4 +94,51 @@ def test_batched_media_kinds_agree_in_fp64():
     assert Ja == Jb
     e = float(np.linalg.norm(ga - gb) / np.linalg.norm(gb))
     print(f"stacked (rho, lambda, mu) vs stacked 8 arrays, FP64 gradient: rel {e:.2e}", flush=True)
+    assert e < 1e-12
+
+
+@pytest.mark.parametrize("medium", ["elastic", "scaled"])
+def test_whole_model_window_equals_plain_operator(medium):
+    """A window covering the whole model reproduces the plain one-grid FWI
+    operator (FwiProblem, no window table) on the same padded model: the same
+    observed data, misfit and gradient (mapped back through the window's
+    padding) in FP64. The two paths interpolate traces in different orders
+    (host numpy against device index_add), hence 1e-12 rather than bit
+    equality."""
+    import torch
+    from elasticisolib_3d_amd.acquisition import ShotGeometry
+    from elasticisolib_3d_amd.batch import BatchedFwi, WindowStack
+    from elasticisolib_3d_amd.fwi import FwiProblem, FwiShot
+    m = _model()
+    start = [x * (1 + 0.02 * np.random.default_rng(5 + k).standard_normal(x.shape)) for k, x in enumerate(m)]
+    shot = _shots()[2]
+    ny, nx, nz = m[0].shape
+    assert (shot.window.x0, shot.window.y0, shot.window.nx, shot.window.ny) == (0, 0, nx, ny)
+
+    ws = WindowStack((nz, nx, ny), D, 5, [shot.window])
+    ax = ws.axes[0]
+    sg = ShotGeometry(ax, shot.src_xyz[:, 2], shot.src_xyz[:, 0], shot.src_xyz[:, 1])
+    rg = ShotGeometry(ax, shot.rec_xyz[:, 2], shot.rec_xyz[:, 0], shot.rec_xyz[:, 1])
+    plain = FwiProblem(ax, NTS, DTS, SUB, 0.95, [FwiShot(sg, rg, shot.signals)], precision="f64", interval=3,
+                       checkpoint="native", medium=medium)
+    plain.synthesize(*[ws.window_model(0, x) for x in m])
+    Jp, gp_pad = plain.misfit_and_gradient(*[ws.window_model(0, x) for x in start])
+    gp = np.zeros((3, ny, nx, nz))
+    ws.window_adjoint(0, gp_pad, gp)
+    obs_plain = [x.cpu().numpy() for x in plain.shots[0].observed]
+
+    batched = BatchedFwi(m[0].shape, D, 5, NTS, DTS, SUB, 0.95, [shot], batch=1, precision="f64", interval=3,
+                         checkpoint="native", medium=medium)
+    batched.synthesize(m)
+    Jb, gb = batched.misfit_and_gradient(start)
+    torch.cuda.synchronize()
+
+    for a, b in zip(obs_plain, shot.observed):
+        scale = float(np.abs(a).max())
+        assert scale > 0
+        assert float(np.abs(a - b).max()) <= 1e-12 * scale
+    e = float(np.linalg.norm(gb - gp) / np.linalg.norm(gp))
+    print(f"whole-model window vs plain operator ({medium}): J {Jp:.12e} {Jb:.12e}, gradient rel {e:.2e}",
+          flush=True)
+    assert abs(Jp - Jb) <= 1e-12 * Jp
     assert e < 1e-12
