@@ -134,3 +134,90 @@ def test_full_size_superposition_fp64():
     e = math.sqrt(num / den)
     print(f"full size superposition: rel L2 {e:.2e}", flush=True)
     assert den > 0 and e < 1e-12
+
+
+def _random_devices(rng, per_grid, spread):
+    """`per_grid` distinct random interior cells on each of the 7 grids,
+    within `spread` cells of the source cell."""
+    pos = []
+    for _ in range(7):
+        cells = set()
+        while len(cells) < per_grid:
+            dz, dx, dy = rng.integers(-spread, spread + 1, 3)
+            cells.add(_flat(CZ + int(dz), CX + int(dx), CY + int(dy)))
+        pos.append(np.array(sorted(cells), np.int64))
+    return pos
+
+
+def _random_traces(rng, pos):
+    from elasticisolib_3d_amd.acquisition import COMP_GRID
+    return [rng.standard_normal((len(pos[COMP_GRID[c]]), NTS)) for c in range(9)]
+
+
+class _Devices:
+    """Positions of the 7 grids, as the trace scaling reads them."""
+
+    def __init__(self, pos):
+        self._pos = pos
+
+    def positions(self):
+        return self._pos
+
+
+def test_full_size_nonlinear_dot_test_fp64():
+    """<F s, r> = <s, F* r> for the reference's nonlinear operator pair on
+    random sources and receivers on all 7 grids, FP64, full grid: the forward
+    injects medium-scaled source traces, the adjoint injects medium-scaled
+    receiver traces (nonlinearPropElasticGpu_3D.cpp:200-277, :481-558)."""
+    import torch
+    from elasticisolib_3d_amd.acquisition import scale_regular_traces
+    from elasticisolib_3d_amd.propagator import scale_medium
+    rng = np.random.default_rng(11)
+    spos, rpos = _random_devices(rng, 2, 12), _random_devices(rng, 5, 14)
+    s, r = _random_traces(rng, spos), _random_traces(rng, rpos)
+    shape = (NY, NX, NZ)
+    med8 = scale_medium(np.full(shape, RHO), np.full(shape, RHO * VP ** 2 - 2 * RHO * VS ** 2),
+                        np.full(shape, RHO * VS ** 2), DTS / SUB)
+    s_scaled = scale_regular_traces(med8, _Devices(spos), s, D ** 3)
+    r_scaled = scale_regular_traces(med8, _Devices(rpos), r, D ** 3)
+    del med8
+    print("full size f64: nonlinear dot test", flush=True)
+    prop = _propagator("f64")
+    d = prop.nl_forward(spos, s_scaled, rpos)
+    a = prop.nl_adjoint(spos, r_scaled, rpos)
+    del prop
+    torch.cuda.empty_cache()
+    lhs = sum(float(np.sum(x * y)) for x, y in zip(d, r))
+    rhs = sum(float(np.sum(x * y)) for x, y in zip(s, a))
+    e = abs(lhs - rhs) / abs(lhs)
+    print(f"full size nonlinear dot test: {lhs:.15e} {rhs:.15e} rel {e:.2e}", flush=True)
+    assert e < 1e-12
+
+
+def test_full_size_born_dot_test_fp64():
+    """<B dm, r> = <dm, B* r> for the Born operator with a dense random
+    perturbation of all 8 staggered arrays (zero in the FAT layers), FP64,
+    full grid (every coarse background slice kept in HBM)."""
+    import torch
+    rng = np.random.default_rng(12)
+    spos, rpos = _random_devices(rng, 2, 12), _random_devices(rng, 5, 14)
+    s = [torch.tensor(x, device="cuda") for x in _random_traces(rng, spos)]
+    r = [torch.tensor(x, device="cuda") for x in _random_traces(rng, rpos)]
+    g = torch.Generator(device="cuda").manual_seed(13)
+    dm = torch.randn((8, NY, NX, NZ), dtype=torch.float64, device="cuda", generator=g)
+    for sl in ((slice(None), slice(0, 4)), (slice(None), slice(-4, None))):   # FAT layers stay 0
+        dm[sl] = 0
+        dm[:, :, sl[1]] = 0
+        dm[:, :, :, sl[1]] = 0
+    print("full size f64: Born dot test", flush=True)
+    prop = _propagator("f64")
+    b = prop.born_forward(spos, s, dm, rpos)
+    img = prop.born_adjoint(spos, s, r, rpos)
+    torch.cuda.synchronize()
+    lhs = sum(float(torch.sum(x * y)) for x, y in zip(b, r))
+    rhs = float(torch.dot(dm.reshape(-1), img.reshape(-1)))
+    del prop, dm, img
+    torch.cuda.empty_cache()
+    e = abs(lhs - rhs) / abs(lhs)
+    print(f"full size Born dot test: {lhs:.15e} {rhs:.15e} rel {e:.2e}", flush=True)
+    assert e < 1e-11
