@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 import scipy.sparse as sp
@@ -174,14 +174,21 @@ def scale_regular_traces(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.
 
 
 def source_scaling_gradient(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.ndarray],
-                            gscaled: Sequence[np.ndarray], dxyz: float) -> np.ndarray:
+                            gscaled: Sequence[np.ndarray], dxyz: float, out: Optional[np.ndarray] = None) -> np.ndarray:
     """Gradient, with respect to the 8 scaled medium arrays, of a function of
     the scaled source traces, through scale_regular_traces: `reg` are the
     unscaled regular traces, `gscaled` the gradient with respect to the scaled
     ones. Returns an [8, ny, nx, nz] array that is zero away from the source
-    points (np.add.at accumulates points shared by components)."""
+    points (np.add.at accumulates points shared by components); with `out`
+    (a C-contiguous float64 [8, ny, nx, nz] array) the gradient is added into
+    it and `out` is returned, without a dense temporary."""
     shape = med8.shape
-    g = np.zeros((8, int(np.prod(shape[1:]))), np.float64)
+    if out is None:
+        g = np.zeros((8, int(np.prod(shape[1:]))), np.float64)
+    else:
+        if out.shape != shape or out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float64 array of the medium's shape")
+        g = out.reshape(8, -1)
     pos = geom.positions()
     inv = 1.0 / dxyz
     r = [np.asarray(x, np.float64) for x in reg]
@@ -199,4 +206,4 @@ def source_scaling_gradient(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[
     add(5, pos[4], np.sum(s[6] * r[6], axis=1))
     add(6, pos[5], np.sum(s[7] * r[7], axis=1))
     add(7, pos[6], np.sum(s[8] * r[8], axis=1))
-    return g.reshape(shape)
+    return out if out is not None else g.reshape(shape)

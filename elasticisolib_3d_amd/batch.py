@@ -44,12 +44,18 @@ def _pad_index(n: int, lo: int, hi: int) -> np.ndarray:
 def _unpad(g: np.ndarray, axis: int, lo: int, n: int) -> np.ndarray:
     """Transpose of edge-replication padding along one axis: the n core
     planes, with the low padding summed into the first and the high padding
-    into the last."""
-    g = np.moveaxis(g, axis, 0)
-    core = g[lo:lo + n].copy()
-    core[0] += g[:lo].sum(axis=0)
-    core[-1] += g[lo + n:].sum(axis=0)
-    return np.moveaxis(core, 0, axis)
+    into the last (slices along the axis itself, no transposed copies)."""
+    axis %= g.ndim
+
+    def along(a, b):
+        idx = [slice(None)] * g.ndim
+        idx[axis] = slice(a, b)
+        return tuple(idx)
+
+    core = g[along(lo, lo + n)].copy()
+    core[along(0, 1)] += g[along(0, lo)].sum(axis=axis, keepdims=True)
+    core[along(n - 1, n)] += g[along(lo + n, None)].sum(axis=axis, keepdims=True)
+    return core
 
 
 class WindowStack:
@@ -95,9 +101,15 @@ class WindowStack:
         return iy, ix, iz
 
     def window_model(self, k: int, m: np.ndarray) -> np.ndarray:
-        """Padded window k of a [..., ny, nx, nz] model array."""
-        iy, ix, iz = self._cut_index(k)
-        return m[..., iy[:, None, None], ix[None, :, None], iz[None, None, :]]
+        """Padded window k of a [..., ny, nx, nz] model array: the window's
+        cells, edge-replicated outwards (the cells _cut_index names)."""
+        w = self.windows[k]
+        zm, zp = self.zpads
+        xm, xp = self.xpads[k]
+        core = m[..., w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :]
+        widths = [(0, 0)] * (m.ndim - 3) + [(FAT + self.pad, FAT + self.pad), (FAT + xm, FAT + xp),
+                                            (FAT + zm, FAT + zp)]
+        return np.pad(core, widths, mode="edge")
 
     def window_adjoint(self, k: int, g: np.ndarray, out: np.ndarray) -> None:
         """out (full model [..., ny, nx, nz]) += the transpose of window_model
@@ -123,11 +135,14 @@ class WindowStack:
     def unstack(self, k: int, a: np.ndarray) -> np.ndarray:
         return a[..., self.y_off[k]:self.y_off[k] + self.ny_w[k], :self.nx_w[k], :]
 
-    def stacked_medium(self, rho, lam, mu, dtw: float):
-        """(stacked med8, list of per-window med8): staggering and scaling
-        window by window."""
-        meds = [P.scale_medium(self.window_model(k, rho), self.window_model(k, lam), self.window_model(k, mu), dtw)
+    def window_media(self, rho, lam, mu, dtw: float) -> List[np.ndarray]:
+        """Per-window med8: staggering and scaling window by window."""
+        return [P.scale_medium(self.window_model(k, rho), self.window_model(k, lam), self.window_model(k, mu), dtw)
                 for k in range(len(self.windows))]
+
+    def stacked_medium(self, rho, lam, mu, dtw: float):
+        """(stacked med8, list of per-window med8)."""
+        meds = self.window_media(rho, lam, mu, dtw)
         return self.stack(meds), meds
 
     def stacked_elastic(self, rho, lam, mu) -> np.ndarray:
@@ -200,7 +215,7 @@ class BatchedFwi:
         import torch
         ws = WindowStack((self.model_shape[2], self.model_shape[1], self.model_shape[0]), self.d, self.pad,
                          [s.window for s in group])
-        med, meds = ws.stacked_medium(model[0], model[1], model[2], self.dtw)
+        meds = ws.window_media(model[0], model[1], model[2], self.dtw)
         prop = P.Propagator(ws.nz, ws.nx, ws.ny, self.nts, self.sub, self.d, self.d, self.d, self.dts, ws.min_pad,
                             self.alpha, self.precision, self.device)
         ws.set_windows(prop)
@@ -213,7 +228,7 @@ class BatchedFwi:
             rlm = ws.stacked_elastic(model[0], model[1], model[2])
             prop.set_elastic(torch.tensor(rlm, dtype=td, device="cuda"), self.dtw)
         else:
-            prop.set_medium(torch.tensor(med, dtype=td, device="cuda"))
+            prop.set_medium(torch.tensor(ws.stack(meds), dtype=td, device="cuda"))
         if self.autotune and not self.tiling:
             from .autotune import autotune
             autotune(prop, reps=5)
@@ -311,7 +326,8 @@ class BatchedFwi:
                     gs.append(gsrc[c][off[g]:off[g] + n].double().cpu().numpy())
                 for g in range(7):
                     off[g] += sg.counts()[g]
-                g8k = ws.unstack(k, g8) + source_scaling_gradient(meds[k], sg, raw[k], gs, self.d ** 3)
+                g8k = source_scaling_gradient(meds[k], sg, raw[k], gs, self.d ** 3,
+                                              out=np.ascontiguousarray(ws.unstack(k, g8)))
                 ws.window_adjoint(k, P.medium_gradient(g8k, meds[k], self.dtw), g3)
             del prop
             torch.cuda.empty_cache()

@@ -9,34 +9,92 @@
 //   out[n-1] = in[n-1] + in[n-2]/2.
 // Scaled arrays: rho?Dtw = 2 dtw / rho_?, lamb2MuDtw = 2 dtw (lambda + 2 mu),
 // lambDtw = 2 dtw lambda, mu??Dtw = 2 dtw mu_??.
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/elastic_iso_3d.h"
 
 namespace {
 
+// Host worker threads: the hardware's, at most 16 and at most OMP_NUM_THREADS
+// when that is set.
+int host_threads() {
+  int nt = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+  nt = std::min(nt, 16);
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) nt = std::min(nt, v);
+  }
+  return nt;
+}
+
+// f(begin, end) over [0, n) split into contiguous ranges, one per thread
+// (serial below 64 Ki elements, each index standing for `work` elements).
+// Every caller writes disjoint elements per index.
+template <typename F>
+void parallel_for(int64_t n, F f, int64_t work = 1) {
+  const int64_t nt = host_threads();
+  if (nt <= 1 || n < 2 || n * work < (int64_t(1) << 16)) {
+    f(int64_t(0), n);
+    return;
+  }
+  const int64_t chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int64_t b = 0; b < n; b += chunk) th.emplace_back(f, b, std::min(n, b + chunk));
+  for (auto& t : th) t.join();
+}
+
 // Adjoint half-cell average along an axis of stride s and length na on a
 // [ny][nx][nz] array.
-void half_cell_adjoint(const double* in, double* out, int64_t n, int64_t s, int64_t na) {
-  for (int64_t i = 0; i < n; ++i) {
-    const int64_t a = (i / s) % na;
-    double v;
-    if (a == 0) v = 0.5 * in[i];
-    else if (a < na - 1) v = 0.5 * (in[i] + in[i - s]);
-    else v = in[i] + 0.5 * in[i - s];
-    out[i] = v;
+// The array is walked as [n / (s na)][na][s] blocks, so the axis index comes
+// from the loop nest rather than a division per element. Threads take whole
+// blocks, or the na rows of the single block when the axis is the slowest.
+template <typename Row>
+void along_axis(int64_t n, int64_t s, int64_t na, Row row) {
+  const int64_t block = s * na, nblocks = n / block;
+  if (nblocks >= 64) {
+    parallel_for(nblocks, [&](int64_t b0, int64_t b1) {
+      for (int64_t b = b0; b < b1; ++b)
+        for (int64_t a = 0; a < na; ++a) row(b * block + a * s, a);
+    }, block);
+  } else {
+    for (int64_t b = 0; b < nblocks; ++b)
+      parallel_for(na, [&](int64_t a0, int64_t a1) {
+        for (int64_t a = a0; a < a1; ++a) row(b * block + a * s, a);
+      }, s);
   }
+}
+
+void half_cell_adjoint(const double* in, double* out, int64_t n, int64_t s, int64_t na) {
+  along_axis(n, s, na, [=](int64_t off, int64_t a) {
+    const double* r = in + off;
+    double* w = out + off;
+    if (a == 0) {
+      for (int64_t j = 0; j < s; ++j) w[j] = 0.5 * r[j];
+    } else if (a < na - 1) {
+      for (int64_t j = 0; j < s; ++j) w[j] = 0.5 * (r[j] + r[j - s]);
+    } else {
+      for (int64_t j = 0; j < s; ++j) w[j] = r[j] + 0.5 * r[j - s];
+    }
+  });
 }
 
 // Forward half-cell average (the transpose of half_cell_adjoint):
 //   out[i] = (in[i] + in[i+1])/2 for i < n-1,  out[n-1] = in[n-1].
 void half_cell_forward(const double* in, double* out, int64_t n, int64_t s, int64_t na) {
-  for (int64_t i = 0; i < n; ++i) {
-    const int64_t a = (i / s) % na;
-    out[i] = a < na - 1 ? 0.5 * (in[i] + in[i + s]) : in[i];
-  }
+  along_axis(n, s, na, [=](int64_t off, int64_t a) {
+    const double* r = in + off;
+    double* w = out + off;
+    if (a < na - 1) {
+      for (int64_t j = 0; j < s; ++j) w[j] = 0.5 * (r[j] + r[j + s]);
+    } else {
+      for (int64_t j = 0; j < s; ++j) w[j] = r[j];
+    }
+  });
 }
 
 bool dims_ok(int64_t nz, int64_t nx, int64_t ny) { return nz >= 2 && nx >= 2 && ny >= 2; }
@@ -66,16 +124,18 @@ extern "C" int ela_scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, 
   half_cell_adjoint(mu, t.data(), n, sy, ny);   // mu staggered along y
   half_cell_adjoint(t.data(), Myz, n, sz, nz);  // ... then z
   const double tdt = 2.0 * dtw;
-  for (int64_t i = 0; i < n; ++i) {
-    Bx[i] = tdt / Bx[i];
-    By[i] = tdt / By[i];
-    Bz[i] = tdt / Bz[i];
-    L2M[i] = tdt * (lame[i] + 2.0 * mu[i]);
-    L[i] = tdt * lame[i];
-    Mxz[i] *= tdt;
-    Mxy[i] *= tdt;
-    Myz[i] *= tdt;
-  }
+  parallel_for(n, [=](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      Bx[i] = tdt / Bx[i];
+      By[i] = tdt / By[i];
+      Bz[i] = tdt / Bz[i];
+      L2M[i] = tdt * (lame[i] + 2.0 * mu[i]);
+      L[i] = tdt * lame[i];
+      Mxz[i] *= tdt;
+      Mxy[i] *= tdt;
+      Myz[i] *= tdt;
+    }
+  });
   return ELA_OK;
 }
 
@@ -101,12 +161,14 @@ extern "C" int ela_born_stagger(int64_t nz, int64_t nx, int64_t ny, double dtw, 
   half_cell_adjoint(t.data(), out8 + 6 * n, n, sy, ny);
   half_cell_adjoint(dmu, t.data(), n, sy, ny);
   half_cell_adjoint(t.data(), out8 + 7 * n, n, sz, nz);
-  for (int64_t i = 0; i < n; ++i) {
-    for (int k = 0; k < 3; ++k) out8[k * n + i] *= med8[k * n + i];
-    out8[3 * n + i] = tdt * dlam[i];
-    out8[4 * n + i] = tdt * dmu[i];
-    for (int k = 5; k < 8; ++k) out8[k * n + i] *= tdt;
-  }
+  parallel_for(n, [=](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      for (int k = 0; k < 3; ++k) out8[k * n + i] *= med8[k * n + i];
+      out8[3 * n + i] = tdt * dlam[i];
+      out8[4 * n + i] = tdt * dmu[i];
+      for (int k = 5; k < 8; ++k) out8[k * n + i] *= tdt;
+    }
+  });
   return ELA_OK;
 }
 
@@ -123,23 +185,36 @@ extern "C" int ela_born_unstagger(int64_t nz, int64_t nx, int64_t ny, double dtw
   double* dmu = out3 + 2 * n;
   std::memset(drho, 0, n * sizeof(double));
   const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
+  double* pa = a.data();
+  const double* pt = t.data();
+  const double* pu = u.data();
   for (int k = 0; k < 3; ++k) {
-    for (int64_t i = 0; i < n; ++i) a[i] = img8[k * n + i] * med8[k * n + i];
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) pa[i] = img8[k * n + i] * med8[k * n + i];
+    });
     half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
-    for (int64_t i = 0; i < n; ++i) drho[i] += t[i];
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) drho[i] += pt[i];
+    });
   }
-  for (int64_t i = 0; i < n; ++i) {
-    dlam[i] = tdt * img8[3 * n + i];
-    dmu[i] = tdt * img8[4 * n + i];
-  }
+  parallel_for(n, [=](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      dlam[i] = tdt * img8[3 * n + i];
+      dmu[i] = tdt * img8[4 * n + i];
+    }
+  });
   // (outer, inner) forward averages of muxz, muxy, muyz
   const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
   const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
   for (int k = 0; k < 3; ++k) {
-    for (int64_t i = 0; i < n; ++i) a[i] = tdt * img8[(5 + k) * n + i];
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) pa[i] = tdt * img8[(5 + k) * n + i];
+    });
     half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
     half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
-    for (int64_t i = 0; i < n; ++i) dmu[i] += u[i];
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) dmu[i] += pu[i];
+    });
   }
   return ELA_OK;
 }
@@ -161,25 +236,38 @@ extern "C" int ela_medium_gradient(int64_t nz, int64_t nx, int64_t ny, double dt
   double* gmu = out3 + 2 * n;
   std::memset(grho, 0, n * sizeof(double));
   const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
+  double* pa = a.data();
+  const double* pt = t.data();
+  const double* pu = u.data();
   for (int k = 0; k < 3; ++k) {
-    for (int64_t i = 0; i < n; ++i) {
-      const double b = med8[k * n + i];
-      a[i] = -b * b / tdt * g8[k * n + i];
-    }
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) {
+        const double b = med8[k * n + i];
+        pa[i] = -b * b / tdt * g8[k * n + i];
+      }
+    });
     half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
-    for (int64_t i = 0; i < n; ++i) grho[i] += t[i];
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) grho[i] += pt[i];
+    });
   }
-  for (int64_t i = 0; i < n; ++i) {
-    glam[i] = tdt * (g8[3 * n + i] + g8[4 * n + i]);
-    gmu[i] = 2.0 * tdt * g8[3 * n + i];
-  }
+  parallel_for(n, [=](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      glam[i] = tdt * (g8[3 * n + i] + g8[4 * n + i]);
+      gmu[i] = 2.0 * tdt * g8[3 * n + i];
+    }
+  });
   const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
   const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
   for (int k = 0; k < 3; ++k) {
-    for (int64_t i = 0; i < n; ++i) a[i] = tdt * g8[(5 + k) * n + i];
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) pa[i] = tdt * g8[(5 + k) * n + i];
+    });
     half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
     half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
-    for (int64_t i = 0; i < n; ++i) gmu[i] += u[i];
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) gmu[i] += pu[i];
+    });
   }
   return ELA_OK;
 }

@@ -35,6 +35,9 @@ def test_window_model_adjoint():
         xm = ws.xpads[k][0]
         assert np.array_equal(wm[:, 4 + 5:4 + 5 + w.ny, 4 + xm:4 + xm + w.nx, 4 + zm:4 + zm + 20],
                               m[:, w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :])
+        # every padded cell copies the model cell the edge-replication index map names
+        iy, ix, iz = ws._cut_index(k)
+        assert np.array_equal(wm, m[:, iy[:, None, None], ix[None, :, None], iz[None, None, :]])
 
 
 def test_stacked_positions_and_stack():
