@@ -44,7 +44,14 @@ void parallel_for(int64_t n, F f, int64_t work = 1) {
   }
   const int64_t chunk = (n + nt - 1) / nt;
   std::vector<std::thread> th;
-  for (int64_t b = 0; b < n; b += chunk) th.emplace_back(f, b, std::min(n, b + chunk));
+  th.reserve(static_cast<size_t>(nt));
+  int64_t b = 0;
+  try {
+    for (; b < n; b += chunk) th.emplace_back(f, b, std::min(n, b + chunk));
+  } catch (...) {
+    // a thread could not be started: the ranges left run on this thread
+  }
+  for (; b < n; b += chunk) f(b, std::min(n, b + chunk));
   for (auto& t : th) t.join();
 }
 
