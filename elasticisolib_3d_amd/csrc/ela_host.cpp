@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -30,6 +31,19 @@ int host_threads() {
     if (v > 0) nt = std::min(nt, v);
   }
   return nt;
+}
+
+// Runs an entry point's body with no exception crossing the C ABI: a failed
+// host allocation is ELA_ERR_MEM, anything else ELA_ERR_ARG.
+template <typename F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return ELA_ERR_MEM;
+  } catch (...) {
+    return ELA_ERR_ARG;
+  }
 }
 
 // f(begin, end) over [0, n) split into contiguous ranges, one per thread
@@ -110,40 +124,42 @@ bool dims_ok(int64_t nz, int64_t nx, int64_t ny) { return nz >= 2 && nx >= 2 && 
 
 extern "C" int ela_scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* rho,
                                 const double* lame, const double* mu, double* med8) {
-  if (nz < 2 || nx < 2 || ny < 2 || !rho || !lame || !mu || !med8 || !(dtw > 0)) return ELA_ERR_ARG;
-  const int64_t n = nz * nx * ny;
-  const int64_t sz = 1, sx = nz, sy = nz * nx;
-  double* Bx = med8;
-  double* By = med8 + n;
-  double* Bz = med8 + 2 * n;
-  double* L2M = med8 + 3 * n;
-  double* L = med8 + 4 * n;
-  double* Mxz = med8 + 5 * n;
-  double* Mxy = med8 + 6 * n;
-  double* Myz = med8 + 7 * n;
-  std::vector<double> t(n);
-  half_cell_adjoint(rho, Bx, n, sx, nx);
-  half_cell_adjoint(rho, By, n, sy, ny);
-  half_cell_adjoint(rho, Bz, n, sz, nz);
-  half_cell_adjoint(mu, t.data(), n, sx, nx);   // mu staggered along x
-  half_cell_adjoint(t.data(), Mxz, n, sz, nz);  // ... then z
-  half_cell_adjoint(t.data(), Mxy, n, sy, ny);  // ... then y
-  half_cell_adjoint(mu, t.data(), n, sy, ny);   // mu staggered along y
-  half_cell_adjoint(t.data(), Myz, n, sz, nz);  // ... then z
-  const double tdt = 2.0 * dtw;
-  parallel_for(n, [=](int64_t i0, int64_t i1) {
-    for (int64_t i = i0; i < i1; ++i) {
-      Bx[i] = tdt / Bx[i];
-      By[i] = tdt / By[i];
-      Bz[i] = tdt / Bz[i];
-      L2M[i] = tdt * (lame[i] + 2.0 * mu[i]);
-      L[i] = tdt * lame[i];
-      Mxz[i] *= tdt;
-      Mxy[i] *= tdt;
-      Myz[i] *= tdt;
-    }
+  return guarded([&]() -> int {
+    if (nz < 2 || nx < 2 || ny < 2 || !rho || !lame || !mu || !med8 || !(dtw > 0)) return ELA_ERR_ARG;
+    const int64_t n = nz * nx * ny;
+    const int64_t sz = 1, sx = nz, sy = nz * nx;
+    double* Bx = med8;
+    double* By = med8 + n;
+    double* Bz = med8 + 2 * n;
+    double* L2M = med8 + 3 * n;
+    double* L = med8 + 4 * n;
+    double* Mxz = med8 + 5 * n;
+    double* Mxy = med8 + 6 * n;
+    double* Myz = med8 + 7 * n;
+    std::vector<double> t(n);
+    half_cell_adjoint(rho, Bx, n, sx, nx);
+    half_cell_adjoint(rho, By, n, sy, ny);
+    half_cell_adjoint(rho, Bz, n, sz, nz);
+    half_cell_adjoint(mu, t.data(), n, sx, nx);   // mu staggered along x
+    half_cell_adjoint(t.data(), Mxz, n, sz, nz);  // ... then z
+    half_cell_adjoint(t.data(), Mxy, n, sy, ny);  // ... then y
+    half_cell_adjoint(mu, t.data(), n, sy, ny);   // mu staggered along y
+    half_cell_adjoint(t.data(), Myz, n, sz, nz);  // ... then z
+    const double tdt = 2.0 * dtw;
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) {
+        Bx[i] = tdt / Bx[i];
+        By[i] = tdt / By[i];
+        Bz[i] = tdt / Bz[i];
+        L2M[i] = tdt * (lame[i] + 2.0 * mu[i]);
+        L[i] = tdt * lame[i];
+        Mxz[i] *= tdt;
+        Mxy[i] *= tdt;
+        Myz[i] *= tdt;
+      }
+    });
+    return ELA_OK;
   });
-  return ELA_OK;
 }
 
 // Born perturbations (BornElasticShotsGpu_3D.cpp:167-235): (drho, dlambda, dmu)
@@ -153,77 +169,81 @@ extern "C" int ela_scale_medium(int64_t nz, int64_t nx, int64_t ny, double dtw, 
 // medium (half_cell_adjoint).
 extern "C" int ela_born_stagger(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
                                 const double* dm3, double* out8) {
-  if (!dims_ok(nz, nx, ny) || !med8 || !dm3 || !out8 || !(dtw > 0)) return ELA_ERR_ARG;
-  const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
-  const double* drho = dm3;
-  const double* dlam = dm3 + n;
-  const double* dmu = dm3 + 2 * n;
-  const double tdt = 2.0 * dtw;
-  std::vector<double> t(n);
-  half_cell_adjoint(drho, out8, n, sx, nx);
-  half_cell_adjoint(drho, out8 + n, n, sy, ny);
-  half_cell_adjoint(drho, out8 + 2 * n, n, sz, nz);
-  half_cell_adjoint(dmu, t.data(), n, sx, nx);
-  half_cell_adjoint(t.data(), out8 + 5 * n, n, sz, nz);
-  half_cell_adjoint(t.data(), out8 + 6 * n, n, sy, ny);
-  half_cell_adjoint(dmu, t.data(), n, sy, ny);
-  half_cell_adjoint(t.data(), out8 + 7 * n, n, sz, nz);
-  parallel_for(n, [=](int64_t i0, int64_t i1) {
-    for (int64_t i = i0; i < i1; ++i) {
-      for (int k = 0; k < 3; ++k) out8[k * n + i] *= med8[k * n + i];
-      out8[3 * n + i] = tdt * dlam[i];
-      out8[4 * n + i] = tdt * dmu[i];
-      for (int k = 5; k < 8; ++k) out8[k * n + i] *= tdt;
-    }
+  return guarded([&]() -> int {
+    if (!dims_ok(nz, nx, ny) || !med8 || !dm3 || !out8 || !(dtw > 0)) return ELA_ERR_ARG;
+    const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
+    const double* drho = dm3;
+    const double* dlam = dm3 + n;
+    const double* dmu = dm3 + 2 * n;
+    const double tdt = 2.0 * dtw;
+    std::vector<double> t(n);
+    half_cell_adjoint(drho, out8, n, sx, nx);
+    half_cell_adjoint(drho, out8 + n, n, sy, ny);
+    half_cell_adjoint(drho, out8 + 2 * n, n, sz, nz);
+    half_cell_adjoint(dmu, t.data(), n, sx, nx);
+    half_cell_adjoint(t.data(), out8 + 5 * n, n, sz, nz);
+    half_cell_adjoint(t.data(), out8 + 6 * n, n, sy, ny);
+    half_cell_adjoint(dmu, t.data(), n, sy, ny);
+    half_cell_adjoint(t.data(), out8 + 7 * n, n, sz, nz);
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) {
+        for (int k = 0; k < 3; ++k) out8[k * n + i] *= med8[k * n + i];
+        out8[3 * n + i] = tdt * dlam[i];
+        out8[4 * n + i] = tdt * dmu[i];
+        for (int k = 5; k < 8; ++k) out8[k * n + i] *= tdt;
+      }
+    });
+    return ELA_OK;
   });
-  return ELA_OK;
 }
 
 // Adjoint of ela_born_stagger (BornElasticShotsGpu_3D.cpp:541-607): the 8
 // images -> (drho, dlambda, dmu), overwritten.
 extern "C" int ela_born_unstagger(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
                                   const double* img8, double* out3) {
-  if (!dims_ok(nz, nx, ny) || !med8 || !img8 || !out3 || !(dtw > 0)) return ELA_ERR_ARG;
-  const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
-  const double tdt = 2.0 * dtw;
-  std::vector<double> a(n), t(n), u(n);
-  double* drho = out3;
-  double* dlam = out3 + n;
-  double* dmu = out3 + 2 * n;
-  std::memset(drho, 0, n * sizeof(double));
-  const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
-  double* pa = a.data();
-  const double* pt = t.data();
-  const double* pu = u.data();
-  for (int k = 0; k < 3; ++k) {
-    parallel_for(n, [=](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) pa[i] = img8[k * n + i] * med8[k * n + i];
-    });
-    half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
-    parallel_for(n, [=](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) drho[i] += pt[i];
-    });
-  }
-  parallel_for(n, [=](int64_t i0, int64_t i1) {
-    for (int64_t i = i0; i < i1; ++i) {
-      dlam[i] = tdt * img8[3 * n + i];
-      dmu[i] = tdt * img8[4 * n + i];
+  return guarded([&]() -> int {
+    if (!dims_ok(nz, nx, ny) || !med8 || !img8 || !out3 || !(dtw > 0)) return ELA_ERR_ARG;
+    const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
+    const double tdt = 2.0 * dtw;
+    std::vector<double> a(n), t(n), u(n);
+    double* drho = out3;
+    double* dlam = out3 + n;
+    double* dmu = out3 + 2 * n;
+    std::memset(drho, 0, n * sizeof(double));
+    const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
+    double* pa = a.data();
+    const double* pt = t.data();
+    const double* pu = u.data();
+    for (int k = 0; k < 3; ++k) {
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) pa[i] = img8[k * n + i] * med8[k * n + i];
+      });
+      half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) drho[i] += pt[i];
+      });
     }
+    parallel_for(n, [=](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) {
+        dlam[i] = tdt * img8[3 * n + i];
+        dmu[i] = tdt * img8[4 * n + i];
+      }
+    });
+    // (outer, inner) forward averages of muxz, muxy, muyz
+    const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
+    const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
+    for (int k = 0; k < 3; ++k) {
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) pa[i] = tdt * img8[(5 + k) * n + i];
+      });
+      half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
+      half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) dmu[i] += pu[i];
+      });
+    }
+    return ELA_OK;
   });
-  // (outer, inner) forward averages of muxz, muxy, muyz
-  const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
-  const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
-  for (int k = 0; k < 3; ++k) {
-    parallel_for(n, [=](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) pa[i] = tdt * img8[(5 + k) * n + i];
-    });
-    half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
-    half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
-    parallel_for(n, [=](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) dmu[i] += pu[i];
-    });
-  }
-  return ELA_OK;
 }
 
 // Gradient of a function of the 8 scaled medium arrays with respect to
@@ -234,49 +254,51 @@ extern "C" int ela_born_unstagger(int64_t nz, int64_t nx, int64_t ny, double dtw
 // 2 dtw S_outer(S_inner(g)) for mu. out3 is overwritten.
 extern "C" int ela_medium_gradient(int64_t nz, int64_t nx, int64_t ny, double dtw, const double* med8,
                                    const double* g8, double* out3) {
-  if (!dims_ok(nz, nx, ny) || !med8 || !g8 || !out3 || !(dtw > 0)) return ELA_ERR_ARG;
-  const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
-  const double tdt = 2.0 * dtw;
-  std::vector<double> a(n), t(n), u(n);
-  double* grho = out3;
-  double* glam = out3 + n;
-  double* gmu = out3 + 2 * n;
-  std::memset(grho, 0, n * sizeof(double));
-  const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
-  double* pa = a.data();
-  const double* pt = t.data();
-  const double* pu = u.data();
-  for (int k = 0; k < 3; ++k) {
+  return guarded([&]() -> int {
+    if (!dims_ok(nz, nx, ny) || !med8 || !g8 || !out3 || !(dtw > 0)) return ELA_ERR_ARG;
+    const int64_t n = nz * nx * ny, sz = 1, sx = nz, sy = nz * nx;
+    const double tdt = 2.0 * dtw;
+    std::vector<double> a(n), t(n), u(n);
+    double* grho = out3;
+    double* glam = out3 + n;
+    double* gmu = out3 + 2 * n;
+    std::memset(grho, 0, n * sizeof(double));
+    const int64_t st[3] = {sx, sy, sz}, na[3] = {nx, ny, nz};
+    double* pa = a.data();
+    const double* pt = t.data();
+    const double* pu = u.data();
+    for (int k = 0; k < 3; ++k) {
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+          const double b = med8[k * n + i];
+          pa[i] = -b * b / tdt * g8[k * n + i];
+        }
+      });
+      half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) grho[i] += pt[i];
+      });
+    }
     parallel_for(n, [=](int64_t i0, int64_t i1) {
       for (int64_t i = i0; i < i1; ++i) {
-        const double b = med8[k * n + i];
-        pa[i] = -b * b / tdt * g8[k * n + i];
+        glam[i] = tdt * (g8[3 * n + i] + g8[4 * n + i]);
+        gmu[i] = 2.0 * tdt * g8[3 * n + i];
       }
     });
-    half_cell_forward(a.data(), t.data(), n, st[k], na[k]);
-    parallel_for(n, [=](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) grho[i] += pt[i];
-    });
-  }
-  parallel_for(n, [=](int64_t i0, int64_t i1) {
-    for (int64_t i = i0; i < i1; ++i) {
-      glam[i] = tdt * (g8[3 * n + i] + g8[4 * n + i]);
-      gmu[i] = 2.0 * tdt * g8[3 * n + i];
+    const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
+    const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
+    for (int k = 0; k < 3; ++k) {
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) pa[i] = tdt * g8[(5 + k) * n + i];
+      });
+      half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
+      half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
+      parallel_for(n, [=](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) gmu[i] += pu[i];
+      });
     }
+    return ELA_OK;
   });
-  const int64_t so[3][2] = {{sx, sz}, {sx, sy}, {sy, sz}};
-  const int64_t no[3][2] = {{nx, nz}, {nx, ny}, {ny, nz}};
-  for (int k = 0; k < 3; ++k) {
-    parallel_for(n, [=](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) pa[i] = tdt * g8[(5 + k) * n + i];
-    });
-    half_cell_forward(a.data(), t.data(), n, so[k][1], no[k][1]);
-    half_cell_forward(t.data(), u.data(), n, so[k][0], no[k][0]);
-    parallel_for(n, [=](int64_t i0, int64_t i1) {
-      for (int64_t i = i0; i < i1; ++i) gmu[i] += pu[i];
-    });
-  }
-  return ELA_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -376,60 +398,64 @@ extern "C" int ela_interp_weights(const ela_interp_axes* ax, int64_t n_dev, cons
                                   const double* yc, int method, int half_length, int dipole, double z_shift,
                                   double x_shift, double y_shift, int64_t* nbr, double* weight,
                                   int64_t* unique_pos, int64_t* n_unique) {
-  if (!ax || n_dev < 0 || (n_dev > 0 && (!zc || !xc || !yc || !nbr || !weight || !unique_pos || !n_unique)))
-    return ELA_ERR_ARG;
-  if (method != ELA_INTERP_LINEAR && method != ELA_INTERP_SINC) return ELA_ERR_ARG;
-  if (method == ELA_INTERP_LINEAR && half_length != 1) return ELA_ERR_ARG;
-  if (half_length < 1 || half_length > 32 || (dipole != 0 && dipole != 1)) return ELA_ERR_ARG;
-  const int64_t nz = ax->nz, nx = ax->nx, ny = ax->ny, fat = ax->fat;
-  // device bounds (checkOutOfBounds, spaceInterpGpu_3D.cpp:177-211)
-  const float zMin = float(ax->oz + (fat + ax->z_pad_minus) * ax->dz);
-  const float xMin = float(ax->ox + (fat + ax->x_pad_minus) * ax->dx);
-  const float yMin = float(ax->oy + (fat + ax->y_pad) * ax->dy);
-  const float zMax = zMin + float((nz - 2 * fat - ax->z_pad_minus - ax->z_pad_plus - 1) * ax->dz);
-  const float xMax = xMin + float((nx - 2 * fat - ax->x_pad_minus - ax->x_pad_plus - 1) * ax->dx);
-  const float yMax = yMin + float((ny - 2 * fat - 2 * ax->y_pad - 1) * ax->dy);
-  const float tol = float(ax->tolerance);
-  // linear: both neighbours inside the unpadded model; sinc: the support inside the stepped interior
-  const InterpBounds lin{{fat + ax->z_pad_minus, fat + ax->x_pad_minus, fat + ax->y_pad},
-                         {nz - fat - ax->z_pad_plus, nx - fat - ax->x_pad_plus, ny - fat - ax->y_pad}};
-  const InterpBounds snc{{fat, fat, fat}, {nz - fat, nx - fat, ny - fat}};
-  const int64_t nf1 = method == ELA_INTERP_LINEAR ? 8 : int64_t(8) * half_length * half_length * half_length;
-  const int64_t nf = dipole ? 2 * nf1 : nf1;
-  std::vector<int64_t> idx(nf);
-  std::unordered_map<int64_t, int64_t> seen;
-  seen.reserve(size_t(n_dev) * size_t(nf));
-  int64_t nu = 0;
-  for (int64_t d = 0; d < n_dev; ++d) {
-    const float z = float(zc[d]), x = float(xc[d]), y = float(yc[d]);
-    if (z - zMax > tol || z - zMin < -tol || x - xMax > tol || x - xMin < -tol || y - yMax > tol || y - yMin < -tol)
+  return guarded([&]() -> int {
+    if (!ax || n_dev < 0 || (n_dev > 0 && (!zc || !xc || !yc || !nbr || !weight || !unique_pos || !n_unique)))
       return ELA_ERR_ARG;
-    double* w = weight + d * nf;
-    if (pole_weights(ax, method, half_length, z, x, y, 1.0, lin, snc, idx.data(), w) != ELA_OK) return ELA_ERR_ARG;
-    if (dipole &&
-        pole_weights(ax, method, half_length, float(zc[d] + z_shift), float(xc[d] + x_shift), float(yc[d] + y_shift),
-                     -1.0, lin, snc, idx.data() + nf1, w + nf1) != ELA_OK)
-      return ELA_ERR_ARG;
-    for (int64_t k = 0; k < nf; ++k) {
-      auto it = seen.find(idx[k]);
-      int64_t u;
-      if (it == seen.end()) {
-        u = nu++;
-        seen.emplace(idx[k], u);
-        unique_pos[u] = idx[k];
-      } else {
-        u = it->second;
+    if (method != ELA_INTERP_LINEAR && method != ELA_INTERP_SINC) return ELA_ERR_ARG;
+    if (method == ELA_INTERP_LINEAR && half_length != 1) return ELA_ERR_ARG;
+    if (half_length < 1 || half_length > 32 || (dipole != 0 && dipole != 1)) return ELA_ERR_ARG;
+    const int64_t nz = ax->nz, nx = ax->nx, ny = ax->ny, fat = ax->fat;
+    // device bounds (checkOutOfBounds, spaceInterpGpu_3D.cpp:177-211)
+    const float zMin = float(ax->oz + (fat + ax->z_pad_minus) * ax->dz);
+    const float xMin = float(ax->ox + (fat + ax->x_pad_minus) * ax->dx);
+    const float yMin = float(ax->oy + (fat + ax->y_pad) * ax->dy);
+    const float zMax = zMin + float((nz - 2 * fat - ax->z_pad_minus - ax->z_pad_plus - 1) * ax->dz);
+    const float xMax = xMin + float((nx - 2 * fat - ax->x_pad_minus - ax->x_pad_plus - 1) * ax->dx);
+    const float yMax = yMin + float((ny - 2 * fat - 2 * ax->y_pad - 1) * ax->dy);
+    const float tol = float(ax->tolerance);
+    // linear: both neighbours inside the unpadded model; sinc: the support inside the stepped interior
+    const InterpBounds lin{{fat + ax->z_pad_minus, fat + ax->x_pad_minus, fat + ax->y_pad},
+                           {nz - fat - ax->z_pad_plus, nx - fat - ax->x_pad_plus, ny - fat - ax->y_pad}};
+    const InterpBounds snc{{fat, fat, fat}, {nz - fat, nx - fat, ny - fat}};
+    const int64_t nf1 = method == ELA_INTERP_LINEAR ? 8 : int64_t(8) * half_length * half_length * half_length;
+    const int64_t nf = dipole ? 2 * nf1 : nf1;
+    std::vector<int64_t> idx(nf);
+    std::unordered_map<int64_t, int64_t> seen;
+    seen.reserve(size_t(n_dev) * size_t(nf));
+    int64_t nu = 0;
+    for (int64_t d = 0; d < n_dev; ++d) {
+      const float z = float(zc[d]), x = float(xc[d]), y = float(yc[d]);
+      if (z - zMax > tol || z - zMin < -tol || x - xMax > tol || x - xMin < -tol || y - yMax > tol || y - yMin < -tol)
+        return ELA_ERR_ARG;
+      double* w = weight + d * nf;
+      if (pole_weights(ax, method, half_length, z, x, y, 1.0, lin, snc, idx.data(), w) != ELA_OK) return ELA_ERR_ARG;
+      if (dipole &&
+          pole_weights(ax, method, half_length, float(zc[d] + z_shift), float(xc[d] + x_shift), float(yc[d] + y_shift),
+                       -1.0, lin, snc, idx.data() + nf1, w + nf1) != ELA_OK)
+        return ELA_ERR_ARG;
+      for (int64_t k = 0; k < nf; ++k) {
+        auto it = seen.find(idx[k]);
+        int64_t u;
+        if (it == seen.end()) {
+          u = nu++;
+          seen.emplace(idx[k], u);
+          unique_pos[u] = idx[k];
+        } else {
+          u = it->second;
+        }
+        nbr[d * nf + k] = u;
       }
-      nbr[d * nf + k] = u;
     }
-  }
-  *n_unique = nu;
-  return ELA_OK;
+    *n_unique = nu;
+    return ELA_OK;
+  });
 }
 
 extern "C" int ela_interp_linear(const ela_interp_axes* ax, int64_t n_dev, const double* zc, const double* xc,
                                  const double* yc, int64_t* nbr, double* weight, int64_t* unique_pos,
                                  int64_t* n_unique) {
-  return ela_interp_weights(ax, n_dev, zc, xc, yc, ELA_INTERP_LINEAR, 1, 0, 0.0, 0.0, 0.0, nbr, weight, unique_pos,
-                            n_unique);
+  return guarded([&]() -> int {
+    return ela_interp_weights(ax, n_dev, zc, xc, yc, ELA_INTERP_LINEAR, 1, 0, 0.0, 0.0, 0.0, nbr, weight, unique_pos,
+                              n_unique);
+  });
 }

@@ -64,7 +64,7 @@ extern "C" {
 #define ELA_OK 0
 #define ELA_ERR_ARG 1     /* invalid argument or inconsistent sizes */
 #define ELA_ERR_HIP 2     /* a HIP runtime call failed */
-#define ELA_ERR_MEM 3     /* device allocation failed */
+#define ELA_ERR_MEM 3     /* device or host allocation failed */
 #define ELA_ERR_STATE 4   /* call out of order (e.g. no medium set) */
 
 #define ELA_MEM_HOST 0
