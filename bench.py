@@ -289,7 +289,7 @@ def _fwi_shots(w, ax):
 
 def bench_fwi(args):
     """FWI misfit gradient (adjoint state) on the 256x256x160 workload: 4 shots,
-    FP32 stencil, bfloat16 checkpoints every 20 fine steps, FP64 gradient
+    FP32 stencil, int16 block-scaled checkpoints (default) every 20 fine steps, FP64 gradient
     accumulation; one shot per launch sequence (the plain path). Accuracy of
     the FP32 gradient against the FP64 gradient (exact checkpoints) on shot 1,
     and central differences of the FP64 misfit on `--fd-cells` cells."""
@@ -524,7 +524,7 @@ def main():
                          "(the reference's layout); 0 skips them")
     ap.add_argument("--fd-cells", type=int, default=16,
                     help="fwi: cells checked by central differences (-1: skip the accuracy part)")
-    ap.add_argument("--checkpoint", choices=("bf16", "s16", "native"), default="bf16",
+    ap.add_argument("--checkpoint", choices=("bf16", "s16", "native"), default="s16",
                     help="fwi, fwi_batch: storage of the gradient's outer checkpoints")
     ap.add_argument("--born-segment", type=int, default=0,
                     help="Born adjoint storage: 0 automatic, -1 all samples, K checkpoint segment")

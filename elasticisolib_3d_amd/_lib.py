@@ -27,7 +27,7 @@ EXPORTS = (
     "ela_set_tile_order", "ela_set_born_storage", "ela_born_storage_used",
     "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
     "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
-    "ela_set_windows", "ela_set_graph", "ela_stream_probe", "ela_set_elastic",
+    "ela_set_windows", "ela_set_graph", "ela_stream_probe", "ela_set_elastic", "ela_set_born_mode",
 )
 
 

@@ -179,17 +179,17 @@ class BatchShot:
 
 class BatchedFwi:
     """FWI misfit and gradient over shots with shot-local windows, `batch`
-    shots per launch sequence on one GPU (FP32 stencil, bfloat16, int16
-    block-scaled or exact checkpoints, FP64 gradient accumulation, as
+    shots per launch sequence on one GPU (FP32 stencil, int16 block-scaled
+    (default), bfloat16 or exact checkpoints, FP64 gradient accumulation, as
     FwiProblem). `model_shape` is the [ny, nx, nz] shape of the (unpadded)
-    model arrays; `checkpoint` ("bf16", "s16", "native") overrides the older
-    `bf16` flag; `autotune` times the launch shapes of every stacked grid
+    model arrays; `checkpoint` is "s16", "bf16" or "native", and the older
+    `bf16` flag, when given, means "bf16" / "native"; `autotune` times the launch shapes of every stacked grid
     unless `tiling` fixes them; `medium` ("elastic" or "scaled") chooses how
     the stacked model reaches the library, as in FwiProblem."""
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
-                 bf16: bool = True, device: int = 0, tiling=None, checkpoint=None, autotune: bool = False,
+                 bf16=None, device: int = 0, tiling=None, checkpoint: str = "s16", autotune: bool = False,
                  medium: str = "elastic"):
         self.model_shape, self.d, self.pad = tuple(model_shape), float(d), int(pad)
         self.nts, self.dts, self.sub, self.alpha = int(nts), float(dts), int(sub), float(alpha_cos)
@@ -197,7 +197,7 @@ class BatchedFwi:
         self.shots = list(shots)
         self.batch = max(1, int(batch))
         self.precision, self.interval, self.device = precision, int(interval), int(device)
-        self.checkpoint = bool(bf16) if checkpoint is None else checkpoint   # "bf16", "s16", "native" or a bool
+        self.checkpoint = checkpoint if bf16 is None else ("bf16" if bf16 else "native")
         self.tiling = tiling
         self.autotune = bool(autotune)   # time the launch shapes of every stacked grid (cached per grid shape)
         # "elastic": hand the stacked (rho, lambda, mu) to the library (its forward steps form the

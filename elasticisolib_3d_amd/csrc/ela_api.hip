@@ -26,6 +26,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -143,6 +146,7 @@ struct ela_handle {
   int64_t born_segment = 0; // Born adjoint storage: 0 auto, -1 all samples, K > 0 checkpoints every K
   int64_t born_budget = 0;  // device-memory budget of the automatic plan [bytes], 0: free memory
   int64_t born_last_segment = 0;  // plan of the last Born adjoint (-1 full storage)
+  int born_physics = 0;     // Born yz term: 0 the reference's xz pattern, 1 the yz pattern (ela_set_born_mode)
   // adjoint-state gradient
   DevBuf ck1;               // outer checkpoints [n1][18][interior] (bfloat16, block-scaled int16 or native)
   DevBuf ck2;               // inner checkpoints of one outer segment [K1/K][18][interior] (native)
@@ -153,7 +157,7 @@ struct ela_handle {
   DevBuf stage2;            // host-call staging of the source-trace gradient
   int64_t grad_interval = 20;   // inner interval K (fine steps)
   int64_t grad_outer = 0;       // outer interval K1 (0: automatic)
-  int grad_fmt = ELA_CK_BF16;   // storage of the outer checkpoints
+  int grad_fmt = ELA_CK_S16;    // storage of the outer checkpoints (int16 block-scaled: inside 1e-3 of FP64)
   int64_t grad_budget = 0;      // device-memory budget of the automatic plan (0: free memory)
   bool grad_ready = false;      // a gradient forward has stored its checkpoints
   int64_t plan_K = 0, plan_K1 = 0, plan_n1 = 0;
@@ -768,6 +772,7 @@ ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* Vo, const T* Vc, const 
   set_coefs<T>(h, a.cx, a.cy, a.cz);
   a.dts_inv = T(1.0 / (2.0 * h->cfg.dts));
   a.its = (int)its; a.nts = (int)h->cfg.nts;
+  a.physics = h->born_physics;
   a.nz = (int)h->cfg.nz; a.nx = (int)h->cfg.nx; a.ny = (int)h->cfg.ny;
   a.nzi = (int)h->nzi; a.zoff = (int)h->zoff;
   a.ys = h->cfg.nx * h->nzi;
@@ -1090,7 +1095,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
 
 struct GradPlan {
   int64_t K = 20, K1 = 20, n1 = 0, n2 = 1;
-  int fmt = ELA_CK_BF16;
+  int fmt = ELA_CK_S16;
   size_t ck1_bytes = 0, ck2_bytes = 0;  // one checkpoint
 };
 
@@ -1472,6 +1477,25 @@ int set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem) {
   return ELA_OK;
 }
 
+namespace {
+
+// Every extern "C" entry runs inside guarded(): no C++ exception (a failed
+// host allocation in a std::vector, a handle, a chunk table) crosses the C ABI.
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(ELA_ERR_MEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ELA_ERR_ARG, "%s", e.what());
+  } catch (...) {
+    return fail(ELA_ERR_ARG, "unexpected exception");
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* ela_last_error(void) { return g_err.c_str(); }
@@ -1479,324 +1503,385 @@ const char* ela_last_error(void) { return g_err.c_str(); }
 const char* ela_version(void) { return "elastic_iso_3d gfx950 0.2"; }
 
 int ela_create(const ela_config* cfg, ela_handle** out) {
-  if (!out) return fail(ELA_ERR_ARG, "null output handle");
-  *out = nullptr;
-  CHK(check_cfg(cfg));
-  int ndev = 0;
-  HIPCHK(hipGetDeviceCount(&ndev));
-  if (cfg->device < 0 || cfg->device >= ndev)
-    return fail(ELA_ERR_ARG, "device %d not present (%d devices)", cfg->device, ndev);
-  DeviceGuard dg(cfg->device);
-  ela_handle* h = new ela_handle();
-  h->cfg = *cfg;
-  h->esize = cfg->precision;
-  h->n_model = cfg->nz * cfg->nx * cfg->ny;
-  const int64_t line = 128 / h->esize;  // elements per 128-byte cache line
-  h->zoff = line - ela::FAT;
-  h->nzi = round_up(h->zoff + cfg->nz, line);
-  h->n_int = h->nzi * cfg->nx * cfg->ny;
-  // each array carries a guard past its last plane for the stencils' unpredicated loads
-  h->stride = round_up(h->n_int + ela::GUARD_ROWS * h->nzi + ela::GUARD_EXTRA, 64);
-  auto bail = [&](int rc) {
-    delete h;
-    return rc;
-  };
-  if ((uint64_t)h->stride * (uint64_t)h->esize >= (1ull << 32))
-    return bail(fail(ELA_ERR_ARG, "grid of %lld cells exceeds 4 GiB per array (32-bit offsets)", (long long)h->n_model));
-  // a blocking stream: it orders with HIP's null stream, on which callers
-  // (e.g. torch's default stream) produce inputs and consume outputs
-  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamDefault) != hipSuccess)
-    return bail(fail(ELA_ERR_HIP, "stream create failed"));
-  h->stream = h->own_stream;
-  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-    return bail(fail(ELA_ERR_HIP, "event create failed"));
-  int rc = h->wave.alloc(18 * h->stride * h->esize);
-  if (rc) return bail(rc);
-  rc = h->med.alloc(8 * h->stride * h->esize);
-  if (rc) return bail(rc);
-  if (hipMemset(h->med.p, 0, 8 * h->stride * h->esize) != hipSuccess) return bail(fail(ELA_ERR_HIP, "medium clear"));
-  // time filter (nonlinearPropElasticGpuFunctions_3D.cu:224-241) and taper (:248-255)
-  const int64_t sub = cfg->sub, hh = sub + 1, ntw = (cfg->nts - 1) * sub + 1;
-  h->filt.resize(2 * hh);
-  const double sc = 1.0 / std::sqrt(double(ntw) / double(cfg->nts));
-  for (int64_t i = 0; i < hh; ++i) {
-    const double a = 1.0 - double(i) / double(sub);
-    h->filt[i] = a * sc;
-    h->filt[hh + i] = (1.0 - a) * sc;
-  }
-  const int64_t mp = cfg->min_pad;
-  if (mp > 0) {
-    std::vector<double> tap(mp);
-    for (int64_t d = 0; d < mp; ++d)
-      tap[d] = cfg->alpha_cos + (1.0 - cfg->alpha_cos) * std::cos(M_PI / double(mp) * double(mp - d));
-    rc = h->taper.alloc(mp * h->esize);
+  return guarded([&]() -> int {
+    if (!out) return fail(ELA_ERR_ARG, "null output handle");
+    *out = nullptr;
+    CHK(check_cfg(cfg));
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (cfg->device < 0 || cfg->device >= ndev)
+      return fail(ELA_ERR_ARG, "device %d not present (%d devices)", cfg->device, ndev);
+    DeviceGuard dg(cfg->device);
+    std::unique_ptr<ela_handle> owner(new ela_handle());   // freed on every error return and on a throw
+    ela_handle* h = owner.get();
+    h->cfg = *cfg;
+    h->esize = cfg->precision;
+    h->n_model = cfg->nz * cfg->nx * cfg->ny;
+    const int64_t line = 128 / h->esize;  // elements per 128-byte cache line
+    h->zoff = line - ela::FAT;
+    h->nzi = round_up(h->zoff + cfg->nz, line);
+    h->n_int = h->nzi * cfg->nx * cfg->ny;
+    // each array carries a guard past its last plane for the stencils' unpredicated loads
+    h->stride = round_up(h->n_int + ela::GUARD_ROWS * h->nzi + ela::GUARD_EXTRA, 64);
+    auto bail = [&](int rc) {   // the handle's buffers go with `owner`; its stream and events here
+      if (h->ev0) (void)hipEventDestroy(h->ev0);
+      if (h->ev1) (void)hipEventDestroy(h->ev1);
+      if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+      return rc;
+    };
+    if ((uint64_t)h->stride * (uint64_t)h->esize >= (1ull << 32))
+      return bail(fail(ELA_ERR_ARG, "grid of %lld cells exceeds 4 GiB per array (32-bit offsets)", (long long)h->n_model));
+    // a blocking stream: it orders with HIP's null stream, on which callers
+    // (e.g. torch's default stream) produce inputs and consume outputs
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamDefault) != hipSuccess)
+      return bail(fail(ELA_ERR_HIP, "stream create failed"));
+    h->stream = h->own_stream;
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+      return bail(fail(ELA_ERR_HIP, "event create failed"));
+    int rc = h->wave.alloc(18 * h->stride * h->esize);
     if (rc) return bail(rc);
-    hipError_t e;
-    if (h->esize == 4) {
-      std::vector<float> tf(tap.begin(), tap.end());
-      e = hipMemcpy(h->taper.p, tf.data(), mp * 4, hipMemcpyHostToDevice);
-    } else {
-      e = hipMemcpy(h->taper.p, tap.data(), mp * 8, hipMemcpyHostToDevice);
+    rc = h->med.alloc(8 * h->stride * h->esize);
+    if (rc) return bail(rc);
+    if (hipMemset(h->med.p, 0, 8 * h->stride * h->esize) != hipSuccess) return bail(fail(ELA_ERR_HIP, "medium clear"));
+    // time filter (nonlinearPropElasticGpuFunctions_3D.cu:224-241) and taper (:248-255)
+    const int64_t sub = cfg->sub, hh = sub + 1, ntw = (cfg->nts - 1) * sub + 1;
+    h->filt.resize(2 * hh);
+    const double sc = 1.0 / std::sqrt(double(ntw) / double(cfg->nts));
+    for (int64_t i = 0; i < hh; ++i) {
+      const double a = 1.0 - double(i) / double(sub);
+      h->filt[i] = a * sc;
+      h->filt[hh + i] = (1.0 - a) * sc;
     }
-    if (e != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy failed"));
-  }
-  h->by_x = cfg->precision == ELA_FP64 ? 8 : 16;
-  *out = h;
-  return ELA_OK;
+    const int64_t mp = cfg->min_pad;
+    if (mp > 0) {
+      std::vector<double> tap(mp);
+      for (int64_t d = 0; d < mp; ++d)
+        tap[d] = cfg->alpha_cos + (1.0 - cfg->alpha_cos) * std::cos(M_PI / double(mp) * double(mp - d));
+      rc = h->taper.alloc(mp * h->esize);
+      if (rc) return bail(rc);
+      hipError_t e;
+      if (h->esize == 4) {
+        std::vector<float> tf(tap.begin(), tap.end());
+        e = hipMemcpy(h->taper.p, tf.data(), mp * 4, hipMemcpyHostToDevice);
+      } else {
+        e = hipMemcpy(h->taper.p, tap.data(), mp * 8, hipMemcpyHostToDevice);
+      }
+      if (e != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy failed"));
+    }
+    h->by_x = cfg->precision == ELA_FP64 ? 8 : 16;
+    *out = owner.release();
+    return ELA_OK;
+  });
 }
 
 int ela_destroy(ela_handle* h) {
-  if (!h) return ELA_OK;
-  DeviceGuard dg(h->cfg.device);
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
-  (void)release_exec(h);
-  if (h->nl_exec_done) (void)hipEventDestroy(h->nl_exec_done);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
-  if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
-  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
-  delete h;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return ELA_OK;
+    DeviceGuard dg(h->cfg.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    (void)release_exec(h);
+    if (h->nl_exec_done) (void)hipEventDestroy(h->nl_exec_done);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    delete h;
+    return ELA_OK;
+  });
 }
 
 int ela_set_stream(ela_handle* h, void* s) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  h->stream = s ? (hipStream_t)s : h->own_stream;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    h->stream = s ? (hipStream_t)s : h->own_stream;
+    return ELA_OK;
+  });
 }
 
 int ela_synchronize(ela_handle* h) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  DeviceGuard dg(h->cfg.device);
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    DeviceGuard dg(h->cfg.device);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ELA_OK;
+  });
 }
 
 int ela_set_medium(ela_handle* h, const void* med8, int mem) {
-  if (!h || !med8) return fail(ELA_ERR_ARG, "null argument");
-  DeviceGuard dg(h->cfg.device);
-  CHK(copy_in_3d(h, h->med.p, med8, 8, mem));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  h->have_medium = true;
-  h->have_elastic = false;   // the steppers read these 8 arrays
-  h->grad_ready = false;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h || !med8) return fail(ELA_ERR_ARG, "null argument");
+    DeviceGuard dg(h->cfg.device);
+    CHK(copy_in_3d(h, h->med.p, med8, 8, mem));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_medium = true;
+    h->have_elastic = false;   // the steppers read these 8 arrays
+    h->grad_ready = false;
+    return ELA_OK;
+  });
 }
 
 int ela_set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem) {
-  if (!h || !rlm3) return fail(ELA_ERR_ARG, "null argument");
-  if (!(dtw > 0)) return fail(ELA_ERR_ARG, "dtw must be positive");
-  DeviceGuard dg(h->cfg.device);
-  h->have_medium = h->have_elastic = false;
-  CHK(h->esize == 4 ? set_elastic<float>(h, rlm3, dtw, mem) : set_elastic<double>(h, rlm3, dtw, mem));
-  h->have_medium = h->have_elastic = true;
-  h->grad_ready = false;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h || !rlm3) return fail(ELA_ERR_ARG, "null argument");
+    if (!(dtw > 0)) return fail(ELA_ERR_ARG, "dtw must be positive");
+    DeviceGuard dg(h->cfg.device);
+    h->have_medium = h->have_elastic = false;
+    CHK(h->esize == 4 ? set_elastic<float>(h, rlm3, dtw, mem) : set_elastic<double>(h, rlm3, dtw, mem));
+    h->have_medium = h->have_elastic = true;
+    h->grad_ready = false;
+    return ELA_OK;
+  });
 }
 
 int ela_nl_forward(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp, ela_traces* rt,
                    int mem) {
-  if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
-  DeviceGuard dg(h->cfg.device);
-  return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, false, mem) : nl_run<double>(h, sp, st, rp, rt, false, mem);
+  return guarded([&]() -> int {
+    if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, false, mem) : nl_run<double>(h, sp, st, rp, rt, false, mem);
+  });
 }
 
 int ela_nl_adjoint(ela_handle* h, const ela_points* sp, ela_traces* st, const ela_points* rp, const ela_traces* rt,
                    int mem) {
-  if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
-  DeviceGuard dg(h->cfg.device);
-  return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, true, mem) : nl_run<double>(h, sp, st, rp, rt, true, mem);
+  return guarded([&]() -> int {
+    if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, true, mem) : nl_run<double>(h, sp, st, rp, rt, true, mem);
+  });
 }
 
 int ela_born_forward(ela_handle* h, const ela_points* sp, const ela_traces* st, const void* dmod8,
                      const ela_points* rp, ela_traces* rt, int mem) {
-  if (!h || !st || !rt || !dmod8) return fail(ELA_ERR_ARG, "null argument");
-  DeviceGuard dg(h->cfg.device);
-  return h->esize == 4 ? born_run<float>(h, sp, st, dmod8, nullptr, rp, rt, false, mem)
-                       : born_run<double>(h, sp, st, dmod8, nullptr, rp, rt, false, mem);
+  return guarded([&]() -> int {
+    if (!h || !st || !rt || !dmod8) return fail(ELA_ERR_ARG, "null argument");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? born_run<float>(h, sp, st, dmod8, nullptr, rp, rt, false, mem)
+                         : born_run<double>(h, sp, st, dmod8, nullptr, rp, rt, false, mem);
+  });
 }
 
 int ela_born_adjoint(ela_handle* h, const ela_points* sp, const ela_traces* st, void* dmod8, const ela_points* rp,
                      const ela_traces* rt, int mem) {
-  if (!h || !st || !rt || !dmod8) return fail(ELA_ERR_ARG, "null argument");
-  DeviceGuard dg(h->cfg.device);
-  return h->esize == 4 ? born_run<float>(h, sp, st, nullptr, dmod8, rp, rt, true, mem)
-                       : born_run<double>(h, sp, st, nullptr, dmod8, rp, rt, true, mem);
+  return guarded([&]() -> int {
+    if (!h || !st || !rt || !dmod8) return fail(ELA_ERR_ARG, "null argument");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? born_run<float>(h, sp, st, nullptr, dmod8, rp, rt, true, mem)
+                         : born_run<double>(h, sp, st, nullptr, dmod8, rp, rt, true, mem);
+  });
 }
 
 int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_interval, int format,
                                  int64_t budget_bytes) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  if (interval < 1) return fail(ELA_ERR_ARG, "checkpoint interval must be >= 1");
-  if (outer_interval < 0 || (outer_interval > 0 && outer_interval % interval))
-    return fail(ELA_ERR_ARG, "outer interval must be 0 (automatic) or a multiple of the interval");
-  if (budget_bytes < 0) return fail(ELA_ERR_ARG, "negative memory budget");
-  if (format != ELA_CK_NATIVE && format != ELA_CK_BF16 && format != ELA_CK_S16)
-    return fail(ELA_ERR_ARG, "checkpoint format must be ELA_CK_NATIVE, ELA_CK_BF16 or ELA_CK_S16");
-  h->grad_interval = interval;
-  h->grad_outer = outer_interval;
-  h->grad_fmt = format;
-  h->grad_budget = budget_bytes;
-  h->grad_ready = false;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (interval < 1) return fail(ELA_ERR_ARG, "checkpoint interval must be >= 1");
+    if (outer_interval < 0 || (outer_interval > 0 && outer_interval % interval))
+      return fail(ELA_ERR_ARG, "outer interval must be 0 (automatic) or a multiple of the interval");
+    if (budget_bytes < 0) return fail(ELA_ERR_ARG, "negative memory budget");
+    if (format != ELA_CK_NATIVE && format != ELA_CK_BF16 && format != ELA_CK_S16)
+      return fail(ELA_ERR_ARG, "checkpoint format must be ELA_CK_NATIVE, ELA_CK_BF16 or ELA_CK_S16");
+    h->grad_interval = interval;
+    h->grad_outer = outer_interval;
+    h->grad_fmt = format;
+    h->grad_budget = budget_bytes;
+    h->grad_ready = false;
+    return ELA_OK;
+  });
 }
 
 int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* format,
                            int64_t* checkpoint_bytes) {
-  if (!h || !interval || !outer_interval || !format || !checkpoint_bytes) return fail(ELA_ERR_ARG, "null argument");
-  *interval = h->plan_K;
-  *outer_interval = h->plan_K1;
-  *format = h->plan_fmt;
-  *checkpoint_bytes = (int64_t)h->ck1.bytes;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h || !interval || !outer_interval || !format || !checkpoint_bytes) return fail(ELA_ERR_ARG, "null argument");
+    *interval = h->plan_K;
+    *outer_interval = h->plan_K1;
+    *format = h->plan_fmt;
+    *checkpoint_bytes = (int64_t)h->ck1.bytes;
+    return ELA_OK;
+  });
 }
 
 int ela_gradient_forward(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp,
                          ela_traces* rt, int mem) {
-  if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
-  DeviceGuard dg(h->cfg.device);
-  return h->esize == 4 ? grad_forward<float>(h, sp, st, rp, rt, mem) : grad_forward<double>(h, sp, st, rp, rt, mem);
+  return guarded([&]() -> int {
+    if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? grad_forward<float>(h, sp, st, rp, rt, mem) : grad_forward<double>(h, sp, st, rp, rt, mem);
+  });
 }
 
 int ela_gradient_backward(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp,
                           const ela_traces* adj, double* grad8, ela_traces* src_grad, int mem) {
-  if (!h || !st || !adj || !grad8 || !src_grad) return fail(ELA_ERR_ARG, "null argument");
-  DeviceGuard dg(h->cfg.device);
-  return h->esize == 4 ? grad_backward<float>(h, sp, st, rp, adj, grad8, src_grad, mem)
-                       : grad_backward<double>(h, sp, st, rp, adj, grad8, src_grad, mem);
+  return guarded([&]() -> int {
+    if (!h || !st || !adj || !grad8 || !src_grad) return fail(ELA_ERR_ARG, "null argument");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? grad_backward<float>(h, sp, st, rp, adj, grad8, src_grad, mem)
+                         : grad_backward<double>(h, sp, st, rp, adj, grad8, src_grad, mem);
+  });
 }
 
 int ela_set_windows(ela_handle* h, int n, const int64_t* y0, const int64_t* ny, const int64_t* nx) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  if (n < 0 || (n > 0 && (!y0 || !ny || !nx))) return fail(ELA_ERR_ARG, "bad window list");
-  DeviceGuard dg(h->cfg.device);
-  h->win_y0.clear(); h->win_ny.clear(); h->win_nx.clear();
-  h->chunks_yc = 0;
-  h->grad_ready = false;
-  if (n == 0) return ELA_OK;
-  const int64_t NY = h->cfg.ny;
-  std::vector<int> tab(4 * NY, 0);
-  int64_t prev_end = 0;
-  for (int w = 0; w < n; ++w) {
-    if (y0[w] < prev_end || ny[w] < 2 * ela::FAT + 1 || y0[w] + ny[w] > NY || nx[w] < 2 * ela::FAT + 1 ||
-        nx[w] > h->cfg.nx)
-      return fail(ELA_ERR_ARG, "window %d (y0 %lld, ny %lld, nx %lld) overlaps, is too small or leaves the grid", w,
-                  (long long)y0[w], (long long)ny[w], (long long)nx[w]);
-    if (std::min(nx[w], h->cfg.nz) / 2 < h->cfg.min_pad || ny[w] / 2 < h->cfg.min_pad)
-      return fail(ELA_ERR_ARG, "window %d is narrower than twice the sponge", w);
-    for (int64_t y = y0[w]; y < y0[w] + ny[w]; ++y) {
-      tab[4 * y] = (int)y0[w];
-      tab[4 * y + 1] = (int)ny[w];
-      tab[4 * y + 2] = (int)nx[w];
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (n < 0 || (n > 0 && (!y0 || !ny || !nx))) return fail(ELA_ERR_ARG, "bad window list");
+    DeviceGuard dg(h->cfg.device);
+    h->win_y0.clear(); h->win_ny.clear(); h->win_nx.clear();
+    h->chunks_yc = 0;
+    h->grad_ready = false;
+    if (n == 0) return ELA_OK;
+    const int64_t NY = h->cfg.ny;
+    std::vector<int> tab(4 * NY, 0);
+    int64_t prev_end = 0;
+    for (int w = 0; w < n; ++w) {
+      if (y0[w] < prev_end || ny[w] < 2 * ela::FAT + 1 || y0[w] + ny[w] > NY || nx[w] < 2 * ela::FAT + 1 ||
+          nx[w] > h->cfg.nx)
+        return fail(ELA_ERR_ARG, "window %d (y0 %lld, ny %lld, nx %lld) overlaps, is too small or leaves the grid", w,
+                    (long long)y0[w], (long long)ny[w], (long long)nx[w]);
+      if (std::min(nx[w], h->cfg.nz) / 2 < h->cfg.min_pad || ny[w] / 2 < h->cfg.min_pad)
+        return fail(ELA_ERR_ARG, "window %d is narrower than twice the sponge", w);
+      for (int64_t y = y0[w]; y < y0[w] + ny[w]; ++y) {
+        tab[4 * y] = (int)y0[w];
+        tab[4 * y + 1] = (int)ny[w];
+        tab[4 * y + 2] = (int)nx[w];
+      }
+      prev_end = y0[w] + ny[w];
+      h->win_y0.push_back(y0[w]); h->win_ny.push_back(ny[w]); h->win_nx.push_back(nx[w]);
     }
-    prev_end = y0[w] + ny[w];
-    h->win_y0.push_back(y0[w]); h->win_ny.push_back(ny[w]); h->win_nx.push_back(nx[w]);
-  }
-  CHK(h->ytab.alloc(tab.size() * sizeof(int)));
-  HIPCHK(hipMemcpy(h->ytab.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
-  CHK(ensure_chunks(h));
-  return ELA_OK;
+    CHK(h->ytab.alloc(tab.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(h->ytab.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+    CHK(ensure_chunks(h));
+    return ELA_OK;
+  });
 }
 
 int ela_set_graph(ela_handle* h, int on) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  h->use_graph = on != 0;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    h->use_graph = on != 0;
+    return ELA_OK;
+  });
 }
 
 int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  if (tile_x != 4 && tile_x != 8 && tile_x != 16) return fail(ELA_ERR_ARG, "tile_x must be 4, 8 or 16");
-  if (y_chunk < 1) return fail(ELA_ERR_ARG, "y_chunk must be >= 1");
-  h->by_x = tile_x;
-  h->ychunk = y_chunk;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (tile_x != 4 && tile_x != 8 && tile_x != 16) return fail(ELA_ERR_ARG, "tile_x must be 4, 8 or 16");
+    if (y_chunk < 1) return fail(ELA_ERR_ARG, "y_chunk must be >= 1");
+    h->by_x = tile_x;
+    h->ychunk = y_chunk;
+    return ELA_OK;
+  });
 }
 
 int ela_set_tile_order(ela_handle* h, int order, int x_group) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  if (order < 0 || order > 3) return fail(ELA_ERR_ARG, "tile order must be 0, 1, 2 or 3");
-  if (x_group < 1) return fail(ELA_ERR_ARG, "x_group must be >= 1");
-  h->remap = order;
-  h->xgroup = x_group;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (order < 0 || order > 3) return fail(ELA_ERR_ARG, "tile order must be 0, 1, 2 or 3");
+    if (x_group < 1) return fail(ELA_ERR_ARG, "x_group must be >= 1");
+    h->remap = order;
+    h->xgroup = x_group;
+    return ELA_OK;
+  });
 }
 
 int ela_set_born_storage(ela_handle* h, int64_t segment, int64_t budget_bytes) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  if (segment < -1) return fail(ELA_ERR_ARG, "segment must be -1, 0 or a positive number of coarse samples");
-  if (budget_bytes < 0) return fail(ELA_ERR_ARG, "negative memory budget");
-  h->born_segment = segment;
-  h->born_budget = budget_bytes;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (segment < -1) return fail(ELA_ERR_ARG, "segment must be -1, 0 or a positive number of coarse samples");
+    if (budget_bytes < 0) return fail(ELA_ERR_ARG, "negative memory budget");
+    h->born_segment = segment;
+    h->born_budget = budget_bytes;
+    return ELA_OK;
+  });
+}
+
+int ela_set_born_mode(ela_handle* h, int mode) {
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (mode != ELA_BORN_REFERENCE && mode != ELA_BORN_PHYSICS)
+      return fail(ELA_ERR_ARG, "Born mode must be ELA_BORN_REFERENCE or ELA_BORN_PHYSICS");
+    h->born_physics = mode == ELA_BORN_PHYSICS;
+    return ELA_OK;
+  });
 }
 
 int ela_born_storage_used(const ela_handle* h, int64_t* segment) {
-  if (!h || !segment) return fail(ELA_ERR_ARG, "null argument");
-  *segment = h->born_last_segment;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h || !segment) return fail(ELA_ERR_ARG, "null argument");
+    *segment = h->born_last_segment;
+    return ELA_OK;
+  });
 }
 
 int ela_get_tiling(const ela_handle* h, int* tile_x, int* y_chunk) {
-  if (!h || !tile_x || !y_chunk) return fail(ELA_ERR_ARG, "null argument");
-  *tile_x = h->by_x;
-  *y_chunk = h->ychunk;
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h || !tile_x || !y_chunk) return fail(ELA_ERR_ARG, "null argument");
+    *tile_x = h->by_x;
+    *y_chunk = h->ychunk;
+    return ELA_OK;
+  });
 }
 
 int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, double* gbs) {
-  if (!h || !gbs || reps < 1 || n < 4 || n % 4) return fail(ELA_ERR_ARG, "bad argument");
-  void (*kern)(float4* const*, int64_t) = nullptr;
-  if (reads == 1 && writes == 1) kern = ela::k_stream<1, 1>;
-  else if (reads == 12 && writes == 3) kern = ela::k_stream<12, 3>;
-  else if (reads == 14 && writes == 6) kern = ela::k_stream<14, 6>;
-  else if (reads == 10 && writes == 3) kern = ela::k_stream<10, 3>;
-  else if (reads == 11 && writes == 6) kern = ela::k_stream<11, 6>;
-  else if (reads == 12 && writes == 9) kern = ela::k_stream<12, 9>;
-  if (!kern) return fail(ELA_ERR_ARG, "stream probe mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9)");
-  DeviceGuard dg(h->cfg.device);
-  DevBuf buf, tab;
-  CHK(buf.alloc((size_t)reads * n * 4));
-  CHK(tab.alloc(sizeof(float4*) * 16));
-  float4* ptrs[16];
-  for (int k = 0; k < reads; ++k) ptrs[k] = reinterpret_cast<float4*>(static_cast<char*>(buf.p) + (size_t)k * n * 4);
-  HIPCHK(hipMemcpyAsync(tab.p, ptrs, sizeof(float4*) * reads, hipMemcpyHostToDevice, h->stream));
-  CHK(fill<float>(h, static_cast<float*>(buf.p), (int64_t)reads * n));
-  const int64_t n4 = n / 4;
-  // enough 256-thread blocks for about 4 vectors per thread
-  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n4 + 1023) / 1024, 1 << 20));
-  float4* const* t = static_cast<float4* const*>(tab.p);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
-  HIPCHK(hipEventRecord(h->ev0, h->stream));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
-  HIPCHK(hipEventRecord(h->ev1, h->stream));
-  HIPCHK(hipEventSynchronize(h->ev1));
-  HIPCHK(hipGetLastError());
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  *gbs = double(reads + writes) * 4.0 * double(n) * reps / (double(ms) * 1e-3) / 1e9;
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h || !gbs || reps < 1 || n < 4 || n % 4) return fail(ELA_ERR_ARG, "bad argument");
+    void (*kern)(float4* const*, int64_t) = nullptr;
+    if (reads == 1 && writes == 1) kern = ela::k_stream<1, 1>;
+    else if (reads == 12 && writes == 3) kern = ela::k_stream<12, 3>;
+    else if (reads == 14 && writes == 6) kern = ela::k_stream<14, 6>;
+    else if (reads == 10 && writes == 3) kern = ela::k_stream<10, 3>;
+    else if (reads == 11 && writes == 6) kern = ela::k_stream<11, 6>;
+    else if (reads == 12 && writes == 9) kern = ela::k_stream<12, 9>;
+    if (!kern) return fail(ELA_ERR_ARG, "stream probe mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9)");
+    DeviceGuard dg(h->cfg.device);
+    DevBuf buf, tab;
+    CHK(buf.alloc((size_t)reads * n * 4));
+    CHK(tab.alloc(sizeof(float4*) * 16));
+    float4* ptrs[16];
+    for (int k = 0; k < reads; ++k) ptrs[k] = reinterpret_cast<float4*>(static_cast<char*>(buf.p) + (size_t)k * n * 4);
+    HIPCHK(hipMemcpyAsync(tab.p, ptrs, sizeof(float4*) * reads, hipMemcpyHostToDevice, h->stream));
+    CHK(fill<float>(h, static_cast<float*>(buf.p), (int64_t)reads * n));
+    const int64_t n4 = n / 4;
+    // enough 256-thread blocks for about 4 vectors per thread
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n4 + 1023) / 1024, 1 << 20));
+    float4* const* t = static_cast<float4* const*>(tab.p);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    HIPCHK(hipGetLastError());
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *gbs = double(reads + writes) * 4.0 * double(n) * reps / (double(ms) * 1e-3) / 1e9;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return ELA_OK;
+  });
 }
 
 int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* ms_stress) {
-  if (!h || reps < 1 || !ms_velocity || !ms_stress) return fail(ELA_ERR_ARG, "bad argument");
-  if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
-  DeviceGuard dg(h->cfg.device);
-  return h->esize == 4 ? time_kernels<float>(h, reps, ms_velocity, ms_stress)
-                       : time_kernels<double>(h, reps, ms_velocity, ms_stress);
+  return guarded([&]() -> int {
+    if (!h || reps < 1 || !ms_velocity || !ms_stress) return fail(ELA_ERR_ARG, "bad argument");
+    if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? time_kernels<float>(h, reps, ms_velocity, ms_stress)
+                         : time_kernels<double>(h, reps, ms_velocity, ms_stress);
+  });
 }
 
 int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_t* cells) {
-  if (!h) return fail(ELA_ERR_ARG, "null handle");
-  float t = 0.f;
-  DeviceGuard dg(h->cfg.device);
-  HIPCHK(hipEventSynchronize(h->ev1));
-  HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
-  if (ms) *ms = t;
-  if (fine_steps) *fine_steps = h->last_steps;
-  if (cells) *cells = interior_cells(h);
-  return ELA_OK;
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    float t = 0.f;
+    DeviceGuard dg(h->cfg.device);
+    HIPCHK(hipEventSynchronize(h->ev1));
+    HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    if (ms) *ms = t;
+    if (fine_steps) *fine_steps = h->last_steps;
+    if (cells) *cells = interior_cells(h);
+    return ELA_OK;
+  });
 }
 
 }  // extern "C"

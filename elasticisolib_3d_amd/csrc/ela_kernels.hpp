@@ -996,6 +996,7 @@ struct ImagingArgs {
   T cx[4], cy[4], cz[4];
   T dts_inv;
   int its, nts;
+  int physics;      // 0: the yz term with the xz derivative pattern (reference); 1: the yz pattern
   int nz, nx, ny, nzi, zoff;
   int64_t ys;
   int ychunk, nbz, nbx, nby;
@@ -1056,6 +1057,10 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
       const T ezz = dpz<T, BX>(l_vz, r, q, cz);
       const T gxy = dq(qx, cy) + dmx<T, BX>(l_vy, r, q, cx);
       const T gxz = dmz<T, BX>(l_vx, r, q, cz) + dmx<T, BX>(l_vz, r, q, cx);
+      // yz term: the reference uses the xz derivative pattern
+      // (kernelsElaGpu_3D.cu:1419-1431, :1622-1634); physics mode the yz one
+      // of the stepper's syz update (D-y vz + D-z vy)
+      const T gyz = a.physics ? dq(qz, cy) + dmz<T, BX>(l_vy, r, q, cz) : gxz;
       T dv[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -1077,8 +1082,7 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
         a.ss[SZZ][i] = dm[3] * div + T(2.0) * dm[4] * ezz;
         a.ss[SXZ][i] = dm[5] * gxz;
         a.ss[SXY][i] = dm[6] * gxy;
-        // reference behaviour: the yz term uses the xz derivative pattern
-        a.ss[SYZ][i] = dm[7] * gxz;
+        a.ss[SYZ][i] = dm[7] * gyz;
       } else {
         // each coarse sample's term in T, the sum over samples in FP64
         T ss[9];
@@ -1094,7 +1098,7 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
         a.dmo[4][i] = dmo[4] + double(T(2.0) * (exx * sxx + eyy * syy + ezz * szz));
         a.dmo[5][i] = dmo[5] + double(ss[SXZ] * gxz);
         a.dmo[6][i] = dmo[6] + double(ss[SXY] * gxy);
-        a.dmo[7][i] = dmo[7] + double(ss[SYZ] * gxz);
+        a.dmo[7][i] = dmo[7] + double(ss[SYZ] * gyz);
       }
     }
 #pragma unroll

@@ -630,6 +630,10 @@ class BornElasticShotsGpu_3D(_ShotsBase):
                      sourcesVectorXZGrid, sourcesVectorXYGrid, sourcesVectorYZGrid),
                     (recVectorCenterGrid, recVectorXGrid, recVectorYGrid, recVectorZGrid, recVectorXZGrid,
                      recVectorXYGrid, recVectorYZGrid))
+        # par key bornPhysics=1 (not in the reference, default 0): the yz shear
+        # term with the yz derivative pattern instead of the reference's xz one
+        # (kernelsElaGpu_3D.cu:1419-1431), the true linearisation in mu
+        self.prop.set_born_mode(paramP.getInt("bornPhysics", 0) != 0)
 
     def _signal(self, i: int) -> np.ndarray:
         s = self.signals[0] if len(self.signals) == 1 else self.signals[i]

@@ -275,12 +275,13 @@ class Propagator:
         self._keep.clear()
         return grad8, gs
 
-    def set_gradient_checkpoints(self, interval: int = 20, outer_interval: int = 0, checkpoint="bf16",
+    def set_gradient_checkpoints(self, interval: int = 20, outer_interval: int = 0, checkpoint="s16",
                                  budget_bytes: int = 0) -> None:
-        """`checkpoint`: storage of the outer checkpoints, "bf16" (bfloat16,
-        one power-of-two scale per array and snapshot), "s16" (int16, one
-        scale per 64 values) or "native" (exact); True / False mean "bf16" /
-        "native"."""
+        """`checkpoint`: storage of the outer checkpoints, "s16" (int16, one
+        scale per 64 values; the default, the format that keeps the FP32
+        gradient inside 1e-3 of FP64), "bf16" (bfloat16, one power-of-two
+        scale per array and snapshot) or "native" (exact); True / False mean
+        "bf16" / "native"."""
         check(self.lib.ela_set_gradient_checkpoints(self.h, int(interval), int(outer_interval),
                                                     checkpoint_format(checkpoint), int(budget_bytes)),
               "ela_set_gradient_checkpoints")
@@ -320,6 +321,12 @@ class Propagator:
         budget_bytes, 0 = free device memory), -1 every coarse sample kept,
         K > 0 exact checkpoints every K coarse samples with recomputation."""
         check(self.lib.ela_set_born_storage(self.h, int(segment), int(budget_bytes)), "ela_set_born_storage")
+
+    def set_born_mode(self, physics: bool) -> None:
+        """Yz shear term of the Born operators: False (default) keeps the
+        reference's xz derivative pattern (kernelsElaGpu_3D.cu:1419-1431,
+        :1622-1634); True uses the yz pattern, the true linearisation in mu."""
+        check(self.lib.ela_set_born_mode(self.h, 1 if physics else 0), "ela_set_born_mode")
 
     def born_storage_used(self) -> int:
         """Plan of the last Born adjoint: -1 all samples kept, else the segment length."""

@@ -5,9 +5,13 @@ Written with array slicing instead of per-cell loops so that an indexing slip
 in ``ela_oracle.cpp`` and in this file are unlikely to coincide. It follows
 the same reference files:
 elastic_iso_float/src/kernelsElaGpu_3D.cu:110-1226 (steppers, damping),
-elastic_iso_float/src/kernelsElaGpu_3D.cu:11-104 (injection/recording) and
+elastic_iso_float/src/kernelsElaGpu_3D.cu:11-104 (injection/recording),
+elastic_iso_float/src/kernelsElaGpu_3D.cu:1232-1714 (Born imaging, coarse-time
+interpolation, secondary-source injection, adjoint extraction),
 elastic_iso_float/src/nonlinearPropElasticGpuFunctions_3D.cu:909-925, 1330-1345
-(time loops). Arrays are [ny, nx, nz].
+(nonlinear time loops) and
+elastic_iso_float/src/BornElasticGpuFunctions_3D.cu:1268-1407, 2147-2281 (Born
+time loops). Arrays are [ny, nx, nz].
 """
 from __future__ import annotations
 
@@ -140,3 +144,149 @@ def nl_forward(g, med, src, spos, rpos):
                 rec[c][:, its + 1] += v * f[h + it2]
             p0, p1 = p1, p0
     return rec
+
+
+def _pad(inner):
+    """Interior values -> full-grid array (zero FAT layer)."""
+    out = np.zeros(tuple(n + 2 * FAT for n in inner.shape))
+    interior(out)[...] = inner
+    return out
+
+
+def source_wavefield(g, med, src, spos):
+    """Pass 1 of the Born operators: the background propagation with the
+    coarse-time hat accumulation of vx, vy, vz
+    (BornElasticGpuFunctions_3D.cu:1268-1316, kernelsElaGpu_3D.cu:1646-1672).
+    Returns V [nts, 3, ny, nx, nz]."""
+    shape = g.shape
+    w = damping_weights(shape, g.minPad, g.alphaCos)
+    f = time_filter(g.sub, g.nts)
+    h = g.sub + 1
+    p0 = np.zeros((9,) + shape)
+    p1 = np.zeros((9,) + shape)
+    V = np.zeros((g.nts, 3) + shape)
+    for its in range(g.nts - 1):
+        for it2 in range(1, g.sub + 1):
+            p0 = step(p0, p1, med, g.dz, g.dx, 1.0)
+            for c in range(9):
+                fl = p0[c].reshape(-1)
+                np.add.at(fl, spos[COMP_GRID[c]], src[c][:, its] * f[it2 - 1] + src[c][:, its + 1] * f[h + it2 - 1])
+            p0 *= w
+            p1 *= w
+            V[its] += p0[:3] * f[it2]
+            V[its + 1] += p0[:3] * f[h + it2]
+            p0, p1 = p1, p0
+    return V
+
+
+def _velocity_derivatives(Vc, dz, dx):
+    """Strain-rate combinations of one coarse velocity slice on the interior:
+    exx, eyy, ezz, gxz = D-z vx + D-x vz, gxy = D-y vx + D-x vy,
+    gyz = D-y vz + D-z vy (y coefficients over dx, as the reference)."""
+    vx, vy, vz = Vc
+    Y, X, Z = 0, 1, 2
+    exx = deriv(vx, X, True, dx)
+    eyy = deriv(vy, Y, True, dx)
+    ezz = deriv(vz, Z, True, dz)
+    gxz = deriv(vx, Z, False, dz) + deriv(vz, X, False, dx)
+    gxy = deriv(vx, Y, False, dx) + deriv(vy, X, False, dx)
+    gyz = deriv(vz, Y, False, dx) + deriv(vy, Z, False, dz)
+    return exx, eyy, ezz, gxz, gxy, gyz
+
+
+def _dv(V, its, nts):
+    """V[its-1] - V[its+1] with V outside [0, nts) taken as zero."""
+    lo = V[its - 1] if its >= 1 else 0.0
+    hi = V[its + 1] if its + 1 <= nts - 1 else 0.0
+    return lo - hi
+
+
+def imaging_forward(g, V, its, dm, physics=False):
+    """Secondary sources of coarse sample its (imagingElaFwdGpu_3D,
+    kernelsElaGpu_3D.cu:1232-1438): 9 full-grid arrays. In reference mode the
+    yz term uses the xz derivative pattern (:1419-1431)."""
+    dinv = 1.0 / (2.0 * g.dts)
+    exx, eyy, ezz, gxz, gxy, gyz = _velocity_derivatives(V[its], g.dz, g.dx)
+    D = [interior(d) for d in dm]
+    dv = _dv(V, its, g.nts)
+    dv = [interior(dv[k]) if not np.isscalar(dv) else 0.0 for k in range(3)]
+    div = exx + eyy + ezz
+    ss = [D[k] * dv[k] * dinv for k in range(3)]
+    ss += [D[3] * div + 2.0 * D[4] * exx, D[3] * div + 2.0 * D[4] * eyy, D[3] * div + 2.0 * D[4] * ezz,
+           D[5] * gxz, D[6] * gxy, D[7] * (gyz if physics else gxz)]
+    return np.stack([_pad(np.broadcast_to(a, exx.shape)) for a in ss])
+
+
+def imaging_adjoint(g, V, its, ss, physics=False):
+    """Images of coarse sample its (imagingElaAdjGpu_3D,
+    kernelsElaGpu_3D.cu:1440-1641): 8 full-grid arrays to add."""
+    dinv = 1.0 / (2.0 * g.dts)
+    exx, eyy, ezz, gxz, gxy, gyz = _velocity_derivatives(V[its], g.dz, g.dx)
+    S = [interior(a) for a in ss]
+    dv = _dv(V, its, g.nts)
+    dv = [interior(dv[k]) if not np.isscalar(dv) else 0.0 for k in range(3)]
+    img = [S[k] * dv[k] * dinv for k in range(3)]
+    img += [(S[3] + S[4] + S[5]) * (exx + eyy + ezz), 2.0 * (exx * S[3] + eyy * S[4] + ezz * S[5]),
+            S[6] * gxz, S[7] * gxy, S[8] * (gyz if physics else gxz)]
+    return np.stack([_pad(np.broadcast_to(a, exx.shape)) for a in img])
+
+
+def born_forward(g, med, src, spos, dm, rpos, physics=False):
+    """Born forward (BornElasticGpuFunctions_3D.cu:1226-1461): scattered
+    field driven by the secondary sources of two coarse samples per fine step,
+    recorded at the receivers."""
+    shape = g.shape
+    w = damping_weights(shape, g.minPad, g.alphaCos)
+    f = time_filter(g.sub, g.nts)
+    h = g.sub + 1
+    V = source_wavefield(g, med, src, spos)
+    p0 = np.zeros((9,) + shape)
+    p1 = np.zeros((9,) + shape)
+    rec = [np.zeros((len(rpos[COMP_GRID[c]]), g.nts)) for c in range(9)]
+    ssL = imaging_forward(g, V, 0, dm, physics)
+    for its in range(g.nts - 1):
+        ssR = imaging_forward(g, V, its + 1, dm, physics)
+        for it2 in range(1, g.sub + 1):
+            p0 = step(p0, p1, med, g.dz, g.dx, 1.0)
+            p0 += ssL * f[it2 - 1] + ssR * f[h + it2 - 1]
+            p0 *= w
+            p1 *= w
+            for c in range(9):
+                v = p0[c].reshape(-1)[rpos[COMP_GRID[c]]]
+                rec[c][:, its] += v * f[it2]
+                rec[c][:, its + 1] += v * f[h + it2]
+            p0, p1 = p1, p0
+        ssL = ssR
+    return rec
+
+
+def born_adjoint(g, med, src, spos, rec, rpos, physics=False):
+    """Born adjoint (BornElasticGpuFunctions_3D.cu:2103-2336): the transposed
+    stepper driven by the receiver traces, its coarse-time slices imaged
+    against the background V. Returns the 8 images."""
+    shape = g.shape
+    w = damping_weights(shape, g.minPad, g.alphaCos)
+    f = time_filter(g.sub, g.nts)
+    h = g.sub + 1
+    V = source_wavefield(g, med, src, spos)
+    p0 = np.zeros((9,) + shape)
+    p1 = np.zeros((9,) + shape)
+    img = np.zeros((8,) + shape)
+    ssL = np.zeros((9,) + shape)
+    ssR = np.zeros((9,) + shape)
+    for its in range(g.nts - 2, -1, -1):
+        for it2 in range(g.sub - 1, -1, -1):
+            p0 = step_born_adj(p0, p1, med, g.dz, g.dx)
+            for c in range(9):
+                fl = p0[c].reshape(-1)
+                np.add.at(fl, rpos[COMP_GRID[c]], rec[c][:, its] * f[it2 + 1] + rec[c][:, its + 1] * f[h + it2 + 1])
+            p0 *= w
+            p1 *= w
+            ssL += p0 * f[it2]
+            ssR += p0 * f[h + it2]
+            p0, p1 = p1, p0
+        img += imaging_adjoint(g, V, its + 1, ssR, physics)
+        ssR = ssL
+        ssL = np.zeros((9,) + shape)
+    img += imaging_adjoint(g, V, 0, ssR, physics)
+    return img

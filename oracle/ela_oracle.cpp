@@ -350,10 +350,22 @@ std::vector<T> born_source_wavefield(const Ctx<T>& C, const T* const* m, const P
 
 enum { DRX = 0, DRY, DRZ, DLA, DMU, DMXZ, DMXY, DMYZ };
 
+// The yz shear term of the Born secondary source and imaging. Reference mode
+// (physics = false) keeps the reference's xz derivative pattern
+// (kernelsElaGpu_3D.cu:1419-1431, :1622-1634); physics mode uses the yz
+// pattern of the stepper's syz update (kernelsElaGpu_3D.cu:441-453), the true
+// linearisation of the scheme. Nothing else differs between the two modes.
+template <typename T>
+inline T shear_yz(const T* cvx, const T* cvy, const T* cvz, int64_t i, int64_t xs, int64_t ys,
+                  const T* cx, const T* cy, const T* cz, bool physics) {
+  if (physics) return dminus(cvz, i, ys, cy) + dminus(cvy, i, 1, cz);
+  return dminus(cvx, i, 1, cz) + dminus(cvz, i, xs, cx);
+}
+
 // imagingElaFwdGpu_3D (kernelsElaGpu_3D.cu:1232-1438). o/n may be null.
 template <typename T>
 void imaging_fwd(const Ctx<T>& C, const T* o, const T* c, const T* nn, int64_t its,
-                 const T* const* dm, Fields<T>& ss) {
+                 const T* const* dm, Fields<T>& ss, bool physics) {
   const int64_t n = C.nModel, xs = C.xs, ys = C.ys;
   const T *cx = C.cx, *cy = C.cy, *cz = C.cz;
   const T *cvx = c, *cvy = c + n, *cvz = c + 2 * n;
@@ -376,16 +388,16 @@ void imaging_fwd(const Ctx<T>& C, const T* o, const T* c, const T* nn, int64_t i
     ss.f[SZZ][i] = dm[DLA][i] * div + T(2.0) * dm[DMU][i] * dvz_dz;
     ss.f[SXZ][i] = dm[DMXZ][i] * (dminus(cvx, i, 1, cz) + dminus(cvz, i, xs, cx));
     ss.f[SXY][i] = dm[DMXY][i] * (dminus(cvx, i, ys, cy) + dminus(cvy, i, xs, cx));
-    // Reference quirk: the yz secondary source uses the xz derivative pattern
-    // (kernelsElaGpu_3D.cu:1419-1431).
-    ss.f[SYZ][i] = dm[DMYZ][i] * (dminus(cvx, i, 1, cz) + dminus(cvz, i, xs, cx));
+    // Reference quirk (reference mode): the yz secondary source uses the xz
+    // derivative pattern (kernelsElaGpu_3D.cu:1419-1431).
+    ss.f[SYZ][i] = dm[DMYZ][i] * shear_yz(cvx, cvy, cvz, i, xs, ys, cx, cy, cz, physics);
   });
 }
 
 // imagingElaAdjGpu_3D (kernelsElaGpu_3D.cu:1440-1641): accumulates into dm.
 template <typename T>
 void imaging_adj(const Ctx<T>& C, const T* o, const T* c, const T* nn, int64_t its,
-                 T* const* dm, const Fields<T>& ss) {
+                 T* const* dm, const Fields<T>& ss, bool physics) {
   const int64_t n = C.nModel, xs = C.xs, ys = C.ys;
   const T *cx = C.cx, *cy = C.cy, *cz = C.cz;
   const T *cvx = c, *cvy = c + n, *cvz = c + 2 * n;
@@ -404,15 +416,15 @@ void imaging_adj(const Ctx<T>& C, const T* o, const T* c, const T* nn, int64_t i
     dm[DMU][i] += T(2.0) * (dvx_dx * ss.f[SXX][i] + dvy_dy * ss.f[SYY][i] + dvz_dz * ss.f[SZZ][i]);
     dm[DMXZ][i] += ss.f[SXZ][i] * (dminus(cvx, i, 1, cz) + dminus(cvz, i, xs, cx));
     dm[DMXY][i] += ss.f[SXY][i] * (dminus(cvx, i, ys, cy) + dminus(cvy, i, xs, cx));
-    // Same quirk as the forward imaging (kernelsElaGpu_3D.cu:1622-1634).
-    dm[DMYZ][i] += ss.f[SYZ][i] * (dminus(cvx, i, 1, cz) + dminus(cvz, i, xs, cx));
+    // Same quirk as the forward imaging in reference mode (kernelsElaGpu_3D.cu:1622-1634).
+    dm[DMYZ][i] += ss.f[SYZ][i] * shear_yz(cvx, cvy, cvz, i, xs, ys, cx, cy, cz, physics);
   });
 }
 
 // Born forward (BornElasticGpuFunctions_3D.cu:1226-1461).
 template <typename T>
 void born_forward(const Grid& g, const T* med, const T* src, const int64_t* spos, const int64_t* nsrc,
-                  const T* dmod, T* rec, const int64_t* rpos, const int64_t* nrec) {
+                  const T* dmod, T* rec, const int64_t* rpos, const int64_t* nrec, bool physics) {
   Ctx<T> C(g);
   const int64_t n = C.nModel, h = C.h();
   const T* m[8];
@@ -423,9 +435,9 @@ void born_forward(const Grid& g, const T* med, const T* src, const int64_t* spos
   std::vector<T> V = born_source_wavefield(C, m, S);
   auto Vs = [&](int64_t t) -> const T* { return (t < 0 || t >= g.nts) ? nullptr : V.data() + t * 3 * n; };
   Fields<T> p0(n), p1(n), ssL(n), ssR(n);
-  imaging_fwd(C, Vs(-1), Vs(0), Vs(1), 0, dm, ssL);
+  imaging_fwd(C, Vs(-1), Vs(0), Vs(1), 0, dm, ssL, physics);
   for (int64_t its = 0; its < g.nts - 1; ++its) {
-    imaging_fwd(C, Vs(its), Vs(its + 1), Vs(its + 2), its + 1, dm, ssR);
+    imaging_fwd(C, Vs(its), Vs(its + 1), Vs(its + 2), its + 1, dm, ssR, physics);
     for (int64_t it2 = 1; it2 < g.sub + 1; ++it2) {
       step_md(C, m, p0, p1, T(1));
       // injectSecondarySource_3D (kernelsElaGpu_3D.cu:1675-1693), index it2-1
@@ -448,7 +460,7 @@ void born_forward(const Grid& g, const T* med, const T* src, const int64_t* spos
 // Born adjoint (BornElasticGpuFunctions_3D.cu:2103-2336). Accumulates into dmod.
 template <typename T>
 void born_adjoint(const Grid& g, const T* med, const T* src, const int64_t* spos, const int64_t* nsrc,
-                  T* dmod, const T* rec, const int64_t* rpos, const int64_t* nrec) {
+                  T* dmod, const T* rec, const int64_t* rpos, const int64_t* nrec, bool physics) {
   Ctx<T> C(g);
   const int64_t n = C.nModel, h = C.h();
   const T* m[8];
@@ -474,11 +486,11 @@ void born_adjoint(const Grid& g, const T* med, const T* src, const int64_t* spos
       }
       p0.swap(p1);
     }
-    imaging_adj(C, Vs(its), Vs(its + 1), Vs(its + 2), its + 1, dm, ssR);
+    imaging_adj(C, Vs(its), Vs(its + 1), Vs(its + 2), its + 1, dm, ssR, physics);
     ssR.swap(ssL);
     ssL.zero();
   }
-  imaging_adj(C, Vs(-1), Vs(0), Vs(1), 0, dm, ssR);
+  imaging_adj(C, Vs(-1), Vs(0), Vs(1), 0, dm, ssR, physics);
 }
 
 // ---- set-up helpers ------------------------------------------------------
@@ -634,13 +646,13 @@ static Grid to_grid(const orc_grid* p) {
   }                                                                                               \
   void orc_born_forward_##SFX(const orc_grid* g, const T* med, const T* src, const int64_t* spos, \
                               const int64_t* nsrc, const T* dmod, T* rec, const int64_t* rpos,    \
-                              const int64_t* nrec) {                                              \
-    born_forward<T>(to_grid(g), med, src, spos, nsrc, dmod, rec, rpos, nrec);                     \
+                              const int64_t* nrec, int physics) {                                 \
+    born_forward<T>(to_grid(g), med, src, spos, nsrc, dmod, rec, rpos, nrec, physics != 0);       \
   }                                                                                               \
   void orc_born_adjoint_##SFX(const orc_grid* g, const T* med, const T* src, const int64_t* spos, \
                               const int64_t* nsrc, T* dmod, const T* rec, const int64_t* rpos,    \
-                              const int64_t* nrec) {                                              \
-    born_adjoint<T>(to_grid(g), med, src, spos, nsrc, dmod, rec, rpos, nrec);                     \
+                              const int64_t* nrec, int physics) {                                 \
+    born_adjoint<T>(to_grid(g), med, src, spos, nsrc, dmod, rec, rpos, nrec, physics != 0);       \
   }                                                                                               \
   void orc_gradient_##SFX(const orc_grid* g, const T* med, const T* src, const int64_t* spos,       \
                           const int64_t* nsrc, const T* adj, const int64_t* rpos, const int64_t* nrec, \
@@ -673,6 +685,30 @@ static Grid to_grid(const orc_grid* p) {
 
 ORC_INST(double, f64)
 ORC_INST(float, f32)
+
+// The Born imaging kernels alone, for pinning: V slices o, c, n (3 arrays
+// each, o/n may be null at the ends as in the time loops), 8 perturbation
+// arrays dm, 9 secondary-source arrays ss (forward: written; adjoint: read,
+// images accumulated into dm). FP64 only.
+void orc_imaging_fwd_f64(const orc_grid* g, const double* o, const double* c, const double* nn, int64_t its,
+                         const double* dm, double* ss, int physics) {
+  Ctx<double> C(to_grid(g));
+  const double* d[8];
+  for (int i = 0; i < 8; ++i) d[i] = dm + i * C.nModel;
+  Fields<double> f(C.nModel);
+  imaging_fwd(C, o, c, nn, its, d, f, physics != 0);
+  for (int k = 0; k < 9; ++k) std::memcpy(ss + k * C.nModel, f.f[k].data(), C.nModel * sizeof(double));
+}
+
+void orc_imaging_adj_f64(const orc_grid* g, const double* o, const double* c, const double* nn, int64_t its,
+                         double* dm, const double* ss, int physics) {
+  Ctx<double> C(to_grid(g));
+  double* d[8];
+  for (int i = 0; i < 8; ++i) d[i] = dm + i * C.nModel;
+  Fields<double> f(C.nModel);
+  for (int k = 0; k < 9; ++k) std::memcpy(f.f[k].data(), ss + k * C.nModel, C.nModel * sizeof(double));
+  imaging_adj(C, o, c, nn, its, d, f, physics != 0);
+}
 
 // Time filter and taper tables as the reference builds them (for pinning).
 void orc_tables(const orc_grid* g, double* filt, double* taper) {

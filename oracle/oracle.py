@@ -176,7 +176,10 @@ def lib_nl_adjoint(g: Grid, med8, rec_traces, src_pos, rec_pos, dtype=np.float64
     return _unpack_traces(src, sn, g.nts)
 
 
-def lib_born_forward(g: Grid, med8, src_traces, src_pos, dmod8, rec_pos, dtype=np.float64):
+def lib_born_forward(g: Grid, med8, src_traces, src_pos, dmod8, rec_pos, dtype=np.float64, physics=False):
+    """Born forward (BornElasticGpuFunctions_3D.cu:1226-1461). ``physics=True``
+    replaces the reference's xz derivative pattern of the yz secondary source
+    (kernelsElaGpu_3D.cu:1419-1431) with the yz pattern of the stepper."""
     L = lib()
     sp, sn = _pack_pos(src_pos)
     rp, rn = _pack_pos(rec_pos)
@@ -188,12 +191,14 @@ def lib_born_forward(g: Grid, med8, src_traces, src_pos, dmod8, rec_pos, dtype=n
     gc = g.c()
     getattr(L, "orc_born_forward_" + _sfx(dtype))(
         ctypes.byref(gc), _ptr(med, ct), _ptr(src, ct), _ptr(sp, ctypes.c_int64), _ptr(sn, ctypes.c_int64),
-        _ptr(dm, ct), _ptr(rec, ct), _ptr(rp, ctypes.c_int64), _ptr(rn, ctypes.c_int64))
+        _ptr(dm, ct), _ptr(rec, ct), _ptr(rp, ctypes.c_int64), _ptr(rn, ctypes.c_int64), ctypes.c_int(int(physics)))
     return _unpack_traces(rec, rn, g.nts)
 
 
-def lib_born_adjoint(g: Grid, med8, src_traces, src_pos, rec_traces, rec_pos, dtype=np.float64):
-    """Returns the 8 accumulated (scaled, staggered) perturbation images."""
+def lib_born_adjoint(g: Grid, med8, src_traces, src_pos, rec_traces, rec_pos, dtype=np.float64, physics=False):
+    """Returns the 8 accumulated (scaled, staggered) perturbation images
+    (BornElasticGpuFunctions_3D.cu:2103-2336); ``physics`` as for
+    lib_born_forward (kernelsElaGpu_3D.cu:1622-1634)."""
     L = lib()
     sp, sn = _pack_pos(src_pos)
     rp, rn = _pack_pos(rec_pos)
@@ -205,7 +210,7 @@ def lib_born_adjoint(g: Grid, med8, src_traces, src_pos, rec_traces, rec_pos, dt
     gc = g.c()
     getattr(L, "orc_born_adjoint_" + _sfx(dtype))(
         ctypes.byref(gc), _ptr(med, ct), _ptr(src, ct), _ptr(sp, ctypes.c_int64), _ptr(sn, ctypes.c_int64),
-        _ptr(dm, ct), _ptr(rec, ct), _ptr(rp, ctypes.c_int64), _ptr(rn, ctypes.c_int64))
+        _ptr(dm, ct), _ptr(rec, ct), _ptr(rp, ctypes.c_int64), _ptr(rn, ctypes.c_int64), ctypes.c_int(int(physics)))
     return dm
 
 
@@ -228,6 +233,32 @@ def lib_gradient(g: Grid, med8, src_traces, src_pos, adj_traces, rec_pos, dtype=
         ctypes.byref(gc), _ptr(med, ct), _ptr(src, ct), _ptr(sp, ctypes.c_int64), _ptr(sn, ctypes.c_int64),
         _ptr(adj, ct), _ptr(rp, ctypes.c_int64), _ptr(rn, ctypes.c_int64), _ptr(g8, ct), _ptr(gsrc, ct))
     return g8, _unpack_traces(gsrc, sn, g.nts)
+
+
+def lib_imaging(g: Grid, V_o, V_c, V_n, its, dm8=None, ss9=None, adjoint=False, physics=False):
+    """The Born imaging kernels alone (FP64): forward (imagingElaFwdGpu_3D,
+    kernelsElaGpu_3D.cu:1232-1438) maps dm8 to the 9 secondary sources;
+    adjoint (imagingElaAdjGpu_3D, :1440-1641) maps ss9 to the 8 images. V_* are
+    [3, ny, nx, nz] coarse velocity slices (None at the ends of the record)."""
+    L = lib()
+    gc = g.c()
+    ct = ctypes.c_double
+    def arr(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    vo, vc, vn = arr(V_o), arr(V_c), arr(V_n)
+    po = _ptr(vo, ct) if vo is not None else None
+    pn = _ptr(vn, ct) if vn is not None else None
+    if not adjoint:
+        dm = arr(dm8)
+        ss = np.zeros((9,) + g.shape)
+        L.orc_imaging_fwd_f64(ctypes.byref(gc), po, _ptr(vc, ct), pn, ctypes.c_int64(its), _ptr(dm, ct),
+                              _ptr(ss, ct), ctypes.c_int(int(physics)))
+        return ss
+    ss = arr(ss9)
+    dm = np.zeros((8,) + g.shape)
+    L.orc_imaging_adj_f64(ctypes.byref(gc), po, _ptr(vc, ct), pn, ctypes.c_int64(its), _ptr(dm, ct),
+                          _ptr(ss, ct), ctypes.c_int(int(physics)))
+    return dm
 
 
 def lib_stagger(axis: int, adjoint: bool, a, dtype=np.float64):

@@ -57,9 +57,13 @@ class OracleProp:
     def __init__(self, fd):
         self.g = O.Grid(fd.nz, fd.nx, fd.ny, fd.nts, fd.sub, fd.dz, fd.dx, fd.dy, fd.dts, fd.min_pad, fd.alpha_cos)
         self.med = None
+        self.physics = False
 
     def set_medium(self, med8):
         self.med = np.asarray(med8, np.float64)
+
+    def set_born_mode(self, physics):
+        self.physics = bool(physics)
 
     def nl_forward(self, spos, src, rpos):
         return O.lib_nl_forward(self.g, self.med, src, spos, rpos)
@@ -68,10 +72,10 @@ class OracleProp:
         return O.lib_nl_adjoint(self.g, self.med, rec, spos, rpos)
 
     def born_forward(self, spos, src, dm8, rpos):
-        return O.lib_born_forward(self.g, self.med, src, spos, dm8, rpos)
+        return O.lib_born_forward(self.g, self.med, src, spos, dm8, rpos, physics=self.physics)
 
     def born_adjoint(self, spos, src, rec, rpos, dm8=None):
-        img = O.lib_born_adjoint(self.g, self.med, src, spos, rec, rpos)
+        img = O.lib_born_adjoint(self.g, self.med, src, spos, rec, rpos, physics=self.physics)
         return img if dm8 is None else dm8 + img
 
 

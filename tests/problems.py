@@ -53,13 +53,14 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def born_blob_problem(param):
+def born_blob_problem(param, blob_y=20):
     """Homogeneous 40^3 grid, explosive Ricker source at the centre, receivers
     on a line, and a smooth Gaussian perturbation blob of parameter `param`
-    between them (e^-4.5 of its peak at the source). Returns the grid, the
-    medium, the perturbation [3, ny, nx, nz], the positions and fwd(scale):
-    the oracle data, scaled medium and medium-scaled source traces of
-    m + scale * perturbation."""
+    (0 density, 1 lambda, 2 mu) between them (e^-4.5 of its peak at the
+    source), centred on plane y = blob_y (the source's plane by default).
+    Returns the grid, the medium, the perturbation [3, ny, nx, nz], the
+    positions and fwd(scale): the oracle data, scaled medium and
+    medium-scaled source traces of m + scale * perturbation."""
     g = O.Grid(nz=40, nx=40, ny=40, nts=60, sub=8, dz=10., dx=10., dy=10., dts=0.004, minPad=6, alphaCos=0.99)
     shape = g.shape
     rho = np.full(shape, 2000.0)
@@ -78,14 +79,15 @@ def born_blob_problem(param):
     src = [np.zeros((len(spos[O.COMP_GRID[k]]), g.nts)) for k in range(9)]
     # a smooth perturbation blob between source and receivers, zero at the source
     iy, ix, iz = np.meshgrid(np.arange(g.ny), np.arange(g.nx), np.arange(g.nz), indexing="ij")
-    blob = np.exp(-((iz - 14) ** 2 + (ix - 20) ** 2 + (iy - 20) ** 2) / 8.0)
+    blob = np.exp(-((iz - 14) ** 2 + (ix - 20) ** 2 + (iy - blob_y) ** 2) / 8.0)
     d3 = np.zeros((3,) + shape)
-    d3[param] = blob * (rho if param == 0 else lam) * 1.0
+    d3[param] = blob * (rho, lam, mu)[param] * 1.0
 
     def fwd(scale):
         r = rho + scale * d3[0]
         lm = lam + scale * d3[1]
-        med = O.lib_scale_medium(r, lm, mu, g.dtw)
+        m = mu + scale * d3[2]
+        med = O.lib_scale_medium(r, lm, m, g.dtw)
         m8 = med.reshape(8, -1)
         s = [np.zeros_like(x) for x in src]
         pc = spos[0][0]

@@ -1,6 +1,7 @@
 """The HIP path through the C ABI against the committed golden fixture
 (tests/golden/small_fp64.npz, made by the FP64 oracle): FP64 to 1e-11 and
-FP32 to 2e-5 relative L2 for the four operators and the gradient."""
+FP32 to 5e-6 relative L2 for the four operators and the gradient (half the
+1e-5 bar for FP32 forward data)."""
 import numpy as np
 import pytest
 
@@ -9,7 +10,7 @@ from tests.problems import rel
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f64": 1e-11, "f32": 2e-5}
+TOL = {"f64": 1e-11, "f32": 5e-6}
 
 
 @pytest.fixture(scope="module")

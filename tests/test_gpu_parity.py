@@ -1,9 +1,10 @@
 """Parity of the HIP propagator (through the C ABI) with the CPU oracle.
 
 Bars: FP64 GPU vs FP64 oracle to 1e-11 relative L2 (same scheme, only the
-summation order differs); FP32 GPU vs FP64 oracle to 2e-5 relative L2 on
-these short runs (FP32 rounding of the scheme itself, see
-test_fp32_oracle_error_budget for the matching CPU figure).
+summation order differs); FP32 GPU vs FP64 oracle to 5e-6 relative L2 on
+these short runs, half the 1e-5 bar for FP32 forward data (measured <= 4e-7:
+FP32 rounding of the scheme itself, see test_fp32_oracle_error_budget for the
+matching CPU figure).
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ from tests.problems import random_problem, rel
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f64": 1e-11, "f32": 2e-5}
+TOL = {"f64": 1e-11, "f32": 5e-6}
 
 GRIDS = {
     # odd sizes: partial z and x tiles, several y chunks
@@ -59,25 +60,31 @@ def test_nl_adjoint(problems, grid, precision):
 
 @pytest.mark.parametrize("grid", list(GRIDS))
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_born_forward(problems, grid, precision):
+@pytest.mark.parametrize("physics", [False, True])
+def test_born_forward(problems, grid, precision, physics):
+    """Reference mode (the yz quirk) and physics mode against the oracle in
+    the same mode."""
     g, P = GRIDS[grid], problems[grid]
-    ref = O.lib_born_forward(g, P["med"], P["src"], P["spos"], P["dm"], P["rpos"])
+    ref = O.lib_born_forward(g, P["med"], P["src"], P["spos"], P["dm"], P["rpos"], physics=physics)
     prop = make_prop(g, precision)
     prop.set_medium(P["med"])
+    prop.set_born_mode(physics)
     out = prop.born_forward(P["spos"], P["src"], P["dm"], P["rpos"])
-    print(f"born_forward {grid} {precision}: rel={rel(out, ref):.3e}", flush=True)
+    print(f"born_forward {grid} {precision} physics={physics}: rel={rel(out, ref):.3e}", flush=True)
     assert rel(out, ref) < TOL[precision]
 
 
 @pytest.mark.parametrize("grid", list(GRIDS))
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_born_adjoint(problems, grid, precision):
+@pytest.mark.parametrize("physics", [False, True])
+def test_born_adjoint(problems, grid, precision, physics):
     g, P = GRIDS[grid], problems[grid]
-    ref = O.lib_born_adjoint(g, P["med"], P["src"], P["spos"], P["rec"], P["rpos"])
+    ref = O.lib_born_adjoint(g, P["med"], P["src"], P["spos"], P["rec"], P["rpos"], physics=physics)
     prop = make_prop(g, precision)
     prop.set_medium(P["med"])
+    prop.set_born_mode(physics)
     out = prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])
-    print(f"born_adjoint {grid} {precision}: rel={rel(out, ref):.3e}", flush=True)
+    print(f"born_adjoint {grid} {precision} physics={physics}: rel={rel(out, ref):.3e}", flush=True)
     assert rel(out, ref) < TOL[precision]
 
 
