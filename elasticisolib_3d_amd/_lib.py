@@ -102,6 +102,7 @@ def load() -> ctypes.CDLL:
         getattr(lib, name).argtypes = [i64, i64, i64, ctypes.c_double, vp, vp, vp]
     P, Tr = ctypes.POINTER(ElaPoints), ctypes.POINTER(ElaTraces)
     lib.ela_nl_forward.argtypes = [vp, P, Tr, P, Tr, i32]
+    lib.ela_set_born_mode.argtypes = [vp, i32]
     lib.ela_nl_adjoint.argtypes = [vp, P, Tr, P, Tr, i32]
     lib.ela_born_forward.argtypes = [vp, P, Tr, vp, P, Tr, i32]
     lib.ela_born_adjoint.argtypes = [vp, P, Tr, vp, P, Tr, i32]

@@ -88,6 +88,27 @@ def test_born_adjoint(problems, grid, precision, physics):
     assert rel(out, ref) < TOL[precision]
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_born_modes_differ_only_in_the_yz_image(problems, precision):
+    """On the GPU, reference and physics modes give the same 7 images and a
+    different dmuyz image, each equal to the oracle's in its own mode (the
+    image-wide error of the adjoint parity test is dominated by the other 7
+    arrays, so this checks the yz array alone)."""
+    g, P = GRIDS["odd"], problems["odd"]
+    out = {}
+    for physics in (False, True):
+        prop = make_prop(g, precision)
+        prop.set_medium(P["med"])
+        prop.set_born_mode(physics)
+        out[physics] = np.asarray(prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"]))
+        ref = O.lib_born_adjoint(g, P["med"], P["src"], P["spos"], P["rec"], P["rpos"], physics=physics)
+        e7 = rel(out[physics][7], ref[7])
+        print(f"born_adjoint yz image {precision} physics={physics}: rel={e7:.3e}", flush=True)
+        assert e7 < TOL[precision]
+    assert np.array_equal(out[False][:7], out[True][:7])
+    assert rel(out[True][7], out[False][7]) > 0.1
+
+
 def test_born_adjoint_accumulates(problems):
     """The Born adjoint adds into the incoming perturbation arrays (reference
     BornElasticGpuFunctions_3D.cu:2113 copies them to the device first)."""
