@@ -28,6 +28,7 @@ EXPORTS = (
     "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
     "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
     "ela_set_windows", "ela_set_graph", "ela_stream_probe", "ela_set_elastic", "ela_set_born_mode",
+    "ela_nl_forward_noalloc",
 )
 
 
@@ -102,6 +103,7 @@ def load() -> ctypes.CDLL:
         getattr(lib, name).argtypes = [i64, i64, i64, ctypes.c_double, vp, vp, vp]
     P, Tr = ctypes.POINTER(ElaPoints), ctypes.POINTER(ElaTraces)
     lib.ela_nl_forward.argtypes = [vp, P, Tr, P, Tr, i32]
+    lib.ela_nl_forward_noalloc.argtypes = [vp, P, Tr, P, Tr]
     lib.ela_set_born_mode.argtypes = [vp, i32]
     lib.ela_nl_adjoint.argtypes = [vp, P, Tr, P, Tr, i32]
     lib.ela_born_forward.argtypes = [vp, P, Tr, vp, P, Tr, i32]

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -91,6 +92,40 @@ struct DevBuf {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+}  // namespace
+
+// Device scratch of the allocation-free entry point (ela_nl_forward_noalloc):
+// converted point lists, the position error flag and the time-major receiver
+// buffer live here instead of in buffers the library allocates. Every GPU the
+// library runs on holds its own copy until the process ends. In host code the
+// name `ela_scratch` is a host-side stand-in: its device address on the
+// current GPU comes from hipGetSymbolAddress (scratch_address below), and
+// only that address is ever passed to a kernel or a copy.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char ela_scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};   // device address of ela_scratch, per device (null: not resolved)
+
+// Resolve (once per device, with that device current) the device address of
+// ela_scratch.
+int scratch_address(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(ELA_ERR_ARG, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(ela_scratch));
+    if (e != hipSuccess || !p)
+      return fail(ELA_ERR_HIP, "hipGetSymbolAddress(ela_scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ELA_OK;
+}
+
 // ELA_MEM_DEVICE calls: every caller pointer must be memory the GPU can
 // access (device, managed or pinned host memory). A pageable host pointer is
 // an argument error, never a kernel that reads host addresses.
@@ -128,6 +163,7 @@ struct ela_handle {
   double tdt = 0.0;         // 2 dtw of ela_set_elastic
   DevBuf taper;             // min_pad values
   DevBuf scratch_tm;        // time-major trace buffer
+  void* scratch_dev = nullptr;  // device address of ela_scratch on cfg.device (resolved in ela_create)
   DevBuf wave2;             // Born scattered / adjoint wavefield, 2 x 9 arrays
   DevBuf coarse;            // Born coarse-time vx, vy, vz slices [slot][3][stride]
   DevBuf ckpt;              // Born adjoint checkpoints [segment][21][stride]
@@ -373,7 +409,12 @@ struct DevPoints {
 // Copies (host) or reads (device) the caller's int64 positions, converts them
 // to internal int32 offsets on the GPU and checks that every point lies in the
 // FD interior.
-int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, int mem, DevPoints& ds, DevPoints& dr) {
+// Bytes of scratch that convert_points uses for `total` points of a
+// device-memory call (int32 positions and the flag).
+size_t point_scratch_bytes(int64_t total) { return (size_t)round_up(total * 4, 256) + 256; }
+
+int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, int mem, DevPoints& ds, DevPoints& dr,
+                   void* scratch = nullptr) {
   int64_t total = 0;
   for (int g = 0; g < 7; ++g) total += sp->n[g] + rp->n[g];
   if (mem == ELA_MEM_DEVICE)
@@ -381,14 +422,16 @@ int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, in
       if (sp->n[g]) CHK(check_dev_ptr(sp->pos[g], "a source position list"));
       if (rp->n[g]) CHK(check_dev_ptr(rp->pos[g], "a receiver position list"));
     }
-  // layout: [int64 staging (host calls)] [int32 converted] [int flag]
-  const size_t b64 = round_up(total * 8, 256), b32 = round_up(total * 4, 256);
-  CHK(h->pos.alloc(b64 + b32 + 256));
-  char* base = (char*)h->pos.p;
+  // layout: [int64 staging (host calls)] [int32 converted] [int flag]; with
+  // `scratch` (device calls only) the same without the staging, in the caller's scratch
+  if (scratch && mem != ELA_MEM_DEVICE) return fail(ELA_ERR_ARG, "scratch point conversion needs device positions");
+  const size_t b64 = scratch ? 0 : round_up(total * 8, 256), b32 = round_up(total * 4, 256);
+  if (!scratch) CHK(h->pos.alloc(b64 + b32 + 256));
+  char* base = scratch ? (char*)scratch : (char*)h->pos.p;
   int64_t* s64 = (int64_t*)base;
   int32_t* s32 = (int32_t*)(base + b64);
   int* flag = (int*)(base + b64 + b32);
-  HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), h->stream));
+  hipLaunchKernelGGL((ela::k_fill<int>), dim3(1), dim3(64), 0, h->stream, flag, (int64_t)1, 0);
   int64_t off = 0;
   for (int side = 0; side < 2; ++side) {
     const ela_points* p = side ? rp : sp;
@@ -582,15 +625,34 @@ int64_t total_points(const DevPoints& p) {
   return n;
 }
 
+// `scratch` (device-memory forward calls only): the device address of
+// ela_scratch; the point lists, flag and time-major receiver buffer are
+// placed there and nothing is allocated (eager launches, no graph).
 template <typename T>
 int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, const ela_points* rp_in,
-           const ela_traces* rt_in, bool adjoint, int mem) {
+           const ela_traces* rt_in, bool adjoint, int mem, void* scratch = nullptr) {
   if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
   CHK(check_points(sp_in, "source"));
   CHK(check_points(rp_in, "receiver"));
   const int64_t nts = h->cfg.nts, sub = h->cfg.sub, hh = sub + 1;
+  int64_t npts = 0;
+  for (int g = 0; g < 7; ++g) npts += sp_in->n[g] + rp_in->n[g];
+  size_t tm_off = 0;
+  if (scratch) {
+    if (adjoint || mem != ELA_MEM_DEVICE) return fail(ELA_ERR_ARG, "the scratch path is the device-memory forward");
+    if (!h->win_y0.empty() && h->chunks_yc != h->ychunk)
+      return fail(ELA_ERR_STATE, "stacked windows: call ela_set_windows again after ela_set_tiling");
+    int64_t nrec = 0;
+    for (int c = 0; c < 9; ++c) nrec += rp_in->n[kCompGrid[c]];
+    tm_off = point_scratch_bytes(npts);
+    const size_t need = tm_off + (size_t)std::max<int64_t>(1, nrec * nts) * sizeof(T);
+    if (need > kScratchBytes)
+      return fail(ELA_ERR_ARG, "ela_nl_forward_noalloc: %zu bytes of scratch needed (%lld points, %lld receiver "
+                  "values x %lld samples), the limit is %zu", need, (long long)npts, (long long)nrec,
+                  (long long)nts, kScratchBytes);
+  }
   DevPoints sp, rp;
-  CHK(convert_points(h, sp_in, rp_in, mem, sp, rp));
+  CHK(convert_points(h, sp_in, rp_in, mem, sp, rp, scratch));
   void* ds[9];
   void* dr[9];
   CHK(stage_traces(h, sp_in, st_in, rp_in, rt_in, !adjoint, adjoint, mem, ds, dr));
@@ -600,8 +662,8 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
 
   // time-major receiver buffer [nts][nrec_total]: coalesced recording
   const int64_t nrec_tot = total_points(rp);
-  CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
-  T* tm = (T*)h->scratch_tm.p;
+  if (!scratch) CHK(h->scratch_tm.alloc(std::max<int64_t>(1, nrec_tot * nts) * sizeof(T)));
+  T* tm = scratch ? (T*)((char*)scratch + tm_off) : (T*)h->scratch_tm.p;
   T* tmv[9];
   for (int c = 0; c < 9; ++c) tmv[c] = tm;
 
@@ -651,7 +713,7 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
   // sub is even (the doubled sub-sampling) whenever the graph is used, so the
   // wavefield pointers are back in place after one coarse step and the same
   // captured step serves every coarse sample
-  const bool graph = h->use_graph && sub % 2 == 0;
+  const bool graph = h->use_graph && sub % 2 == 0 && !scratch;
   hipGraphExec_t exec = nullptr;
   if (graph) {
     CHK(release_exec(h));
@@ -1569,6 +1631,9 @@ int ela_create(const ela_config* cfg, ela_handle** out) {
       if (e != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy failed"));
     }
     h->by_x = cfg->precision == ELA_FP64 ? 8 : 16;
+    // the device address of the allocation-free entry point's scratch on this device
+    rc = scratch_address(cfg->device, &h->scratch_dev);
+    if (rc) return bail(rc);
     *out = owner.release();
     return ELA_OK;
   });
@@ -1639,6 +1704,17 @@ int ela_nl_forward(ela_handle* h, const ela_points* sp, const ela_traces* st, co
     if (!h || !st || !rt) return fail(ELA_ERR_ARG, "null argument");
     DeviceGuard dg(h->cfg.device);
     return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, false, mem) : nl_run<double>(h, sp, st, rp, rt, false, mem);
+  });
+}
+
+int ela_nl_forward_noalloc(ela_handle* h, const ela_points* sp, const ela_traces* st, const ela_points* rp,
+                           ela_traces* rt) {
+  return guarded([&]() -> int {
+    if (!h || !st || !rt || !sp || !rp) return fail(ELA_ERR_ARG, "null argument");
+    if (!h->scratch_dev) return fail(ELA_ERR_STATE, "scratch address not resolved");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? nl_run<float>(h, sp, st, rp, rt, false, ELA_MEM_DEVICE, h->scratch_dev)
+                         : nl_run<double>(h, sp, st, rp, rt, false, ELA_MEM_DEVICE, h->scratch_dev);
   });
 }
 

@@ -161,6 +161,26 @@ int ela_medium_gradient(int64_t nz, int64_t nx, int64_t ny, double dtw, const do
 int ela_nl_forward(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
                    const ela_points* rec_pos, ela_traces* rec, int mem);
 
+/* Allocation-free nonlinear forward. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory inside the call (serving
+ * loops, code that forbids hipMalloc between set-up and teardown). Same
+ * result as ela_nl_forward with ELA_MEM_DEVICE (positions, traces and output
+ * in device memory, asynchronous on the handle's stream) on the same handle,
+ * but the entry point calls no hipMalloc / hipFree or allocating library
+ * routine: its scratch (converted point lists, an error flag, and the
+ * time-major receiver buffer of nrec_total x nts values) is a file-scope
+ * device array of ELA_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends, and its time loop is launched eagerly (no graph). Calls needing more
+ * scratch than that return ELA_ERR_ARG; stacked windows whose chunk table is
+ * stale (ela_set_tiling after ela_set_windows) return ELA_ERR_STATE. The
+ * handle (wavefields, medium) is allocated beforehand by ela_create /
+ * ela_set_medium / ela_set_elastic. Every call on one GPU shares that GPU's
+ * scratch array: calls that overlap on one GPU (several handles or threads on
+ * different streams) race and must be serialised by the caller. */
+#define ELA_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int ela_nl_forward_noalloc(ela_handle* h, const ela_points* src_pos, const ela_traces* src,
+                           const ela_points* rec_pos, ela_traces* rec);
+
 /* Nonlinear adjoint: receiver traces -> source traces. `src` is overwritten. */
 int ela_nl_adjoint(ela_handle* h, const ela_points* src_pos, ela_traces* src,
                    const ela_points* rec_pos, const ela_traces* rec, int mem);
