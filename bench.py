@@ -9,8 +9,11 @@ source traces) are resident in HBM before the timed region.
 
 Prints one JSON line with the throughput in Gcell-updates/s, the roofline of
 the stencil kernels (algorithmic bytes / HIP-event-timed launch duration vs
-8 TB/s) and the CPU baseline (the FP64 CPU restatement of the scheme timed on
-a bounded slice of the 100^3 homogeneous workload on the host cores).
+8 TB/s), the metric's second half -- FWI-gradient shots/s batched vs plain
+on a bounded slice of the 64-shot windowed workload (`fwi_batch`: its first
+4 shots, 0.5 s record) -- and the CPU baseline (the FP64 CPU restatement of
+the scheme timed on a bounded slice of the 100^3 homogeneous workload on the
+host cores).
 """
 from __future__ import annotations
 
@@ -410,85 +413,99 @@ def bench_fwi(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_fwi_batch(args):
+def fwi_batch_compare(n_shots, batch, nts, checkpoint, autotune, full_shots, smooth_device=None):
     """Batched vs plain FWI gradient with shot-local windows on the 400x400x250
-    layered model: the first `--shots` shots of the 8x8 jittered grid, plain
-    = one shot per launch sequence, batched = `--batch` windows stacked in one
-    grid per launch sequence (same kernels, same checkpointing)."""
+    layered model: the first `n_shots` shots of the 8x8 jittered grid, plain =
+    one shot per launch sequence, batched = `batch` windows stacked in one grid
+    per launch sequence (same kernels, same checkpointing); optionally
+    `full_shots` shots over the whole padded model, one per launch sequence
+    (the reference's layout, whose shot window option is unused). Returns
+    (result dict, log lines)."""
     import torch
     from elasticisolib_3d_amd.batch import BatchShot, BatchedFwi, Window
     from elasticisolib_3d_amd.models import batch_layered_400
 
     t0 = time.time()
-    cfg = batch_layered_400(nts=args.nts)
-    log(f"[bench] models built in {time.time() - t0:.1f}s")
+    cfg = batch_layered_400(nts=nts, smooth_device=smooth_device)
+    log(f"[bench] FWI models built in {time.time() - t0:.1f}s")
     nts = cfg["nts"]
     shots = []
-    for src, rec, (x0, y0, wx, wy) in cfg["shots"][:args.shots]:
+    for src, rec, (x0, y0, wx, wy) in cfg["shots"][:n_shots]:
         sig = [np.zeros((1, nts)) for _ in range(9)]
         for c in (3, 4, 5):
             sig[c][0] = cfg["wavelet"]
         shots.append(BatchShot(src, rec, Window(x0, y0, wx, wy), sig))
     shape = cfg["true"][0].shape
-    kw = dict(precision="f32", interval=20, checkpoint=args.checkpoint, autotune=not args.no_autotune)
+    kw = dict(precision="f32", interval=20, checkpoint=checkpoint, autotune=autotune)
     ref = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], shots, batch=1, **kw)
     t0 = time.perf_counter()
     ref.synthesize(cfg["true"])
     log(f"[bench] observed data of {len(shots)} shots in {time.perf_counter() - t0:.1f}s")
+    del ref
     res = {}
-    for name, b in (("plain", 1), ("batched", args.batch)):
+    for name, b in (("plain", 1), ("batched", batch)):
         op = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], shots, batch=b, **kw)
         J, g = op.misfit_and_gradient(cfg["start"])
         res[name] = (J, g, op.timing)
         log(f"[bench] {name} (batch {b}): J = {J:.9e}, {op.timing}")
+        del op
         torch.cuda.empty_cache()
     gp, gb = res["plain"][1], res["batched"][1]
     diff = float(np.linalg.norm(gb - gp) / np.linalg.norm(gp))
     tp, tb = res["plain"][2]["total_s"], res["batched"][2]["total_s"]
     full = {}
-    if args.full_shots > 0:
+    if full_shots > 0:
         # the reference's layout: every shot propagated over the whole padded model, one shot per
         # launch sequence (timing only: the observed traces are the windowed ones)
         fshots = [BatchShot(s.src_xyz, s.rec_xyz, Window(0, 0, shape[1], shape[0]), s.signals, s.observed)
-                  for s in shots[:args.full_shots]]
+                  for s in shots[:full_shots]]
         op = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], fshots,
                         batch=1, **kw)
         op.misfit_and_gradient(cfg["start"])
         tf = op.timing["total_s"]
         log(f"[bench] full grid (batch 1, {len(fshots)} shots): {op.timing}")
-        full = {"full_grid_shots": len(fshots),
-                "full_grid_shots_per_s": round(len(fshots) / tf, 4),
-                "full_grid_s": round(tf, 2),
-                "full_grid_propagation_s": round(op.timing["propagation_s"], 2),
-                "speedup_batched_over_full_grid": round((tf / len(fshots)) / (tb / len(shots)), 3)}
+        full = {"whole_model_shots": len(fshots),
+                "whole_model_shots_per_s": round(len(fshots) / tf, 4),
+                "whole_model_s": round(tf, 2),
+                "whole_model_propagation_s": round(op.timing["propagation_s"], 2),
+                "speedup_batched_over_whole_model": round((tf / len(fshots)) / (tb / len(shots)), 3)}
         del op
         torch.cuda.empty_cache()
     wins = [s.window for s in shots]
+    return {
+        "workload": f"FWI gradient of {len(shots)} shots with shot-local windows on the 400x400x250 layered model "
+                    f"(d=12.5 m, 30-cell sponge per window), explosive 10 Hz Ricker, {nts} coarse samples x "
+                    f"{cfg['sub']} fine steps, FP32 stencil, {checkpoint} checkpoints every 20 fine steps, "
+                    f"FP64 gradient accumulation",
+        "windows": [[w.x0, w.y0, w.nx, w.ny] for w in wins],
+        "receivers": [int(len(s.rec_xyz)) for s in shots],
+        "plain_shots_per_s": round(len(shots) / tp, 4),
+        "batched_shots_per_s": round(len(shots) / tb, 4),
+        "batch": batch,
+        "speedup_batched_over_plain": round(tp / tb, 3),
+        "plain_s": round(tp, 2), "batched_s": round(tb, 2),
+        "plain_propagation_s": round(res["plain"][2]["propagation_s"], 2),
+        "batched_propagation_s": round(res["batched"][2]["propagation_s"], 2),
+        "gradient_rel_diff_batched_vs_plain": diff,
+        **full,
+    }
+
+
+def bench_fwi_batch(args):
+    """Batched vs plain FWI gradient with shot-local windows on the 400x400x250
+    layered model (fwi_batch_compare with the command-line sizes)."""
+    r = fwi_batch_compare(args.shots, args.batch, args.nts, args.checkpoint, not args.no_autotune, args.full_shots,
+                          "cuda")
     out = {
         "metric": "FWI-gradient shots/s, batched vs plain (shot-local windows)",
-        "value": round(len(shots) / tb, 4),
+        "value": r["batched_shots_per_s"],
         "unit": "shots/s",
         "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": round(tb * 1e3, 1),
+        "ms_per_step": round(r["batched_s"] * 1e3, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": f"f32 stencil, {args.checkpoint} checkpoints, f64 gradient accumulation",
         "data": "synthetic",
-        "config": {
-            "workload": f"FWI gradient of {len(shots)} shots with shot-local windows on the 400x400x250 layered "
-                        f"model (d=12.5 m, 30-cell sponge per window), explosive 10 Hz Ricker, {nts} coarse samples x "
-                        f"{cfg['sub']} fine steps",
-            "windows": [[w.x0, w.y0, w.nx, w.ny] for w in wins],
-            "receivers": [int(len(s.rec_xyz)) for s in shots],
-            "plain_shots_per_s": round(len(shots) / tp, 4),
-            "batched_shots_per_s": round(len(shots) / tb, 4),
-            "batch": args.batch,
-            "speedup_batched_over_plain": round(tp / tb, 3),
-            "plain_s": round(tp, 2), "batched_s": round(tb, 2),
-            "plain_propagation_s": round(res["plain"][2]["propagation_s"], 2),
-            "batched_propagation_s": round(res["batched"][2]["propagation_s"], 2),
-            "gradient_rel_diff_batched_vs_plain": diff,
-            **full,
-        },
+        "config": r,
     }
     print(json.dumps(out), flush=True)
 
@@ -518,6 +535,12 @@ def main():
                          "fp64_check: FP32 vs FP64 traces on the forward workload; fwi: FWI gradient shots/s "
                          "and gradient accuracy")
     ap.add_argument("--shots", type=int, default=4, help="fwi_batch: number of shots")
+    ap.add_argument("--fwi-slice-shots", type=int, default=4,
+                    help="nl_forward: shots of the batched-vs-plain FWI slice added to the line (0: skip)")
+    ap.add_argument("--fwi-slice-nts", type=int, default=126,
+                    help="nl_forward: coarse samples of that slice (126 = 0.5 s, 1000 fine steps)")
+    ap.add_argument("--fwi-slice-full", type=int, default=1,
+                    help="nl_forward: shots of that slice also run over the whole padded model")
     ap.add_argument("--batch", type=int, default=2, help="fwi_batch: shots per batched launch sequence")
     ap.add_argument("--full-shots", type=int, default=1,
                     help="fwi_batch: shots also timed over the whole padded model, one per launch sequence "
@@ -660,6 +683,15 @@ def main():
             "streaming_reference": ceiling,
         },
     }
+    del prop, spos, src, rpos, rec
+    torch.cuda.empty_cache()
+    if args.fwi_slice_shots > 0:
+        # the metric's second half on a bounded slice of the 64-shot workload
+        t0 = time.time()
+        out["fwi_batch"] = fwi_batch_compare(args.fwi_slice_shots, min(args.batch, args.fwi_slice_shots),
+                                             args.fwi_slice_nts, args.checkpoint, not args.no_autotune,
+                                             args.fwi_slice_full, "cuda")
+        out["fwi_batch"]["wall_s"] = round(time.time() - t0, 1)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline()

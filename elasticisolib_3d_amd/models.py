@@ -56,6 +56,32 @@ def gaussian_random_field(shape, sigma_cells: float, rng: np.random.Generator) -
     return g / max(float(np.max(np.abs(g))), 1e-30)
 
 
+def gaussian_smooth(a: np.ndarray, sigma: float, device: Optional[str] = None) -> np.ndarray:
+    """Gaussian smoothing of a 3-D array with scipy.ndimage.gaussian_filter's
+    kernel (radius int(4 sigma + 0.5), weights exp(-x^2 / (2 sigma^2))
+    normalised, edges extended by the nearest value), in float32. With
+    `device` (e.g. "cuda") the separable passes run on that torch device,
+    which takes a fraction of a second instead of tens of seconds for a
+    400 x 400 x 250 model with sigma = 20."""
+    if device is None:
+        from scipy.ndimage import gaussian_filter
+        return gaussian_filter(a.astype(np.float32), sigma, mode="nearest").astype(np.float64)
+    import torch
+    import torch.nn.functional as F
+    r = int(4.0 * sigma + 0.5)
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * (x / sigma) ** 2)
+    w = torch.tensor(w / w.sum(), dtype=torch.float32, device=device)
+    t = torch.tensor(a, dtype=torch.float32, device=device)[None, None]
+    for ax in range(3):
+        pad = [0, 0, 0, 0, 0, 0]
+        pad[2 * (2 - ax)] = pad[2 * (2 - ax) + 1] = r
+        shape = [1, 1, 1, 1, 1]
+        shape[2 + ax] = 2 * r + 1
+        t = F.conv3d(F.pad(t, pad, mode="replicate"), w.reshape(shape))
+    return t[0, 0].double().cpu().numpy()
+
+
 def layered_model(nz: int, nx: int, ny: int, seed: int, n_layers: int = 12, vp_range=(1800.0, 4500.0),
                   pert: float = 0.05, sigma: float = 8.0):
     """Seeded 1-D layered (Vp, Vs, rho) with a smoothed random perturbation.
@@ -241,7 +267,8 @@ def fwi_256(nts: int = 501, seed: int = 2025, n_shot_axis: int = 2, rec_step: in
 
 
 def batch_layered_400(nts: int = 626, seed: int = 2024, seed_geom: int = 11, sigma_start: float = 20.0,
-                      n_axis: int = 8, pitch: int = 50, win_range=(180, 300), rec_range=(60, 200)):
+                      n_axis: int = 8, pitch: int = 50, win_range=(180, 300), rec_range=(60, 200),
+                      smooth_device: Optional[str] = None):
     """Multi-shot workload with shot-local windows on the 400 x 400 x 250
     layered model (d = 12.5 m, 30-cell sponge around every window): 8 x 8
     shots on a jittered 50-cell surface grid (explosive 10 Hz Ricker at
@@ -251,12 +278,12 @@ def batch_layered_400(nts: int = 626, seed: int = 2024, seed_geom: int = 11, sig
     4 ms, 8 fine steps each. Returns a dict with the true and starting
     (rho, lambda, mu) [ny, nx, nz] (start: the true model smoothed with a
     sigma_start-cell Gaussian) and the shots (src_xyz, rec_xyz, window
-    (x0, y0, nx, ny))."""
-    from scipy.ndimage import gaussian_filter
+    (x0, y0, nx, ny)). `smooth_device` runs the starting model's smoothing
+    on that torch device (gaussian_smooth)."""
     nxm, nym, nzm, d = 400, 400, 250, 12.5
     vp, vs, rho0 = layered_model(nzm, nxm, nym, seed, sigma=8.0)
     true = np.stack(vpvsrho_to_rholamemu(vp, vs, rho0))
-    sm = [gaussian_filter(a.astype(np.float32), sigma_start, mode="nearest").astype(np.float64) for a in (vp, vs, rho0)]
+    sm = [gaussian_smooth(a, sigma_start, smooth_device) for a in (vp, vs, rho0)]
     start = np.stack(vpvsrho_to_rholamemu(*sm))
     del vp, vs, rho0, sm
     rng = np.random.default_rng(seed_geom)
