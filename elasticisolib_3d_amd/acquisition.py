@@ -150,27 +150,85 @@ class ShotGeometry:
         return [self.grids[COMP_GRID[c]].forward(traces[c]) for c in range(9)]
 
 
-def scale_regular_traces(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.ndarray], dxyz: float):
+# Grid of the points at which each of the 8 scaled medium arrays scales a
+# component (rhoxDtw at the X grid, ..., lamb2MuDtw and lambDtw at the centre).
+_MEDIUM_GRID = (1, 2, 3, 0, 0, 4, 5, 6)
+
+
+def point_medium(med8, geom: ShotGeometry) -> List[np.ndarray]:
+    """The 8 scaled medium arrays at the regular points where the source
+    scaling reads them (array k at the points of grid _MEDIUM_GRID[k]), as
+    FP64 numpy arrays. med8 is a numpy or torch [8, ny, nx, nz] array (a
+    device tensor is gathered on its device, only the point values travel)."""
+    pos = geom.positions()
+    if hasattr(med8, "device") and not isinstance(med8, np.ndarray):
+        import torch
+        flat = med8.reshape(8, -1)
+        return [flat[k].index_select(0, torch.as_tensor(np.asarray(pos[g], np.int64), device=med8.device))
+                .double().cpu().numpy() for k, g in enumerate(_MEDIUM_GRID)]
+    m = np.asarray(med8).reshape(8, -1)
+    return [np.asarray(m[k][pos[g]], np.float64) for k, g in enumerate(_MEDIUM_GRID)]
+
+
+def scale_regular_traces_at(mp: Sequence[np.ndarray], reg: Sequence[np.ndarray], dxyz: float):
     """Medium scaling applied to injected traces: forces by 2dtw/rho, normal
     moments through the (lambda+2mu, lambda) block, shear moments by 2dtw mu,
     all divided by the cell volume (nonlinearPropElasticGpu_3D.cpp:200-277;
-    the nonlinear adjoint applies it to the receiver traces, :481-558)."""
-    m = med8.reshape(8, -1)
-    pos = geom.positions()
+    the nonlinear adjoint applies it to the receiver traces, :481-558).
+    `mp` are the medium values at the points (point_medium)."""
     out = [None] * 9
-    out[0] = reg[0] * m[0][pos[1]][:, None]
-    out[1] = reg[1] * m[1][pos[2]][:, None]
-    out[2] = reg[2] * m[2][pos[3]][:, None]
-    l2m = m[3][pos[0]][:, None]
-    lam = m[4][pos[0]][:, None]
+    out[0] = reg[0] * mp[0][:, None]
+    out[1] = reg[1] * mp[1][:, None]
+    out[2] = reg[2] * mp[2][:, None]
+    l2m = mp[3][:, None]
+    lam = mp[4][:, None]
     out[3] = reg[3] * l2m + (reg[4] + reg[5]) * lam
     out[4] = reg[4] * l2m + (reg[3] + reg[5]) * lam
     out[5] = reg[5] * l2m + (reg[3] + reg[4]) * lam
-    out[6] = reg[6] * m[5][pos[4]][:, None]
-    out[7] = reg[7] * m[6][pos[5]][:, None]
-    out[8] = reg[8] * m[7][pos[6]][:, None]
+    out[6] = reg[6] * mp[5][:, None]
+    out[7] = reg[7] * mp[6][:, None]
+    out[8] = reg[8] * mp[7][:, None]
     inv = 1.0 / dxyz
     return [o * inv for o in out]
+
+
+def scale_regular_traces(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.ndarray], dxyz: float):
+    """scale_regular_traces_at with the medium read from med8 (numpy or torch)."""
+    return scale_regular_traces_at(point_medium(med8, geom), reg, dxyz)
+
+
+def _source_scaling_terms(reg: Sequence[np.ndarray], gscaled: Sequence[np.ndarray], dxyz: float):
+    """Per medium array k: the gradient values at the points of grid
+    _MEDIUM_GRID[k] of a function of the scaled source traces, through
+    scale_regular_traces_at (`reg` unscaled, `gscaled` the gradient with
+    respect to the scaled traces)."""
+    inv = 1.0 / dxyz
+    r = [np.asarray(x, np.float64) for x in reg]
+    s = [np.asarray(x, np.float64) for x in gscaled]
+    return [
+        np.sum(s[0] * r[0], axis=1) * inv,
+        np.sum(s[1] * r[1], axis=1) * inv,
+        np.sum(s[2] * r[2], axis=1) * inv,
+        np.sum(s[3] * r[3] + s[4] * r[4] + s[5] * r[5], axis=1) * inv,
+        np.sum(s[3] * (r[4] + r[5]) + s[4] * (r[3] + r[5]) + s[5] * (r[3] + r[4]), axis=1) * inv,
+        np.sum(s[6] * r[6], axis=1) * inv,
+        np.sum(s[7] * r[7], axis=1) * inv,
+        np.sum(s[8] * r[8], axis=1) * inv,
+    ]
+
+
+def add_source_scaling_gradient(g8, geom: ShotGeometry, reg: Sequence[np.ndarray], gscaled: Sequence[np.ndarray],
+                                dxyz: float) -> None:
+    """source_scaling_gradient added into a torch FP64 [8, ny, nx, nz] tensor
+    (any device; index_add_ accumulates points shared by components)."""
+    import torch
+    pos = geom.positions()
+    flat = g8.view(8, -1)
+    for k, v in enumerate(_source_scaling_terms(reg, gscaled, dxyz)):
+        p = pos[_MEDIUM_GRID[k]]
+        if len(p):
+            flat[k].index_add_(0, torch.as_tensor(np.asarray(p, np.int64), device=g8.device),
+                               torch.as_tensor(v, dtype=g8.dtype, device=g8.device))
 
 
 def source_scaling_gradient(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[np.ndarray],
@@ -190,20 +248,8 @@ def source_scaling_gradient(med8: np.ndarray, geom: ShotGeometry, reg: Sequence[
             raise ValueError("out must be a C-contiguous float64 array of the medium's shape")
         g = out.reshape(8, -1)
     pos = geom.positions()
-    inv = 1.0 / dxyz
-    r = [np.asarray(x, np.float64) for x in reg]
-    s = [np.asarray(x, np.float64) for x in gscaled]
-
-    def add(k, p, v):
+    for k, v in enumerate(_source_scaling_terms(reg, gscaled, dxyz)):
+        p = pos[_MEDIUM_GRID[k]]
         if len(p):
-            np.add.at(g[k], p, v * inv)
-
-    add(0, pos[1], np.sum(s[0] * r[0], axis=1))
-    add(1, pos[2], np.sum(s[1] * r[1], axis=1))
-    add(2, pos[3], np.sum(s[2] * r[2], axis=1))
-    add(3, pos[0], np.sum(s[3] * r[3] + s[4] * r[4] + s[5] * r[5], axis=1))
-    add(4, pos[0], np.sum(s[3] * (r[4] + r[5]) + s[4] * (r[3] + r[5]) + s[5] * (r[3] + r[4]), axis=1))
-    add(5, pos[4], np.sum(s[6] * r[6], axis=1))
-    add(6, pos[5], np.sum(s[7] * r[7], axis=1))
-    add(7, pos[6], np.sum(s[8] * r[8], axis=1))
+            np.add.at(g[k], p, v)
     return out if out is not None else g.reshape(shape)

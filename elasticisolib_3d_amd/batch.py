@@ -22,8 +22,10 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
+from . import medium_torch as MT
 from . import propagator as P
-from .acquisition import COMP_GRID, PaddedAxes, ShotGeometry, scale_regular_traces, source_scaling_gradient
+from .acquisition import (COMP_GRID, PaddedAxes, ShotGeometry, add_source_scaling_gradient, point_medium,
+                          scale_regular_traces_at)
 from .models import FAT, pad_sizes
 
 
@@ -123,6 +125,46 @@ class WindowStack:
         g = _unpad(g, -3, FAT + self.pad, w.ny)
         out[..., w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :] += g
 
+    # torch versions (any device): the same cut, pad, stack and transpose,
+    # so that batched FWI keeps the model and its gradient on the GPU
+    def window_model_t(self, k: int, m):
+        """window_model on a torch [..., ny, nx, nz] tensor (3 or 4 dims)."""
+        import torch.nn.functional as F
+        w = self.windows[k]
+        zm, zp = self.zpads
+        xm, xp = self.xpads[k]
+        core = m[..., w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :]
+        pad = (FAT + zm, FAT + zp, FAT + xm, FAT + xp, FAT + self.pad, FAT + self.pad)
+        if core.dim() == 3:
+            return F.pad(core[None], pad, mode="replicate")[0]
+        return F.pad(core, pad, mode="replicate")
+
+    def window_adjoint_t(self, k: int, g, out) -> None:
+        """window_adjoint on torch tensors: out += transpose of window_model_t(k) applied to g."""
+        w = self.windows[k]
+        zm, zp = self.zpads
+        xm, xp = self.xpads[k]
+
+        def unpad(a, dim, lo, n):
+            core = a.narrow(dim, lo, n).clone()
+            core.narrow(dim, 0, 1).add_(a.narrow(dim, 0, lo).sum(dim=dim, keepdim=True))
+            core.narrow(dim, n - 1, 1).add_(a.narrow(dim, lo + n, a.shape[dim] - lo - n).sum(dim=dim, keepdim=True))
+            return core
+
+        g = unpad(g, -1, FAT + zm, self.nz_model)
+        g = unpad(g, -2, FAT + xm, w.nx)
+        g = unpad(g, -3, FAT + self.pad, w.ny)
+        out[..., w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :] += g
+
+    def stack_t(self, per_window, fill: float = 0.0):
+        """stack on torch tensors [..., ny_w, nx_w, nz] -> [..., ny, nx, nz]."""
+        import torch
+        a0 = per_window[0]
+        out = torch.full(tuple(a0.shape[:-3]) + (self.ny, self.nx, self.nz), fill, dtype=a0.dtype, device=a0.device)
+        for k, a in enumerate(per_window):
+            out[..., self.y_off[k]:self.y_off[k] + self.ny_w[k], :self.nx_w[k], :] = a
+        return out
+
     def stack(self, per_window: Sequence[np.ndarray], dtype=None, fill: float = 0.0) -> np.ndarray:
         """[..., ny_w, nx_w, nz] arrays -> one stacked [..., ny, nx, nz] array
         (`fill` beyond a narrower window)."""
@@ -210,12 +252,21 @@ class BatchedFwi:
     def _groups(self):
         return [self.shots[i:i + self.batch] for i in range(0, len(self.shots), self.batch)]
 
-    def _setup(self, group, model):
-        """Stack, propagator and per-shot geometry of one batch."""
+    def _model_device(self, model):
+        """The full (rho, lambda, mu) model as one FP64 tensor on the GPU."""
+        import torch
+        return torch.as_tensor(np.asarray(model, np.float64), device="cuda")
+
+    def _setup(self, group, model_d):
+        """Stack, propagator and per-shot geometry of one batch. The windows'
+        media are cut, padded, staggered and scaled on the GPU (FP64, the
+        operations of the host routines); `meds` are the per-window 8 scaled
+        arrays (FP64, device)."""
         import torch
         ws = WindowStack((self.model_shape[2], self.model_shape[1], self.model_shape[0]), self.d, self.pad,
                          [s.window for s in group])
-        meds = ws.window_media(model[0], model[1], model[2], self.dtw)
+        wins = [ws.window_model_t(k, model_d) for k in range(len(group))]
+        meds = [MT.scale_medium(w[0], w[1], w[2], self.dtw) for w in wins]
         prop = P.Propagator(ws.nz, ws.nx, ws.ny, self.nts, self.sub, self.d, self.d, self.d, self.dts, ws.min_pad,
                             self.alpha, self.precision, self.device)
         ws.set_windows(prop)
@@ -225,10 +276,10 @@ class BatchedFwi:
                 prop.set_tile_order(*self.tiling[2:])
         td = torch.float32 if self.precision == "f32" else torch.float64
         if self.medium == "elastic":
-            rlm = ws.stacked_elastic(model[0], model[1], model[2])
-            prop.set_elastic(torch.tensor(rlm, dtype=td, device="cuda"), self.dtw)
+            prop.set_elastic(ws.stack_t(wins, fill=1.0).to(td).contiguous(), self.dtw)
         else:
-            prop.set_medium(torch.tensor(ws.stack(meds), dtype=td, device="cuda"))
+            prop.set_medium(ws.stack_t(meds).to(td).contiguous())
+        del wins
         if self.autotune and not self.tiling:
             from .autotune import autotune
             autotune(prop, reps=5)
@@ -252,13 +303,14 @@ class BatchedFwi:
 
     def _src_traces(self, ws, meds, geos, group, td):
         """Scaled regular source traces of the batch (9 concatenated arrays)
-        and the unscaled regular traces per shot."""
+        and the unscaled regular traces per shot (the scaled medium is read
+        at the source points only)."""
         import torch
         raw, scaled = [], []
         for k, (s, (sg, _)) in enumerate(zip(group, geos)):
             r = sg.to_regular(s.signals)
             raw.append(r)
-            scaled.append(scale_regular_traces(meds[k], sg, r, self.d ** 3))
+            scaled.append(scale_regular_traces_at(point_medium(meds[k], sg), r, self.d ** 3))
         cat = [np.concatenate([sc[c] for sc in scaled]) for c in range(9)]
         return [torch.tensor(c, dtype=td, device="cuda") for c in cat], raw
 
@@ -278,8 +330,9 @@ class BatchedFwi:
 
     def synthesize(self, model) -> None:
         import torch
+        model_d = self._model_device(model)
         for group in self._groups():
-            ws, meds, prop, geos, td = self._setup(group, model)
+            ws, meds, prop, geos, td = self._setup(group, model_d)
             src, _ = self._src_traces(ws, meds, geos, group, td)
             rec = prop.nl_forward(self._merged(ws, geos, 0), src, self._merged(ws, geos, 1))
             torch.cuda.synchronize()
@@ -288,15 +341,18 @@ class BatchedFwi:
             del prop
 
     def misfit_and_gradient(self, model):
-        """(J, dJ/d(rho, lambda, mu) on the full model [3, ny, nx, nz])."""
+        """(J, dJ/d(rho, lambda, mu) on the full model [3, ny, nx, nz]). The
+        gradient stays on the GPU from the adjoint propagation to the full
+        model (chain rule through the scaling, source-scaling term, transpose
+        of the window cut) and is copied to the host once."""
         import torch
-        nym, nxm, nzm = self.model_shape
-        g3 = np.zeros((3, nym, nxm, nzm), np.float64)
+        t0 = time.perf_counter()
+        model_d = self._model_device(model)
+        g3 = torch.zeros_like(model_d)
         J = 0.0
         t_prop = 0.0
-        t0 = time.perf_counter()
         for group in self._groups():
-            ws, meds, prop, geos, td = self._setup(group, model)
+            ws, meds, prop, geos, td = self._setup(group, model_d)
             src, raw = self._src_traces(ws, meds, geos, group, td)
             spos, rpos = self._merged(ws, geos, 0), self._merged(ws, geos, 1)
             ta = time.perf_counter()
@@ -316,7 +372,6 @@ class BatchedFwi:
             g8, gsrc = prop.gradient_backward(spos, src, rpos, adj)
             torch.cuda.synchronize()
             t_prop += time.perf_counter() - ta
-            g8 = g8.cpu().numpy()
             off = [0] * 7
             for k, (sg, rg) in enumerate(geos):
                 gs = []
@@ -326,10 +381,11 @@ class BatchedFwi:
                     gs.append(gsrc[c][off[g]:off[g] + n].double().cpu().numpy())
                 for g in range(7):
                     off[g] += sg.counts()[g]
-                g8k = source_scaling_gradient(meds[k], sg, raw[k], gs, self.d ** 3,
-                                              out=np.ascontiguousarray(ws.unstack(k, g8)))
-                ws.window_adjoint(k, P.medium_gradient(g8k, meds[k], self.dtw), g3)
-            del prop
+                g8k = ws.unstack(k, g8).contiguous()
+                add_source_scaling_gradient(g8k, sg, raw[k], gs, self.d ** 3)
+                ws.window_adjoint_t(k, MT.medium_gradient(g8k, meds[k], self.dtw), g3)
+            del prop, g8, meds
             torch.cuda.empty_cache()
+        g3h = g3.cpu().numpy()
         self.timing = {"total_s": time.perf_counter() - t0, "propagation_s": t_prop}
-        return J, g3
+        return J, g3h

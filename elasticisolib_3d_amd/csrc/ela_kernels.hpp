@@ -1004,93 +1004,134 @@ struct ImagingArgs {
   const int* chunks;
 };
 
+// The imaging kernels' LDS: double-buffered planes of the coarse vx, vy, vz
+// slice V[its] (one LDS object, see k_velocity).
+template <typename T, int BX>
+struct ImagingLds {
+  Plane<T, BX> vx[2], vy[2], vz[2];
+};
+
+// Secondary sources (forward) or images (adjoint) of coarse sample its. The
+// strain rates of V[its] are those of the stress update (the same D+ / D-
+// pattern), so the kernel runs the stress stepper's pipeline: planes of
+// V[its] with x / z halos double-buffered in LDS (FP32 rows straight from
+// HBM), y derivatives from 8-deep register queues, the next plane's loads
+// issued before the current plane is computed, one barrier per plane, 32-bit
+// offsets and unpredicated neighbour loads (every library array carries the
+// guard that keeps them inside its allocation). Per interior cell it reads
+// V[its - 1] and V[its + 1] (not at the ends of the record) and the 8
+// perturbations (forward, writing 9 secondary sources) or the 9 adjoint
+// slices and the 8 FP64 images (adjoint, rewriting the images).
 template <typename T, int BX, bool ADJ>
 __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
-  __shared__ Plane<T, BX> l_vx, l_vy, l_vz;
+  __shared__ ImagingLds<T, BX> L;
+  auto& l_vx = L.vx; auto& l_vy = L.vy; auto& l_vz = L.vz;
   const TileCoord t = tile_coord<BX>(a.nbz, a.nbx, a.nby, a.ychunk, a.nz, a.nzi, a.zoff, a.nx, a.ny, a.remap, a.xgroup, a.chunks);
   if (t.empty) return;  // block-uniform, before any barrier
-  const int64_t ys = a.ys;
-  const bool ok = t.okc;
-  const bool active = t.active;
-  const bool xh = t.xh;
-  const bool okxlo = t.okxl, okxhi = t.okxh;
-  const bool zh = t.zh;
-  const int colh = t.colh;
-  const bool okzh = t.okzh;
-  const int64_t col = t.col, colxlo = t.cxl, colxhi = t.cxh, colzh = t.czh;
+  const uint32_t ys = (uint32_t)a.ys;
+  constexpr bool DMA = ELA_LDS_DMA && sizeof(T) == 4;
+  const int r = t.ty + FAT, q = t.tz + FAT;
+  const int wy = __builtin_amdgcn_readfirstlane(t.ty), wr = wy + FAT;  // wave-uniform row
+  const bool first = a.its == 0, last = a.its == a.nts - 1;            // kernel-uniform
+
+  struct Stage {
+    T hlo[3], hhi[3], hz[3];   // x-halo rows and z-halo column of vx, vy, vz
+    T nx, ny, nz;              // new queue values (vx: plane + 3, vy: + 4, vz: + 3)
+  };
+  auto fetch = [&](Stage& v, int p, int nb) {
+    const uint32_t pl = (uint32_t)p * ys;
+    v.nx = *at(a.c[0], pl + 3 * ys + t.col);
+    v.ny = *at(a.c[1], pl + 4 * ys + t.col);
+    v.nz = *at(a.c[2], pl + 3 * ys + t.col);
+    if (wy < FAT) {  // the x-halo loaders are whole waves
+      if constexpr (DMA) {
+        row_to_lds(a.c[0], pl + t.cxl, &l_vx[nb][wy][FAT]); row_to_lds(a.c[0], pl + t.cxh, &l_vx[nb][wr + BX][FAT]);
+        row_to_lds(a.c[1], pl + t.cxl, &l_vy[nb][wy][FAT]); row_to_lds(a.c[1], pl + t.cxh, &l_vy[nb][wr + BX][FAT]);
+        row_to_lds(a.c[2], pl + t.cxl, &l_vz[nb][wy][FAT]); row_to_lds(a.c[2], pl + t.cxh, &l_vz[nb][wr + BX][FAT]);
+      } else {
+#pragma unroll
+        for (int f = 0; f < 3; ++f) { v.hlo[f] = *at(a.c[f], pl + t.cxl); v.hhi[f] = *at(a.c[f], pl + t.cxh); }
+      }
+    }
+    if (t.zh) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) v.hz[f] = *at(a.c[f], pl + t.czh);
+    }
+  };
+  auto stage = [&](int b, const Stage& v, T cvx, T cvy, T cvz) {
+    l_vx[b][r][q] = cvx; l_vy[b][r][q] = cvy; l_vz[b][r][q] = cvz;
+    if (!DMA && t.xh) {
+      l_vx[b][t.ty][q] = v.hlo[0]; l_vx[b][r + BX][q] = v.hhi[0];
+      l_vy[b][t.ty][q] = v.hlo[1]; l_vy[b][r + BX][q] = v.hhi[1];
+      l_vz[b][t.ty][q] = v.hlo[2]; l_vz[b][r + BX][q] = v.hhi[2];
+    }
+    if (t.zh) { l_vx[b][r][t.colh] = v.hz[0]; l_vy[b][r][t.colh] = v.hz[1]; l_vz[b][r][t.colh] = v.hz[2]; }
+  };
+
+  // queues: vx (D-y, q[j] = f[iy-4+j]), vy (D+y, f[iy-3+j]), vz (D-y)
   T qx[8], qy[8], qz[8];
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
-    qx[j] = ld(a.c[0], (int64_t)(t.y0 - 4 + j) * ys + col, ok);
-    qy[j] = ld(a.c[1], (int64_t)(t.y0 - 3 + j) * ys + col, ok);
-    qz[j] = ld(a.c[2], (int64_t)(t.y0 - 4 + j) * ys + col, ok);
+    qx[j] = *at(a.c[0], (uint32_t)(t.y0 - 4 + j) * ys + t.col);
+    qy[j] = *at(a.c[1], (uint32_t)(t.y0 - 3 + j) * ys + t.col);
+    qz[j] = *at(a.c[2], (uint32_t)(t.y0 - 4 + j) * ys + t.col);
   }
-  const int r = t.ty + FAT, q = t.tz + FAT;
+  {
+    Stage v0;
+    fetch(v0, t.y0, 0);
+    qx[7] = v0.nx; qy[7] = v0.ny; qz[7] = v0.nz;
+    stage(0, v0, qx[4], qy[3], qz[4]);
+  }
+  __syncthreads();
+  int b = 0;
   for (int iy = t.y0; iy < t.y1; ++iy) {
-    const int64_t pl = (int64_t)iy * ys;
-    const int64_t i = pl + col;
-    qx[7] = ld(a.c[0], i + 3 * ys, ok);
-    qy[7] = ld(a.c[1], i + 4 * ys, ok);
-    qz[7] = ld(a.c[2], i + 3 * ys, ok);
-    T hlo[3] = {0, 0, 0}, hhi[3] = {0, 0, 0}, hz[3] = {0, 0, 0};
-    if (xh) {
-#pragma unroll
-      for (int f = 0; f < 3; ++f) { hlo[f] = ld(a.c[f], pl + colxlo, okxlo); hhi[f] = ld(a.c[f], pl + colxhi, okxhi); }
-    }
-    if (zh) {
-#pragma unroll
-      for (int f = 0; f < 3; ++f) hz[f] = ld(a.c[f], pl + colzh, okzh);
-    }
-    __syncthreads();
-    l_vx[r][q] = qx[4]; l_vy[r][q] = qy[3]; l_vz[r][q] = qz[4];
-    if (xh) {
-      l_vx[t.ty][q] = hlo[0]; l_vx[r + BX][q] = hhi[0];
-      l_vy[t.ty][q] = hlo[1]; l_vy[r + BX][q] = hhi[1];
-      l_vz[t.ty][q] = hlo[2]; l_vz[r + BX][q] = hhi[2];
-    }
-    if (zh) { l_vx[r][colh] = hz[0]; l_vy[r][colh] = hz[1]; l_vz[r][colh] = hz[2]; }
-    __syncthreads();
-    if (active) {
-      const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
-      const T exx = dpx<T, BX>(l_vx, r, q, cx);
-      const T eyy = dq(qy, cy);
-      const T ezz = dpz<T, BX>(l_vz, r, q, cz);
-      const T gxy = dq(qx, cy) + dmx<T, BX>(l_vy, r, q, cx);
-      const T gxz = dmz<T, BX>(l_vx, r, q, cz) + dmx<T, BX>(l_vz, r, q, cx);
-      // yz term: the reference uses the xz derivative pattern
-      // (kernelsElaGpu_3D.cu:1419-1431, :1622-1634); physics mode the yz one
-      // of the stepper's syz update (D-y vz + D-z vy)
-      const T gyz = a.physics ? dq(qz, cy) + dmz<T, BX>(l_vy, r, q, cz) : gxz;
+    const bool more = iy + 1 < t.y1;
+    Stage nx_;
+    if (more) fetch(nx_, iy + 1, b ^ 1);  // buffer b ^ 1 is free: every wave passed the last barrier
+    if (t.active) {
+      const uint32_t i = (uint32_t)iy * ys + t.col;
+      // every per-cell operand is loaded before the first store (a load
+      // behind a store that might alias it waits for that store)
       T dv[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        if (a.its == 0) dv[k] = -a.n[k][i];
-        else if (a.its == a.nts - 1) dv[k] = a.o[k][i];
-        else dv[k] = a.o[k][i] - a.n[k][i];
+        const T o = first ? T(0) : *at(a.o[k], i);
+        const T n = last ? T(0) : *at(a.n[k], i);
+        dv[k] = o - n;   // V[its-1] - V[its+1], zero outside the record
       }
-      // all loads of the cell ahead of its stores (a load behind a store that
-      // might alias it waits for that store)
-      if (!ADJ) {
-        T dm[8];
+      T dm[8], ss[9];
+      double dmo[8];
+      if constexpr (!ADJ) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) dm[k] = a.dm[k][i];
+        for (int k = 0; k < 8; ++k) dm[k] = *at(a.dm[k], i);
+      } else {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.ss[k][i] = dm[k] * dv[k] * a.dts_inv;
+        for (int k = 0; k < 9; ++k) ss[k] = *at(a.ss[k], i);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dmo[k] = a.dmo[k][i];   // 64-bit indexing: FP64 offsets can pass 4 GiB
+      }
+      const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
+      const T exx = dpx<T, BX>(l_vx[b], r, q, cx);
+      const T eyy = dq(qy, cy);
+      const T ezz = dpz<T, BX>(l_vz[b], r, q, cz);
+      const T gxy = dq(qx, cy) + dmx<T, BX>(l_vy[b], r, q, cx);
+      const T gxz = dmz<T, BX>(l_vx[b], r, q, cz) + dmx<T, BX>(l_vz[b], r, q, cx);
+      // yz term: the reference uses the xz derivative pattern
+      // (kernelsElaGpu_3D.cu:1419-1431, :1622-1634); physics mode the yz one
+      // of the stepper's syz update (D-y vz + D-z vy)
+      const T gyz = a.physics ? dq(qz, cy) + dmz<T, BX>(l_vy[b], r, q, cz) : gxz;
+      if constexpr (!ADJ) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) st_nt(a.ss[k], i, dm[k] * dv[k] * a.dts_inv);
         const T div = exx + eyy + ezz;
-        a.ss[SXX][i] = dm[3] * div + T(2.0) * dm[4] * exx;
-        a.ss[SYY][i] = dm[3] * div + T(2.0) * dm[4] * eyy;
-        a.ss[SZZ][i] = dm[3] * div + T(2.0) * dm[4] * ezz;
-        a.ss[SXZ][i] = dm[5] * gxz;
-        a.ss[SXY][i] = dm[6] * gxy;
-        a.ss[SYZ][i] = dm[7] * gyz;
+        st_nt(a.ss[SXX], i, dm[3] * div + T(2.0) * dm[4] * exx);
+        st_nt(a.ss[SYY], i, dm[3] * div + T(2.0) * dm[4] * eyy);
+        st_nt(a.ss[SZZ], i, dm[3] * div + T(2.0) * dm[4] * ezz);
+        st_nt(a.ss[SXZ], i, dm[5] * gxz);
+        st_nt(a.ss[SXY], i, dm[6] * gxy);
+        st_nt(a.ss[SYZ], i, dm[7] * gyz);
       } else {
         // each coarse sample's term in T, the sum over samples in FP64
-        T ss[9];
-        double dmo[8];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) ss[k] = a.ss[k][i];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) dmo[k] = a.dmo[k][i];
 #pragma unroll
         for (int k = 0; k < 3; ++k) a.dmo[k][i] = dmo[k] + double(ss[k] * dv[k] * a.dts_inv);
         const T sxx = ss[SXX], syy = ss[SYY], szz = ss[SZZ];
@@ -1103,6 +1144,12 @@ __global__ void __launch_bounds__(TZ * BX) k_imaging(ImagingArgs<T> a) {
     }
 #pragma unroll
     for (int j = 0; j < 7; ++j) { qx[j] = qx[j + 1]; qy[j] = qy[j + 1]; qz[j] = qz[j + 1]; }
+    if (more) {
+      qx[7] = nx_.nx; qy[7] = nx_.ny; qz[7] = nx_.nz;
+      stage(b ^ 1, nx_, qx[4], qy[3], qz[4]);
+    }
+    __syncthreads();
+    b ^= 1;
   }
 }
 

@@ -228,20 +228,22 @@ def born_layered_400(nts: int = 501, seed: int = 2024, n_rec_axis: int = 150, si
                     ricker(nts, dts, 10.0, 0.1), "explosive", extra={"dm3": dmp})
 
 
-def fwi_256(nts: int = 501, seed: int = 2025, n_shot_axis: int = 2, rec_step: int = 4, sigma_start: float = 20.0):
+def fwi_256(nts: int = 501, seed: int = 2025, n_shot_axis: int = 2, rec_step: int = 4, sigma_start: float = 20.0,
+            smooth_device: Optional[str] = None):
     """FWI workload: true model = the layered recipe on 256 x 256 x 160 cells
     (x, y, z), d = 15 m, 30-cell sponge; starting model = the true one
     smoothed with a sigma_start-cell Gaussian; 2 x 2 explosive shots at
     z = 10 cells (8 Hz Ricker, delay 0.125 s), 2.0 s = 501 samples of 4 ms
     with 8 fine steps each; a seeded surface receiver grid every `rec_step`
     cells (jittered by up to a quarter cell). Returns (true Workload, start
-    (rho, lambda, mu) on the padded grid, list of source positions)."""
-    from scipy.ndimage import gaussian_filter
+    (rho, lambda, mu) on the padded grid, list of source positions).
+    `smooth_device` runs the starting model's smoothing on that torch device
+    (gaussian_smooth)."""
     nxm, nym, nzm, d, pad = 256, 256, 160, 15.0, 30
     vp, vs, rho0 = layered_model(nzm, nxm, nym, seed, sigma=8.0)
-    vp_s = gaussian_filter(vp.astype(np.float32), sigma_start, mode="nearest").astype(np.float64)
-    vs_s = gaussian_filter(vs.astype(np.float32), sigma_start, mode="nearest").astype(np.float64)
-    rho_s = gaussian_filter(rho0.astype(np.float32), sigma_start, mode="nearest").astype(np.float64)
+    vp_s = gaussian_smooth(vp, sigma_start, smooth_device)
+    vs_s = gaussian_smooth(vs, sigma_start, smooth_device)
+    rho_s = gaussian_smooth(rho0, sigma_start, smooth_device)
     true = np.stack(vpvsrho_to_rholamemu(vp, vs, rho0))
     start = np.stack(vpvsrho_to_rholamemu(vp_s, vs_s, rho_s))
     del vp, vs, rho0, vp_s, vs_s, rho_s

@@ -2,6 +2,7 @@
 import numpy as np
 import pytest
 
+from elasticisolib_3d_amd import propagator as P
 from elasticisolib_3d_amd.batch import Window, WindowStack
 
 
@@ -53,3 +54,45 @@ def test_stacked_positions_and_stack():
     # beyond a narrower window the stack holds zeros
     k = int(np.argmin(ws.nx_w))
     assert np.all(st[ws.y_off[k]:ws.y_off[k] + ws.ny_w[k], ws.nx_w[k]:, :] == 0)
+
+
+def test_window_cut_and_transpose_in_torch_match_numpy():
+    """The torch window cut, stack and transpose of the cut (used to keep the
+    batched gradient on the GPU) give the numpy methods' numbers."""
+    import torch
+    rng = np.random.default_rng(5)
+    m = rng.standard_normal((3, 30, 34, 20))   # [.., ny, nx, nz]
+    ws = WindowStack((20, 34, 30), 10.0, 5, [Window(2, 3, 16, 14), Window(10, 8, 20, 18)])
+    mt = torch.from_numpy(m)
+    for k in range(2):
+        assert np.array_equal(ws.window_model_t(k, mt).numpy(), ws.window_model(k, m))
+    per = [ws.window_model(k, m) for k in range(2)]
+    assert np.array_equal(ws.stack_t([torch.from_numpy(a) for a in per], fill=1.0).numpy(), ws.stack(per, fill=1.0))
+    for k in range(2):
+        g = rng.standard_normal(per[k].shape)
+        a = np.zeros_like(m)
+        ws.window_adjoint(k, g, a)
+        b = torch.zeros_like(mt)
+        ws.window_adjoint_t(k, torch.from_numpy(g), b)
+        assert np.allclose(b.numpy(), a, rtol=1e-13, atol=1e-13)
+
+
+def test_medium_scaling_and_its_transpose_in_torch_match_the_library():
+    """medium_torch.scale_medium / medium_gradient give the same numbers as
+    the library's host ela_scale_medium / ela_medium_gradient (the same
+    operations in the same order)."""
+    import torch
+    from elasticisolib_3d_amd import medium_torch as MT
+    rng = np.random.default_rng(6)
+    shape = (9, 11, 13)
+    rho = 2000 * (1 + 0.2 * rng.random(shape))
+    mu = 3e9 * (1 + 0.2 * rng.random(shape))
+    lam = 4e9 * (1 + 0.2 * rng.random(shape))
+    dtw = 5e-4
+    med = P.scale_medium(rho, lam, mu, dtw)
+    medt = MT.scale_medium(torch.from_numpy(rho), torch.from_numpy(lam), torch.from_numpy(mu), dtw).numpy()
+    assert np.array_equal(medt, med)
+    g8 = rng.standard_normal((8,) + shape)
+    g3 = P.medium_gradient(g8, med, dtw)
+    g3t = MT.medium_gradient(torch.from_numpy(g8), torch.from_numpy(med), dtw).numpy()
+    assert np.allclose(g3t, g3, rtol=1e-14, atol=0)

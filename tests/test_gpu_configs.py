@@ -1,0 +1,250 @@
+"""The library on the five benchmark configurations at their real sizes.
+
+C1  100^3 homogeneous model, point force, the full 0.6 s record (151 coarse
+    samples, 1200 fine steps, 81 receivers): FP64 and FP32 against the FP64
+    CPU oracle.
+C2  400x400x250 layered model, explosive source, the full 2.5 s record (5000
+    fine steps, 200x200 receivers, 9 components): FP32 against FP64 on the GPU
+    (the bar for FP32 forward data is 1e-5).
+C3  the Born pair at this size is covered by tests/test_gpu_fullsize.py (FP64
+    dot test on the padded 328x472x468 grid).
+C4  256x256x160 FWI gradient, record shortened to 0.5 s (126 coarse samples,
+    1000 fine steps): FP32 with the default int16 checkpoints against the FP64
+    gradient with exact checkpoints (bar 1e-3), and the FP64 gradient against
+    central differences of the FP64 misfit on 4 cells.
+C5  the 64 shot-local windows of the 400x400x250 model with their receiver
+    counts, record shortened to 6 coarse samples (40 fine steps): each shot's
+    receiver traces and gradient from 4-window stacks (batched) equal its
+    propagation alone (plain), FP32 with exact checkpoints.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    num = sum(float(((x - y) ** 2).sum()) for x, y in zip(a, b))
+    den = sum(float((y ** 2).sum()) for y in b)
+    return (num / den) ** 0.5
+
+
+# ---- C1 -------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c1():
+    from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
+    from elasticisolib_3d_amd.models import homogeneous_100, source_traces
+    w = homogeneous_100()
+    a = w.axes
+    g = O.Grid(a.nz, a.nx, a.ny, w.nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos)
+    med = O.lib_scale_medium(w.rho, w.lame, w.mu, w.dtw)
+    sg = ShotGeometry(a, w.src_xyz[:, 2], w.src_xyz[:, 0], w.src_xyz[:, 1])
+    rg = ShotGeometry(a, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
+    src = scale_regular_traces(med, sg, sg.to_regular(source_traces(w)), a.dx * a.dy * a.dz)
+    print(f"C1: oracle, {a.nz}x{a.nx}x{a.ny}, {(w.nts - 1) * w.sub} fine steps, {O.num_threads()} threads",
+          flush=True)
+    ref = O.lib_nl_forward(g, med, src, sg.positions(), rg.positions())
+    return w, g, med, src, sg, rg, ref
+
+
+@pytest.mark.parametrize("precision,medium,tol", [("f64", "scaled", 1e-11), ("f32", "scaled", 5e-6),
+                                                  ("f32", "elastic", 5e-6)])
+def test_c1_full_record_against_the_oracle(c1, precision, medium, tol):
+    from elasticisolib_3d_amd.propagator import Propagator
+    w, g, med, src, sg, rg, ref = c1
+    prop = Propagator(g.nz, g.nx, g.ny, g.nts, g.sub, g.dz, g.dx, g.dy, g.dts, g.minPad, g.alphaCos, precision)
+    if medium == "elastic":
+        prop.set_elastic(np.stack([w.rho, w.lame, w.mu]), w.dtw)
+    else:
+        prop.set_medium(med)
+    prop.set_graph(True)
+    out = prop.nl_forward(sg.positions(), src, rg.positions())
+    e = _rel(out, ref)
+    print(f"C1 {precision} {medium}: rel L2 {e:.2e} over {len(rg.positions()[1])} vx points x {g.nts} samples",
+          flush=True)
+    assert e < tol
+    assert float(np.abs(out[2]).max()) > 0
+
+
+# ---- C2 -------------------------------------------------------------------
+
+def test_c2_full_record_fp32_against_fp64():
+    import torch
+    from elasticisolib_3d_amd import propagator as P
+    from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
+    from elasticisolib_3d_amd.models import layered_400, source_traces
+    w = layered_400()
+    a = w.axes
+    med = P.scale_medium(w.rho, w.lame, w.mu, w.dtw)
+    sg = ShotGeometry(a, w.src_xyz[:, 2], w.src_xyz[:, 0], w.src_xyz[:, 1])
+    rg = ShotGeometry(a, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
+    src = scale_regular_traces(med, sg, sg.to_regular(source_traces(w)), a.dx * a.dy * a.dz)
+    del med
+    rlm = np.stack([w.rho, w.lame, w.mu])
+    out = {}
+    for precision in ("f64", "f32"):
+        dt = torch.float64 if precision == "f64" else torch.float32
+        print(f"C2 {precision}: {a.nz}x{a.nx}x{a.ny}, {w.fine_steps} fine steps", flush=True)
+        prop = P.Propagator(a.nz, a.nx, a.ny, w.nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos, precision)
+        prop.set_elastic(torch.tensor(rlm, dtype=dt, device="cuda"), w.dtw)
+        prop.set_graph(precision == "f32")
+        spos = [torch.tensor(p, device="cuda") for p in sg.positions()]
+        rpos = [torch.tensor(p, device="cuda") for p in rg.positions()]
+        rec = prop.nl_forward(spos, [torch.tensor(s, dtype=dt, device="cuda") for s in src], rpos)
+        torch.cuda.synchronize()
+        out[precision] = [r.double() for r in rec]
+        del prop, rec
+        torch.cuda.empty_cache()
+    e = _rel(out["f32"], out["f64"])
+    per = [_rel([x], [y]) for x, y in zip(out["f32"], out["f64"])]
+    print(f"C2 FP32 vs FP64: rel L2 {e:.2e}, per component {np.round(per, 8)}", flush=True)
+    assert e < 1e-5
+    assert all(np.isfinite(per))
+
+
+# ---- C4 -------------------------------------------------------------------
+
+def test_c4_gradient_shortened_record():
+    import torch
+    from elasticisolib_3d_amd.acquisition import ShotGeometry
+    from elasticisolib_3d_amd.fwi import FwiProblem, FwiShot
+    from elasticisolib_3d_amd.models import fwi_256
+    w = fwi_256(nts=126, smooth_device="cuda")
+    ax = w.axes
+    start = w.extra["start"]
+    rec = ShotGeometry(ax, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
+    xyz = w.extra["shots"][0]
+    src = ShotGeometry(ax, xyz[:, 2], xyz[:, 0], xyz[:, 1])
+    sig = [np.zeros((1, w.nts)) for _ in range(9)]
+    for c in (3, 4, 5):
+        sig[c][0] = w.wavelet
+    res = {}
+    for name, prec, ck in (("f64_exact", "f64", "native"), ("f32_default", "f32", None)):
+        kw = {} if ck is None else {"checkpoint": ck}
+        p = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, [FwiShot(src, rec, sig)], prec, interval=20, **kw)
+        p.synthesize(w.rho, w.lame, w.mu)
+        print(f"C4 {name}: {ax.nz}x{ax.nx}x{ax.ny}, {w.fine_steps} fine steps", flush=True)
+        J, g3 = p.misfit_and_gradient(*start)
+        res[name] = (p, J, g3)
+        if name == "f32_default":
+            assert p.prop.gradient_plan_used()[2] == "s16"
+            del p
+            torch.cuda.empty_cache()
+    g64, g32 = res["f64_exact"][2], res["f32_default"][2]
+    e = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    per = [float(np.linalg.norm(g32[k] - g64[k]) / np.linalg.norm(g64[k])) for k in range(3)]
+    print(f"C4 FP32 (int16 checkpoints) vs FP64 gradient: rel L2 {e:.2e} (rho, lambda, mu {np.round(per, 7)})",
+          flush=True)
+    assert e < 1e-3
+    # FP64 gradient against central differences of the FP64 misfit on 4 illuminated cells
+    p64 = res["f64_exact"][0]
+    score = np.abs(g64 * start).ravel()
+    top = np.argsort(score)[-score.size // 20:]
+    pick = np.random.default_rng(3).choice(top, size=4, replace=False)
+    h = 1e-3
+    for q in pick:
+        k, rest = divmod(int(q), start[0].size)
+        idx = np.unravel_index(rest, start[0].shape)
+        mp = [m.copy() for m in start]
+        mm = [m.copy() for m in start]
+        mp[k][idx] *= 1 + h
+        mm[k][idx] *= 1 - h
+        fd = (p64.misfit(*mp) - p64.misfit(*mm)) / (2 * h * start[k][idx])
+        an = float(g64[k][idx])
+        print(f"C4 FD cell param {k} {tuple(int(i) for i in idx)}: FD {fd:.8e} adjoint {an:.8e} "
+              f"rel {abs(fd - an) / abs(an):.2e}", flush=True)
+        assert abs(fd - an) <= 1e-4 * abs(an)
+
+
+# ---- C5 -------------------------------------------------------------------
+
+def _stack_on_device(ws, m):
+    """ws.stacked_elastic on the GPU: windows cut from m ([3, ny, nx, nz]
+    torch, device) and edge-replicated outwards, stacked along y, 1 beyond
+    narrower windows."""
+    import torch
+    import torch.nn.functional as F
+    from elasticisolib_3d_amd.models import FAT
+    out = torch.ones((3, ws.ny, ws.nx, ws.nz), dtype=m.dtype, device=m.device)
+    zm, zp = ws.zpads
+    for k, w in enumerate(ws.windows):
+        xm, xp = ws.xpads[k]
+        core = m[:, w.y0:w.y0 + w.ny, w.x0:w.x0 + w.nx, :][None]
+        win = F.pad(core, (FAT + zm, FAT + zp, FAT + xm, FAT + xp, FAT + ws.pad, FAT + ws.pad), mode="replicate")[0]
+        out[:, ws.y_off[k]:ws.y_off[k] + ws.ny_w[k], :ws.nx_w[k], :] = win
+    return out
+
+
+def _run_stack(shots, model_dev, cfg, per_stack):
+    """Receiver traces (per shot, 9 device arrays) and FP64 gradients (per
+    shot, [8, ny_w, nx_w, nz] device) of the shots, `per_stack` windows per
+    propagator; source traces unscaled (the same inputs for both paths)."""
+    import torch
+    from elasticisolib_3d_amd import propagator as P
+    from elasticisolib_3d_amd.acquisition import COMP_GRID, ShotGeometry
+    from elasticisolib_3d_amd.batch import Window, WindowStack
+    ny_m, nx_m, nz_m = model_dev.shape[1:]
+    traces, grads = [], []
+    for i in range(0, len(shots), per_stack):
+        group = shots[i:i + per_stack]
+        ws = WindowStack((nz_m, nx_m, ny_m), cfg["d"], cfg["pad"], [Window(*s[2]) for s in group])
+        prop = P.Propagator(ws.nz, ws.nx, ws.ny, cfg["nts"], cfg["sub"], cfg["d"], cfg["d"], cfg["d"], cfg["dts"],
+                            ws.min_pad, cfg["alpha_cos"], "f32")
+        ws.set_windows(prop)
+        prop.set_elastic(_stack_on_device(ws, model_dev).float(), cfg["dts"] / cfg["sub"])
+        prop.set_gradient_checkpoints(4, 0, "native")
+        geos = []
+        for k, (src_xyz, rec_xyz, _) in enumerate(group):
+            ax = ws.axes[k]
+            geos.append((ShotGeometry(ax, src_xyz[:, 2], src_xyz[:, 0], src_xyz[:, 1]),
+                         ShotGeometry(ax, rec_xyz[:, 2], rec_xyz[:, 0], rec_xyz[:, 1])))
+        spos = [np.concatenate(p) for p in zip(*[ws.stacked_positions(k, g[0].positions()) for k, g in enumerate(geos)])]
+        rpos = [np.concatenate(p) for p in zip(*[ws.stacked_positions(k, g[1].positions()) for k, g in enumerate(geos)])]
+        sig = [np.zeros((1, cfg["nts"])) for _ in range(9)]
+        for c in (3, 4, 5):
+            sig[c][0] = cfg["wavelet"] * 1e9
+        src = [torch.tensor(np.concatenate([g[0].to_regular(sig)[c] for g in geos]), dtype=torch.float32,
+                            device="cuda") for c in range(9)]
+        sp = [torch.tensor(p, device="cuda") for p in spos]
+        rp = [torch.tensor(p, device="cuda") for p in rpos]
+        rec = prop.gradient_forward(sp, src, rp)
+        adj = [r * 1e-3 for r in rec]
+        g8, _ = prop.gradient_backward(sp, src, rp, adj)
+        torch.cuda.synchronize()
+        off = [0] * 7
+        for k, (sg, rg) in enumerate(geos):
+            cnt = rg.counts()
+            traces.append([rec[c][off[COMP_GRID[c]]:off[COMP_GRID[c]] + cnt[COMP_GRID[c]]] for c in range(9)])
+            for gi in range(7):
+                off[gi] += cnt[gi]
+            grads.append(g8[:, ws.y_off[k]:ws.y_off[k] + ws.ny_w[k], :ws.nx_w[k], :].clone())
+        del prop, g8, rec, adj
+        torch.cuda.empty_cache()
+    return traces, grads
+
+
+def test_c5_64_windows_batched_equal_plain():
+    import torch
+    from elasticisolib_3d_amd.models import batch_layered_400
+    cfg = batch_layered_400(nts=6, smooth_device="cuda")
+    shots = cfg["shots"]
+    assert len(shots) == 64
+    nrec = [len(s[1]) for s in shots]
+    print(f"C5: 64 windows {min(s[2][2] for s in shots)}-{max(s[2][2] for s in shots)} x "
+          f"{min(s[2][3] for s in shots)}-{max(s[2][3] for s in shots)} cells, {min(nrec)}-{max(nrec)} receivers",
+          flush=True)
+    model = torch.tensor(cfg["start"], dtype=torch.float64, device="cuda")
+    plain_tr, plain_g = _run_stack(shots, model, cfg, 1)
+    print("C5: plain done", flush=True)
+    batch_tr, batch_g = _run_stack(shots, model, cfg, 4)
+    print("C5: batched done", flush=True)
+    worst_d = max(_rel([x.double() for x in b], [y.double() for y in p]) for b, p in zip(batch_tr, plain_tr))
+    worst_g = max(float(torch.linalg.norm(b - p) / torch.linalg.norm(p)) for b, p in zip(batch_g, plain_g))
+    same = all(bool(torch.equal(b, p)) for b, p in zip(batch_g, plain_g))
+    print(f"C5 batched vs plain: worst data rel {worst_d:.2e}, worst gradient rel {worst_g:.2e}, "
+          f"gradients bit-identical {same}", flush=True)
+    assert worst_d < 1e-6 and worst_g < 1e-6
+    assert all(float(torch.linalg.norm(g)) > 0 for g in plain_g)
