@@ -28,7 +28,7 @@ EXPORTS = (
     "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
     "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
     "ela_set_windows", "ela_set_graph", "ela_stream_probe", "ela_set_elastic", "ela_set_born_mode",
-    "ela_nl_forward_noalloc",
+    "ela_nl_forward_noalloc", "ela_get_medium",
 )
 
 
@@ -119,6 +119,7 @@ def load() -> ctypes.CDLL:
     lib.ela_set_graph.argtypes = [vp, i32]
     lib.ela_stream_probe.argtypes = [vp, i32, i32, i64, i32, ctypes.POINTER(ctypes.c_double)]
     lib.ela_set_elastic.argtypes = [vp, vp, ctypes.c_double, i32]
+    lib.ela_get_medium.argtypes = [vp, vp, i32]
     lib.ela_gradient_forward.argtypes = [vp, P, Tr, P, Tr, i32]
     lib.ela_gradient_backward.argtypes = [vp, P, Tr, P, Tr, vp, Tr, i32]
     lib.ela_set_gradient_checkpoints.argtypes = [vp, i64, i64, i32, i64]

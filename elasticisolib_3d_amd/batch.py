@@ -227,12 +227,13 @@ class BatchedFwi:
     model arrays; `checkpoint` is "s16", "bf16" or "native", and the older
     `bf16` flag, when given, means "bf16" / "native"; `autotune` times the launch shapes of every stacked grid
     unless `tiling` fixes them; `medium` ("elastic" or "scaled") chooses how
-    the stacked model reaches the library, as in FwiProblem."""
+    the stacked model reaches the library, as in FwiProblem; `graph` replays
+    the gradient's time steps as HIP graphs (as FwiProblem)."""
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
                  bf16=None, device: int = 0, tiling=None, checkpoint: str = "s16", autotune: bool = False,
-                 medium: str = "elastic"):
+                 medium: str = "elastic", graph: bool = True):
         self.model_shape, self.d, self.pad = tuple(model_shape), float(d), int(pad)
         self.nts, self.dts, self.sub, self.alpha = int(nts), float(dts), int(sub), float(alpha_cos)
         self.dtw = self.dts / self.sub
@@ -247,6 +248,7 @@ class BatchedFwi:
         if medium not in ("elastic", "scaled"):
             raise ValueError("medium must be 'elastic' or 'scaled'")
         self.medium = medium
+        self.graph = bool(graph)   # replay the gradient's step pairs and checkpoint segments as HIP graphs
         self.timing = {}
 
     def _groups(self):
@@ -284,6 +286,7 @@ class BatchedFwi:
             from .autotune import autotune
             autotune(prop, reps=5)
         prop.set_gradient_checkpoints(self.interval, 0, self.checkpoint)
+        prop.set_graph(self.graph)
         geos = []
         for k, s in enumerate(group):
             ax = ws.axes[k]

@@ -31,6 +31,8 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/elastic_iso_3d.h"
@@ -91,6 +93,24 @@ struct DevBuf {
 };
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// An executable graph kept across calls together with the signature of the
+// launches it was captured from (kernel, launch shape and argument bytes of
+// every launch, in order).
+struct GraphCache {
+  std::vector<unsigned char> sig;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;   // recorded on the stream after the exec's last replay
+};
+
+// The handle's graph caches.
+enum { G_NL = 0,      // one coarse step of the nonlinear forward
+       G_NL_ADJ,      // one coarse step of the nonlinear adjoint
+       G_FWD,         // two gradient forward steps (no recording)
+       G_FWD_REC,     // two gradient forward steps with receiver recording
+       G_STORE,       // a checkpoint segment of forward steps storing their derivative combinations
+       G_ADJ,         // a checkpoint segment of transposed steps with imaging
+       G_COUNT };
 
 }  // namespace
 
@@ -204,14 +224,16 @@ struct ela_handle {
   DevBuf chunks;            // per y chunk {y0, y1, window y0, ny, nx, 0, 0, 0} (int32)
   int chunks_yc = 0;        // y chunk the table was built for
   int nchunks = 0;
-  // graph replay of the coarse step of the nonlinear propagations
+  // graph replays of the time loops (ela_set_graph): the coarse step of the
+  // nonlinear propagations, the fine-step pairs and checkpoint segments of the
+  // gradient; each executable graph is kept until the launches it holds change
+  // or the handle is destroyed
   bool use_graph = false;
-  DevBuf counter;           // device coarse-step counter (one int)
+  DevBuf counter;           // device step counters (coarse sample, fine step)
+  DevBuf dfilt;             // the time filter in the handle's precision (replayed point kernels read it)
   hipStream_t cap_stream = nullptr;
-  // the last run's executable graph, destroyed once its launches have run
-  // (nl_exec_done is recorded after them)
-  hipGraphExec_t nl_exec = nullptr;
-  hipEvent_t nl_exec_done = nullptr;
+  GraphCache graphs[G_COUNT];
+  std::vector<unsigned char>* rec = nullptr;  // launch recorder of graph_for's signature pass
 };
 
 namespace {
@@ -219,6 +241,98 @@ namespace {
 template <typename T>
 T* arr(DevBuf& b, int64_t stride, int k) {
   return reinterpret_cast<T*>(b.p) + stride * k;
+}
+
+template <typename V>
+void sig_put(std::vector<unsigned char>& s, const V& v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  s.insert(s.end(), p, p + sizeof(V));
+}
+
+// Every kernel launch of the library goes through here. While the handle's
+// launch recorder is set (the signature pass of a graph, see graph_for) the
+// kernel, its launch shape and its argument bytes are appended to the
+// recorder and nothing is launched.
+template <typename... KP, typename... A>
+void launch(ela_handle* h, void (*kern)(KP...), dim3 grid, dim3 block, const A&... args) {
+  static_assert(sizeof...(KP) == sizeof...(A), "kernel argument count");
+  std::tuple<std::decay_t<KP>...> v(static_cast<std::decay_t<KP>>(args)...);
+  if (h->rec) {
+    sig_put(*h->rec, reinterpret_cast<const void*>(kern));
+    sig_put(*h->rec, grid);
+    sig_put(*h->rec, block);
+    std::apply([&](const auto&... x) { (sig_put(*h->rec, x), ...); }, v);
+    return;
+  }
+  std::apply([&](auto&... x) {
+    void* ptrs[] = {static_cast<void*>(&x)...};
+    (void)hipLaunchKernel(reinterpret_cast<const void*>(kern), grid, block, ptrs, 0, h->stream);
+  }, v);
+}
+
+// Destroys a cached executable graph once its launches have completed.
+int drop_graph(GraphCache& gc) {
+  if (!gc.exec) return ELA_OK;
+  HIPCHK(hipEventSynchronize(gc.done));
+  HIPCHK(hipGraphExecDestroy(gc.exec));
+  gc.exec = nullptr;
+  gc.sig.clear();
+  return ELA_OK;
+}
+
+// Executable graph of the launches `body` issues on h->stream. The body is
+// first run with the launch recorder set; if the recorded launches equal those
+// the cached graph was captured from, the cached graph is returned as it is
+// (the same kernels, launch shapes and arguments, hence the same work);
+// otherwise the body is captured and instantiated once more and replaces the
+// cache. The body must issue nothing but launches through launch() and must
+// leave the host state it reads as it found it (both passes see the same
+// state). The caller records gc.done on h->stream after its last replay.
+template <typename F>
+int graph_for(ela_handle* h, GraphCache& gc, F&& body, hipGraphExec_t* out) {
+  std::vector<unsigned char> sig;
+  sig.reserve(gc.sig.size());
+  h->rec = &sig;
+  const int rs = body();
+  h->rec = nullptr;
+  CHK(rs);
+  if (gc.exec && sig == gc.sig) {
+    *out = gc.exec;
+    return ELA_OK;
+  }
+  CHK(drop_graph(gc));
+  if (!gc.done) HIPCHK(hipEventCreateWithFlags(&gc.done, hipEventDisableTiming));
+  if (!h->cap_stream) HIPCHK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+  hipStream_t keep = h->stream;
+  h->stream = h->cap_stream;
+  const int rc = body();
+  h->stream = keep;
+  const hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
+  if (rc != ELA_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(ELA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(ELA_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+  gc.exec = exec;
+  gc.sig.swap(sig);
+  *out = exec;
+  return ELA_OK;
+}
+
+// Replays `exec` n times on h->stream and marks the cache's completion event.
+int replay(ela_handle* h, GraphCache& gc, hipGraphExec_t exec, int64_t n) {
+  for (int64_t k = 0; k < n; ++k) {
+    const hipError_t e = hipGraphLaunch(exec, h->stream);
+    if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
+  }
+  HIPCHK(hipEventRecord(gc.done, h->stream));
+  return ELA_OK;
 }
 
 int check_cfg(const ela_config* c) {
@@ -280,16 +394,6 @@ int ensure_chunks(ela_handle* h) {
   return ELA_OK;
 }
 
-// Destroy the previous run's executable graph after its in-flight launches
-// have completed (the launches may still reference it until they run).
-int release_exec(ela_handle* h) {
-  if (!h->nl_exec) return ELA_OK;
-  HIPCHK(hipEventSynchronize(h->nl_exec_done));
-  HIPCHK(hipGraphExecDestroy(h->nl_exec));
-  h->nl_exec = nullptr;
-  return ELA_OK;
-}
-
 Tiles tiles_for(const ela_handle* h, int bx, int ychunk) {
   Tiles t;
   const int64_t iz = h->cfg.nz - 2 * ela::FAT, ix = h->cfg.nx - 2 * ela::FAT, iy = h->cfg.ny - 2 * ela::FAT;
@@ -334,8 +438,8 @@ int launch_step_bx(ela_handle* h, ela::StepArgs<T> a, int which) {
   Tiles t = tiles_for(h, BX, a.ychunk);
   a.nbz = t.nbz; a.nbx = t.nbx; a.nby = t.nby;
   dim3 block(ela::TZ, BX);
-  if (which & 1) hipLaunchKernelGGL((ela::k_velocity<T, BX, MODE, FV>), dim3(t.nblocks), block, 0, h->stream, a);
-  if (which & 2) hipLaunchKernelGGL((ela::k_stress<T, BX, MODE, FS>), dim3(t.nblocks), block, 0, h->stream, a);
+  if (which & 1) launch(h, ela::k_velocity<T, BX, MODE, FV>, dim3(t.nblocks), block, a);
+  if (which & 2) launch(h, ela::k_stress<T, BX, MODE, FS>, dim3(t.nblocks), block, a);
   return ELA_OK;
 }
 
@@ -365,7 +469,7 @@ template <typename T>
 int fill(ela_handle* h, void* p, int64_t n) {
   if (n <= 0) return ELA_OK;
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL((ela::k_fill<T>), dim3((unsigned)blocks), dim3(256), 0, h->stream, (T*)p, n, T(0));
+  launch(h, ela::k_fill<T>, dim3((unsigned)blocks), dim3(256), (T*)p, n, T(0));
   return ELA_OK;
 }
 
@@ -431,7 +535,7 @@ int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, in
   int64_t* s64 = (int64_t*)base;
   int32_t* s32 = (int32_t*)(base + b64);
   int* flag = (int*)(base + b64 + b32);
-  hipLaunchKernelGGL((ela::k_fill<int>), dim3(1), dim3(64), 0, h->stream, flag, (int64_t)1, 0);
+  launch(h, ela::k_fill<int>, dim3(1), dim3(64), flag, (int64_t)1, 0);
   int64_t off = 0;
   for (int side = 0; side < 2; ++side) {
     const ela_points* p = side ? rp : sp;
@@ -446,7 +550,7 @@ int convert_points(ela_handle* h, const ela_points* sp, const ela_points* rp, in
           HIPCHK(hipMemcpyAsync(s64 + off, p->pos[g], n * 8, hipMemcpyHostToDevice, h->stream));
           src = s64 + off;
         }
-        hipLaunchKernelGGL(ela::k_convert_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src,
+        launch(h, ela::k_convert_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), src,
                            s32 + off, n, (int)h->cfg.nz, (int)h->cfg.nx, (int)h->cfg.ny, (int)h->nzi, (int)h->zoff,
                            h->win_y0.empty() ? nullptr : (const int*)h->ytab.p, flag);
       }
@@ -486,27 +590,47 @@ ela::PointArgs<T> make_points(ela_handle* h, const DevPoints& pts, T* const fiel
   return a;
 }
 
+// Host time arguments of a point launch. A launch that reads its coarse sample
+// (its_dev) or its fine step (step_dev) from device memory keeps them out of
+// its arguments, so that the replayed graph's launches do not depend on them.
+template <typename T>
+void set_time(ela::PointArgs<T>& a, int its, double fa, double fb) {
+  a.its = a.its_dev || a.step_dev ? 0 : its;
+  a.fa = a.step_dev ? T(0) : T(fa);
+  a.fb = a.step_dev ? T(0) : T(fb);
+}
+
+// The point launch reads its fine step from kdev and its weights from the
+// handle's filter table (offset fo: 0 for filt[i2], 1 for filt[i2 + 1]).
+template <typename T>
+void set_step(ela_handle* h, ela::PointArgs<T>& a, const int* kdev, int fo) {
+  a.step_dev = kdev;
+  a.filt = static_cast<const T*>(h->dfilt.p);
+  a.sub = (int)h->cfg.sub;
+  a.fo = fo;
+}
+
 template <typename T, bool ACC, bool GRAD = false, bool PM = false>
 void launch_inject(ela_handle* h, ela::PointArgs<T> a, int its, double fa, double fb) {
   if (a.off[9] == 0) return;
-  a.its = its; a.fa = T(fa); a.fb = T(fb);
+  set_time(a, its, fa, fb);
   const unsigned nb = (unsigned)((a.off[9] + 255) / 256);
-  hipLaunchKernelGGL((ela::k_inject<T, ACC, GRAD, PM>), dim3(nb), dim3(256), 0, h->stream, a);
+  launch(h, ela::k_inject<T, ACC, GRAD, PM>, dim3(nb), dim3(256), a);
 }
 
 template <typename T>
 void launch_record(ela_handle* h, ela::PointArgs<T> a, int its, double fa, double fb) {
   if (a.off[9] == 0) return;
-  a.its = its; a.fa = T(fa); a.fb = T(fb);
+  set_time(a, its, fa, fb);
   const unsigned nb = (unsigned)((a.off[9] + 255) / 256);
-  hipLaunchKernelGGL((ela::k_record<T>), dim3(nb), dim3(256), 0, h->stream, a);
+  launch(h, ela::k_record<T>, dim3(nb), dim3(256), a);
 }
 
 template <typename T>
 void launch_transpose(ela_handle* h, const T* in, T* out, int64_t rows, int64_t cols, int64_t ld_in, int64_t ld_out) {
   if (rows <= 0 || cols <= 0) return;
   dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
-  hipLaunchKernelGGL((ela::k_transpose<T>), grid, dim3(256), 0, h->stream, in, out, rows, cols, ld_in, ld_out);
+  launch(h, ela::k_transpose<T>, grid, dim3(256), in, out, rows, cols, ld_in, ld_out);
 }
 
 // ---- trace staging for host-memory calls ------------------------------------
@@ -716,37 +840,20 @@ int nl_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, cons
   const bool graph = h->use_graph && sub % 2 == 0 && !scratch;
   hipGraphExec_t exec = nullptr;
   if (graph) {
-    CHK(release_exec(h));
-    if (!h->nl_exec_done) HIPCHK(hipEventCreateWithFlags(&h->nl_exec_done, hipEventDisableTiming));
     CHK(ensure_chunks(h));  // no synchronous upload inside the capture
     CHK(h->counter.alloc(256));
     int* cnt = (int*)h->counter.p;
-    hipLaunchKernelGGL((ela::k_fill<int>), dim3(1), dim3(64), 0, h->stream, cnt, (int64_t)1,
-                       adjoint ? (int)(nts - 2) : 0);
-    if (!h->cap_stream) HIPCHK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
-    hipStream_t keep = h->stream;
-    h->stream = h->cap_stream;
-    int rc = coarse_step(0, cnt);
-    if (rc == ELA_OK) hipLaunchKernelGGL(ela::k_count, dim3(1), dim3(64), 0, h->stream, cnt, adjoint ? -1 : 1);
-    h->stream = keep;
-    const hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
-    if (rc != ELA_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (ec != hipSuccess) return fail(ELA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
-    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ei != hipSuccess) return fail(ELA_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+    auto body = [&]() -> int {
+      CHK(coarse_step(0, cnt));
+      launch(h, ela::k_count, dim3(1), dim3(64), cnt, adjoint ? -1 : 1);
+      return ELA_OK;
+    };
+    CHK(graph_for(h, h->graphs[adjoint ? G_NL_ADJ : G_NL], body, &exec));
+    launch(h, ela::k_fill<int>, dim3(1), dim3(64), cnt, (int64_t)1, adjoint ? (int)(nts - 2) : 0);
   }
   HIPCHK(hipEventRecord(h->ev0, h->stream));
   if (graph) {
-    hipError_t e = hipSuccess;
-    for (int64_t k = 0; k < nts - 1 && e == hipSuccess; ++k) e = hipGraphLaunch(exec, h->stream);
-    // keep the executable graph until its launches have run (release_exec)
-    h->nl_exec = exec;
-    const hipError_t er = hipEventRecord(h->nl_exec_done, h->stream);
-    if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
-    if (er != hipSuccess) return fail(ELA_ERR_HIP, "event record failed: %s", hipGetErrorString(er));
+    CHK(replay(h, h->graphs[adjoint ? G_NL_ADJ : G_NL], exec, nts - 1));
   } else if (!adjoint) {
     for (int64_t its = 0; its < nts - 1; ++its) CHK(coarse_step(its, nullptr));
   } else {
@@ -853,7 +960,7 @@ ela::ImagingArgs<T> imaging_args(ela_handle* h, const T* Vo, const T* Vc, const 
 template <typename T, bool ADJ>
 void launch_imaging(ela_handle* h, const ela::ImagingArgs<T>& a) {
   Tiles t = tiles_for(h, 8, a.ychunk);
-  hipLaunchKernelGGL((ela::k_imaging<T, 8, ADJ>), dim3(t.nblocks), dim3(ela::TZ, 8), 0, h->stream, a);
+  launch(h, ela::k_imaging<T, 8, ADJ>, dim3(t.nblocks), dim3(ela::TZ, 8), a);
 }
 
 // Storage plan of the Born adjoint's coarse wavefield.
@@ -1106,8 +1213,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
       CHK(h->dmod64.alloc(8 * np * sizeof(double)));
       dm64 = (double*)h->dmod64.p;
       CHK(fill<double>(h, dm64, 8 * np));
-      hipLaunchKernelGGL((ela::k_fold<T>), dim3((unsigned)std::min<int64_t>((8 * np + 255) / 256, 65536)), dim3(256),
-                         0, h->stream, dm, dm64, 8 * np);
+      launch(h, ela::k_fold<T>, dim3((unsigned)std::min<int64_t>((8 * np + 255) / 256, 65536)), dim3(256), dm, dm64, 8 * np);
     }
   }
   HIPCHK(hipEventRecord(h->ev0, h->stream));
@@ -1115,8 +1221,7 @@ int born_run(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in, co
   else CHK(born_adjoint_run<T>(h, sp, src_tr, rp, tm, nrec_tot, dm64));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   if (adjoint && sizeof(T) == 4)
-    hipLaunchKernelGGL((ela::k_narrow<T>), dim3((unsigned)std::min<int64_t>((8 * np + 255) / 256, 65536)), dim3(256),
-                       0, h->stream, dm64, dm, 8 * np);
+    launch(h, ela::k_narrow<T>, dim3((unsigned)std::min<int64_t>((8 * np + 255) / 256, 65536)), dim3(256), dm64, dm, 8 * np);
   HIPCHK(hipGetLastError());
 
   if (!adjoint) {
@@ -1241,16 +1346,16 @@ int save_checkpoint(ela_handle* h, const Wave<T>& w, void* out, int fmt, float* 
   ela::CkArgs<T> a = ck_args<T>(h, w);
   const dim3 grid = ck_grid(a.nI);
   if (fmt == ELA_CK_S16) {
-    hipLaunchKernelGGL((ela::k_ck_pack_s16<T>), grid, dim3(256), 0, h->stream, a, static_cast<int16_t*>(out));
+    launch(h, ela::k_ck_pack_s16<T>, grid, dim3(256), a, static_cast<int16_t*>(out));
     return ELA_OK;
   }
   uint32_t* amax = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(h->ckscale.p) + h->ckscale.bytes - 18 * 4);
   if (fmt == ELA_CK_BF16) {
-    hipLaunchKernelGGL((ela::k_fill<uint32_t>), dim3(1), dim3(64), 0, h->stream, amax, (int64_t)18, 0u);
-    hipLaunchKernelGGL((ela::k_ck_amax<T>), grid, dim3(256), 0, h->stream, a, amax);
-    hipLaunchKernelGGL((ela::k_ck_pack<T, true>), grid, dim3(256), 0, h->stream, a, out, amax, scale);
+    launch(h, ela::k_fill<uint32_t>, dim3(1), dim3(64), amax, (int64_t)18, 0u);
+    launch(h, ela::k_ck_amax<T>, grid, dim3(256), a, amax);
+    launch(h, ela::k_ck_pack<T, true>, grid, dim3(256), a, out, amax, scale);
   } else {
-    hipLaunchKernelGGL((ela::k_ck_pack<T, false>), grid, dim3(256), 0, h->stream, a, out, amax, scale);
+    launch(h, ela::k_ck_pack<T, false>, grid, dim3(256), a, out, amax, scale);
   }
   return ELA_OK;
 }
@@ -1259,16 +1364,19 @@ template <typename T>
 int load_checkpoint(ela_handle* h, const Wave<T>& w, const void* in, int fmt, const float* scale) {
   ela::CkArgs<T> a = ck_args<T>(h, w);
   if (fmt == ELA_CK_S16)
-    hipLaunchKernelGGL((ela::k_ck_unpack_s16<T>), ck_grid(a.nI), dim3(256), 0, h->stream, a,
+    launch(h, ela::k_ck_unpack_s16<T>, ck_grid(a.nI), dim3(256), a,
                        static_cast<const int16_t*>(in));
-  else if (fmt == ELA_CK_BF16) hipLaunchKernelGGL((ela::k_ck_unpack<T, true>), ck_grid(a.nI), dim3(256), 0, h->stream, a, in, scale);
-  else hipLaunchKernelGGL((ela::k_ck_unpack<T, false>), ck_grid(a.nI), dim3(256), 0, h->stream, a, in, scale);
+  else if (fmt == ELA_CK_BF16) launch(h, ela::k_ck_unpack<T, true>, ck_grid(a.nI), dim3(256), a, in, scale);
+  else launch(h, ela::k_ck_unpack<T, false>, ck_grid(a.nI), dim3(256), a, in, scale);
   return ELA_OK;
 }
 
-// One forward fine step k (source injection, optional store of D(c_k)).
+// One forward fine step k (source injection, optional store of D(c_k)). With
+// kdev the step index is read from device memory and counted up there (graph
+// replays); k is then unused.
 template <typename T, bool STORE>
-int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k, T* dst) {
+int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k, T* dst,
+                  int* kdev = nullptr) {
   const int64_t sub = h->cfg.sub, hh = sub + 1, np = h->stride;
   const double* f = h->filt.data();
   const int its = (int)(k / sub), i2 = (int)(k % sub);  // forward it2 = i2 + 1
@@ -1279,8 +1387,54 @@ int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_t
   } else {
     CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
   }
-  launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), its, f[i2], f[hh + i2]);
+  ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
+  if (kdev) set_step(h, S, kdev, 0);
+  launch_inject<T, false>(h, S, its, f[i2], f[hh + i2]);
+  if (kdev) launch(h, ela::k_count, dim3(1), dim3(64), kdev, 1);
   w.swap();
+  return ELA_OK;
+}
+
+// Device fine-step counter of the gradient's graph replays.
+int* step_counter(ela_handle* h) { return static_cast<int*>(h->counter.p) + 1; }
+
+void set_counter(ela_handle* h, int* c, int64_t v) {
+  launch(h, ela::k_fill<int>, dim3(1), dim3(64), c, (int64_t)1, (int)v);
+}
+
+// Steps k0 .. k1 - 1 of the forward recursion without derivative store,
+// native checkpoints of the state saved into `ck` every K steps from k0 when
+// ck is set. With graphs, pairs of steps are replayed (K even).
+template <typename T>
+int grad_fwd_range(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k0, int64_t k1,
+                   char* ck, int64_t K, size_t ck_bytes) {
+  const bool graph = h->use_graph && (!ck || K % 2 == 0);
+  hipGraphExec_t ex = nullptr;
+  int* kst = nullptr;
+  if (graph && k1 - k0 >= 2) {
+    CHK(ensure_chunks(h));  // no synchronous upload inside the capture
+    kst = step_counter(h);
+    auto pair = [&]() -> int {
+      Wave<T> wl = w;
+      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, kst)));
+      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, kst)));
+      return ELA_OK;
+    };
+    CHK(graph_for(h, h->graphs[G_FWD], pair, &ex));
+    set_counter(h, kst, k0);
+  }
+  for (int64_t k = k0; k < k1;) {
+    if (ck && (k - k0) % K == 0) CHK(save_checkpoint<T>(h, w, ck + ((k - k0) / K) * ck_bytes, ELA_CK_NATIVE, nullptr));
+    if (ex && k + 2 <= k1) {
+      const hipError_t e = hipGraphLaunch(ex, h->stream);
+      if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
+      k += 2;
+    } else {
+      CHK((grad_fwd_step<T, false>(h, w, sp, src_tr, k, nullptr)));
+      ++k;
+    }
+  }
+  if (ex) HIPCHK(hipEventRecord(h->graphs[G_FWD].done, h->stream));
   return ELA_OK;
 }
 
@@ -1313,21 +1467,51 @@ int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in
   CHK(fill<T>(h, h->wave.p, 18 * h->stride));
   CHK(fill<T>(h, tm, nrec_tot * nts));
   const double* f = h->filt.data();
+  // one fine step k (host index) or the next one of the device counter kdev
+  auto step = [&](Wave<T>& wv, int64_t k, int* kdev) -> int {
+    const int its = (int)(k / sub), i2 = (int)(k % sub);
+    ela::StepArgs<T> a = base_args<T>(h, wv.p0, wv.p1);
+    CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
+    ela::PointArgs<T> S = make_points<T>(h, sp, wv.p0, src_tr, false, 0);
+    ela::PointArgs<T> R = make_points<T>(h, rp, wv.p0, TmViews<T>(tm).v, true, nrec_tot);
+    if (kdev) { set_step(h, S, kdev, 0); set_step(h, R, kdev, 1); }
+    launch_inject<T, false>(h, S, its, f[i2], f[hh + i2]);
+    launch_record<T>(h, R, its, f[i2 + 1], f[hh + i2 + 1]);
+    if (kdev) launch(h, ela::k_count, dim3(1), dim3(64), kdev, 1);
+    wv.swap();
+    return ELA_OK;
+  };
+  // graph replays of pairs of steps (the checkpoints fall on even steps)
+  hipGraphExec_t ex = nullptr;
+  if (h->use_graph && plan.K1 % 2 == 0 && N >= 2) {
+    int* kst = step_counter(h);
+    auto pair = [&]() -> int {
+      Wave<T> wl = w;
+      CHK(step(wl, 0, kst));
+      CHK(step(wl, 0, kst));
+      return ELA_OK;
+    };
+    CHK(ensure_chunks(h));
+    CHK(graph_for(h, h->graphs[G_FWD_REC], pair, &ex));
+    set_counter(h, kst, 0);
+  }
   HIPCHK(hipEventRecord(h->ev0, h->stream));
-  for (int64_t k = 0; k < N; ++k) {
+  for (int64_t k = 0; k < N;) {
     if (k % plan.K1 == 0) {
       const int64_t s1 = k / plan.K1;
       CHK(save_checkpoint<T>(h, w, (char*)h->ck1.p + s1 * plan.ck1_bytes, plan.fmt,
                              (float*)h->ckscale.p + s1 * 18));
     }
-    const int its = (int)(k / sub), i2 = (int)(k % sub);
-    ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
-    CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
-    launch_inject<T, false>(h, make_points<T>(h, sp, w.p0, src_tr, false, 0), its, f[i2], f[hh + i2]);
-    launch_record<T>(h, make_points<T>(h, rp, w.p0, TmViews<T>(tm).v, true, nrec_tot), its, f[i2 + 1],
-                     f[hh + i2 + 1]);
-    w.swap();
+    if (ex && k + 2 <= N) {
+      const hipError_t e = hipGraphLaunch(ex, h->stream);
+      if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
+      k += 2;
+    } else {
+      CHK(step(w, k, nullptr));
+      ++k;
+    }
   }
+  if (ex) HIPCHK(hipEventRecord(h->graphs[G_FWD_REC].done, h->stream));
   HIPCHK(hipEventRecord(h->ev1, h->stream));
   h->last_steps = N;
   int64_t off = 0;
@@ -1415,22 +1599,32 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   const float* scales = (const float*)h->ckscale.p;
   int64_t steps = 0;
 
-  auto adj_step = [&](int64_t k, const T* d) -> int {
+  // adjoint step k (host index) or the current one of the device counter
+  // kdev, which it counts down
+  auto adj_step = [&](Wave<T>& av, int64_t k, const T* d, int* kdev) -> int {
     const int its = (int)(k / sub), i2 = (int)(k % sub);
-    ela::StepArgs<T> a = base_args<T>(h, ad.p0, ad.p1);
-    set_pm<T>(h, a, ad);
+    ela::StepArgs<T> a = base_args<T>(h, av.p0, av.p1);
+    set_pm<T>(h, a, av);
     for (int c = 0; c < 9; ++c) a.drv[c] = d + c * np;
     for (int m = 0; m < 8; ++m) a.grad[m] = g32 + m * np;
     CHK((launch_step_m<T, ela::MODE_BORN_ADJ_PM, ela::F_GRAD, ela::F_GRAD>(h, a)));
-    ela::PointArgs<T> R = make_points<T>(h, rp, ad.p0, TmViews<T>(tm_adj).v, true, nrec_tot);
-    set_pm<T>(h, R, ad);
+    ela::PointArgs<T> R = make_points<T>(h, rp, av.p0, TmViews<T>(tm_adj).v, true, nrec_tot);
+    set_pm<T>(h, R, av);
     for (int c = 0; c < 9; ++c) R.drv[c] = a.drv[c];
     for (int m = 0; m < 8; ++m) R.grad[m] = a.grad[m];
+    ela::PointArgs<T> S = make_points<T>(h, sp, av.p0, TmViews<T>(tm_src).v, true, nsrc_tot);
+    if (kdev) { set_step(h, R, kdev, 1); set_step(h, S, kdev, 0); }
     launch_inject<T, false, true, true>(h, R, its, f[i2 + 1], f[hh + i2 + 1]);
-    launch_record<T>(h, make_points<T>(h, sp, ad.p0, TmViews<T>(tm_src).v, true, nsrc_tot), its, f[i2], f[hh + i2]);
-    ad.swap();
+    launch_record<T>(h, S, its, f[i2], f[hh + i2]);
+    if (kdev) launch(h, ela::k_count, dim3(1), dim3(64), kdev, -1);
+    av.swap();
     return ELA_OK;
   };
+  // graph replays of whole inner segments (K steps; K even so that the
+  // wavefield pointers are back in place after a segment)
+  const bool seg_graph = h->use_graph && plan.K % 2 == 0;
+  int* kst = step_counter(h);
+  if (seg_graph) CHK(ensure_chunks(h));
 
   HIPCHK(hipEventRecord(h->ev0, h->stream));
   for (int64_t s1 = plan.n1 - 1; s1 >= 0; --s1) {
@@ -1439,12 +1633,7 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
     const void* c1 = (const char*)h->ck1.p + s1 * plan.ck1_bytes;
     if (nseg > 1) {
       CHK(load_checkpoint<T>(h, bg, c1, plan.fmt, scales + s1 * 18));
-      for (int64_t k = a0; k < a1; ++k) {
-        if ((k - a0) % plan.K == 0 && (k - a0) / plan.K < nseg)
-          CHK(save_checkpoint<T>(h, bg, (char*)h->ck2.p + ((k - a0) / plan.K) * plan.ck2_bytes, ELA_CK_NATIVE,
-                                 nullptr));
-        CHK((grad_fwd_step<T, false>(h, bg, sp, src_tr, k, nullptr)));
-      }
+      CHK(grad_fwd_range<T>(h, bg, sp, src_tr, a0, a1, (char*)h->ck2.p, plan.K, plan.ck2_bytes));
       steps += a1 - a0;
     }
     for (int64_t s2 = nseg - 1; s2 >= 0; --s2) {
@@ -1452,12 +1641,34 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
       if (nseg > 1)
         CHK(load_checkpoint<T>(h, bg, (const char*)h->ck2.p + s2 * plan.ck2_bytes, ELA_CK_NATIVE, nullptr));
       else CHK(load_checkpoint<T>(h, bg, c1, plan.fmt, scales + s1 * 18));
-      for (int64_t k = j0; k < j1; ++k) CHK((grad_fwd_step<T, true>(h, bg, sp, src_tr, k, drv + (k - j0) * 9 * np)));
-      for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(k, drv + (k - j0) * 9 * np));
+      if (seg_graph && j1 - j0 == plan.K) {
+        // recompute with derivative store, then the transposed steps, each one replayed graph
+        auto store_seg = [&]() -> int {
+          Wave<T> wl = bg;
+          for (int64_t kk = 0; kk < plan.K; ++kk)
+            CHK((grad_fwd_step<T, true>(h, wl, sp, src_tr, 0, drv + kk * 9 * np, kst)));
+          return ELA_OK;
+        };
+        auto adj_seg = [&]() -> int {
+          Wave<T> wl = ad;
+          for (int64_t kk = plan.K - 1; kk >= 0; --kk) CHK(adj_step(wl, 0, drv + kk * 9 * np, kst));
+          return ELA_OK;
+        };
+        hipGraphExec_t ex = nullptr;
+        CHK(graph_for(h, h->graphs[G_STORE], store_seg, &ex));
+        set_counter(h, kst, j0);
+        CHK(replay(h, h->graphs[G_STORE], ex, 1));
+        CHK(graph_for(h, h->graphs[G_ADJ], adj_seg, &ex));
+        set_counter(h, kst, j1 - 1);
+        CHK(replay(h, h->graphs[G_ADJ], ex, 1));
+      } else {
+        for (int64_t k = j0; k < j1; ++k)
+          CHK((grad_fwd_step<T, true>(h, bg, sp, src_tr, k, drv + (k - j0) * 9 * np)));
+        for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(ad, k, drv + (k - j0) * 9 * np, nullptr));
+      }
       steps += 2 * (j1 - j0);
       const int64_t n8 = 8 * np;
-      hipLaunchKernelGGL((ela::k_fold<T>), dim3((unsigned)std::min<int64_t>((n8 + 255) / 256, 65536)), dim3(256), 0,
-                         h->stream, g32, g64, n8);
+      launch(h, ela::k_fold<T>, dim3((unsigned)std::min<int64_t>((n8 + 255) / 256, 65536)), dim3(256), g32, g64, n8);
     }
   }
   HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -1531,8 +1742,7 @@ int set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem) {
   a.tdt = (T)(2.0 * dtw);
   a.tdt64 = 2.0 * dtw;
   const int64_t n = h->cfg.nz * h->cfg.nx * h->cfg.ny;
-  hipLaunchKernelGGL((ela::k_scale_medium<T>), dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20)), dim3(256),
-                     0, h->stream, a);
+  launch(h, ela::k_scale_medium<T>, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1 << 20)), dim3(256), a);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   h->tdt = 2.0 * dtw;
@@ -1630,6 +1840,21 @@ int ela_create(const ela_config* cfg, ela_handle** out) {
       }
       if (e != hipSuccess) return bail(fail(ELA_ERR_HIP, "taper copy failed"));
     }
+    // the filter in the handle's precision and the step counters of the graph replays
+    rc = h->dfilt.alloc(2 * hh * h->esize);
+    if (rc) return bail(rc);
+    {
+      hipError_t e;
+      if (h->esize == 4) {
+        std::vector<float> ff(h->filt.begin(), h->filt.end());
+        e = hipMemcpy(h->dfilt.p, ff.data(), ff.size() * 4, hipMemcpyHostToDevice);
+      } else {
+        e = hipMemcpy(h->dfilt.p, h->filt.data(), h->filt.size() * 8, hipMemcpyHostToDevice);
+      }
+      if (e != hipSuccess) return bail(fail(ELA_ERR_HIP, "filter copy failed"));
+    }
+    rc = h->counter.alloc(256);
+    if (rc) return bail(rc);
     h->by_x = cfg->precision == ELA_FP64 ? 8 : 16;
     // the device address of the allocation-free entry point's scratch on this device
     rc = scratch_address(cfg->device, &h->scratch_dev);
@@ -1644,8 +1869,10 @@ int ela_destroy(ela_handle* h) {
     if (!h) return ELA_OK;
     DeviceGuard dg(h->cfg.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    (void)release_exec(h);
-    if (h->nl_exec_done) (void)hipEventDestroy(h->nl_exec_done);
+    for (GraphCache& gc : h->graphs) {
+      (void)drop_graph(gc);
+      if (gc.done) (void)hipEventDestroy(gc.done);
+    }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
@@ -1694,6 +1921,17 @@ int ela_set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem) {
     CHK(h->esize == 4 ? set_elastic<float>(h, rlm3, dtw, mem) : set_elastic<double>(h, rlm3, dtw, mem));
     h->have_medium = h->have_elastic = true;
     h->grad_ready = false;
+    return ELA_OK;
+  });
+}
+
+int ela_get_medium(ela_handle* h, void* med8, int mem) {
+  return guarded([&]() -> int {
+    if (!h || !med8) return fail(ELA_ERR_ARG, "null argument");
+    if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
+    DeviceGuard dg(h->cfg.device);
+    CHK(copy_out_3d(h, med8, h->med.p, 8, mem));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return ELA_OK;
   });
 }
@@ -1922,9 +2160,9 @@ int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, 
     // enough 256-thread blocks for about 4 vectors per thread
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n4 + 1023) / 1024, 1 << 20));
     float4* const* t = static_cast<float4* const*>(tab.p);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
+    launch(h, kern, dim3(blocks), dim3(256), t, n4);
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, h->stream, t, n4);
+    for (int r = 0; r < reps; ++r) launch(h, kern, dim3(blocks), dim3(256), t, n4);
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     HIPCHK(hipEventSynchronize(h->ev1));
     HIPCHK(hipGetLastError());

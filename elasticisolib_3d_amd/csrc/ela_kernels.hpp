@@ -381,16 +381,17 @@ struct VelStage {
 // same order, so an FP64 stepper reproduces the 8-array stepper exactly.
 //
 // Buoyancy 2 dtw / rho_h of a staggered density rho_h. FP64: the host
-// scaling's division. FP32: formed in FP64 from the FP32 densities (hardware
-// FP64 reciprocal, far inside FP32 rounding) and rounded once, as the host
-// rounds its FP64 scaling -- formed in FP32 arithmetic, the buoyancies and
-// moduli carried one or two extra roundings whose bias more than doubled the
-// FP32 data's distance from FP64 over a 5000-step run. k_scale_medium uses the
-// same functions, so every mode of one handle sees the same medium.
+// scaling's division. FP32: the same correctly rounded FP64 division of the
+// FP64 half-cell sum of the FP32 densities, rounded once to FP32 -- the host's
+// FP64 scaling of the same densities rounded to FP32, bit for bit. Formed in
+// FP32 arithmetic, the buoyancies and moduli carried one or two extra
+// roundings whose bias more than doubled the FP32 data's distance from FP64
+// over a 5000-step run. k_scale_medium uses the same functions, so every mode
+// of one handle sees the same medium.
 template <typename T>
 __device__ __forceinline__ T buoy_h(T tdt, double tdt64, double rho_h) {
   if constexpr (sizeof(T) == 4) {
-    return T(tdt64 * __builtin_amdgcn_rcp(rho_h));
+    return T(tdt64 / rho_h);
   } else {
     return tdt / T(rho_h);
   }
@@ -864,7 +865,29 @@ struct PointArgs {
   int min_pad, nz, nx, ny, nzi, zoff;
   const int* ytab;         // stacked windows: per plane {window y0, window ny, window nx, 0}, or null
   const int* its_dev;      // coarse sample index read from device memory (graph replays), or null
+  // fine step k read from device memory (graph replays of gradient steps), or
+  // null: the coarse sample is k / sub and the weights are filt[i + fo] and
+  // filt[sub + 1 + i + fo] with i = k % sub (the host's fa / fb of that step)
+  const int* step_dev;
+  const T* filt;
+  int sub, fo;
 };
+
+// Coarse sample and time-filter weights of a point launch.
+template <typename T>
+__device__ __forceinline__ void point_time(const PointArgs<T>& a, int& its, T& fa, T& fb) {
+  if (a.step_dev) {
+    const int k = *a.step_dev;
+    its = k / a.sub;
+    const int i = k - its * a.sub + a.fo;
+    fa = a.filt[i];
+    fb = a.filt[a.sub + 1 + i];
+  } else {
+    its = a.its_dev ? *a.its_dev : a.its;
+    fa = a.fa;
+    fb = a.fb;
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ int comp_of(const PointArgs<T>& a, int64_t g) {
@@ -899,13 +922,15 @@ __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   const int64_t k = g - a.off[c];
   const int32_t p = a.pos[c][k];
   if (p < 0) return;
-  const int its = a.its_dev ? *a.its_dev : a.its;
+  int its;
+  T fa, fb;
+  point_time(a, its, fa, fb);
   const T taper = point_taper(a, p);
   // the injected value of component cc at this point (the three normal
   // stresses share their point list, so index k is the same point for them)
   auto injected = [&](int cc) {
     const T* tr = a.tr[cc] + k * a.tr_stride_pt;
-    return taper * (tr[(int64_t)its * a.tr_stride_t] * a.fa + tr[(int64_t)(its + 1) * a.tr_stride_t] * a.fb);
+    return taper * (tr[(int64_t)its * a.tr_stride_t] * fa + tr[(int64_t)(its + 1) * a.tr_stride_t] * fb);
   };
   const T v = injected(c);
   a.field[c][p] += v;
@@ -957,10 +982,12 @@ __global__ void __launch_bounds__(256) k_record(PointArgs<T> a) {
   const int32_t p = a.pos[c][k];
   if (p < 0) return;
   const T v = a.field[c][p];
-  const int its = a.its_dev ? *a.its_dev : a.its;
+  int its;
+  T fa, fb;
+  point_time(a, its, fa, fb);
   T* tr = a.tr[c] + k * a.tr_stride_pt;
-  tr[(int64_t)its * a.tr_stride_t] += v * a.fa;
-  tr[(int64_t)(its + 1) * a.tr_stride_t] += v * a.fb;
+  tr[(int64_t)its * a.tr_stride_t] += v * fa;
+  tr[(int64_t)(its + 1) * a.tr_stride_t] += v * fb;
 }
 
 // Out-of-place transpose of a [rows][cols] block (row stride ld_in) into

@@ -167,12 +167,16 @@ class FwiProblem:
     default). `autotune` times the stencil launch
     shapes once when the first medium is set, `medium` chooses between
     handing (rho, lambda, mu) to the library ("elastic") and the 8 host-scaled
-    arrays ("scaled"); the two agree exactly in FP64."""
+    arrays ("scaled"); the two agree exactly in FP64. `graph` replays the
+    gradient's pairs of forward steps and its checkpoint segments (forward
+    with derivative store, transposed steps with imaging) as HIP graphs
+    captured once and kept by the propagator; the results are bit-identical
+    to launching every step from the host."""
 
     def __init__(self, axes: PaddedAxes, nts: int, dts: float, sub: int, alpha_cos: float, shots: Sequence[FwiShot],
                  precision: str = "f32", interval: int = 20, outer_interval: int = 0, bf16: Optional[bool] = None,
                  device: int = 0, checkpoint: str = "s16", autotune: bool = False, medium: str = "elastic",
-                 components: str = ALL_COMPONENTS):
+                 components: str = ALL_COMPONENTS, graph: bool = True):
         self.axes = axes
         self.components = components
         self.rows = component_rows(components)
@@ -186,6 +190,7 @@ class FwiProblem:
         if bf16 is not None:
             checkpoint = "bf16" if bf16 else "native"
         self.prop.set_gradient_checkpoints(interval, outer_interval, checkpoint)
+        self.prop.set_graph(graph)
         self.dxyz = axes.dx * axes.dy * axes.dz
         self.last_timing = {}
         # autotune: time the stencil launch shapes once, when the first medium is set

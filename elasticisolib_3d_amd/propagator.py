@@ -174,6 +174,13 @@ class Propagator:
         check(self.lib.ela_set_elastic(self.h, self._ptr(rlm3, self.dtype), float(dtw), mem), "ela_set_elastic")
         self.medium_kind = "elastic"
 
+    def get_medium(self) -> np.ndarray:
+        """The handle's 8 scaled medium arrays [8, ny, nx, nz] (those set, or
+        those set_elastic derived on the GPU)."""
+        out = np.empty((8,) + self.shape, dtype=self.dtype)
+        check(self.lib.ela_get_medium(self.h, out.ctypes.data, _lib.ELA_MEM_HOST), "ela_get_medium")
+        return out
+
     def nl_forward(self, src_pos, src_traces, rec_pos, rec_out=None):
         mem = self._mem_of(*src_traces, *(rec_out or []))
         sp, rp = self._points(src_pos, mem), self._points(rec_pos, mem)
@@ -304,7 +311,10 @@ class Propagator:
               "ela_set_windows")
 
     def set_graph(self, on: bool = True) -> None:
-        """Replay the coarse step of nl_forward / nl_adjoint as a HIP graph."""
+        """Replay the time loops as HIP graphs: the coarse step of nl_forward /
+        nl_adjoint, the gradient's pairs of forward steps and its checkpoint
+        segments. Each graph is captured once and replayed while the launches
+        it holds (kernels, shapes, arguments) stay the same."""
         check(self.lib.ela_set_graph(self.h, int(bool(on))), "ela_set_graph")
 
     def set_tiling(self, tile_x: int, y_chunk: int) -> None:

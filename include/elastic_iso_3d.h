@@ -133,6 +133,14 @@ int ela_set_medium(ela_handle* h, const void* med8, int mem);
  * (nonlinearPropElasticGpuFunctions_3D.h:10). */
 int ela_set_elastic(ela_handle* h, const void* rlm3, double dtw, int mem);
 
+/* Copy the handle's 8 scaled medium arrays out ([8][ny][nx][nz], element
+ * type of the handle precision): the arrays of ela_set_medium, or those
+ * ela_set_elastic derived. In FP32 the derived buoyancies and normal moduli
+ * are the host scaling (ela_scale_medium) of the same values rounded to FP32,
+ * bit for bit; the shear moduli are FP32 half-cell sums. No counterpart in the
+ * reference (diagnostics and tests). */
+int ela_get_medium(ela_handle* h, void* med8, int mem);
+
 /* Host-side staggering and scaling of (rho, lambda, mu) into the 8 arrays of
  * ela_set_medium (fdParamElastic_3D.cpp:130-183). Double precision in and out;
  * dtw = dts / sub. */
@@ -341,10 +349,18 @@ int ela_last_timing(const ela_handle* h, double* ms, int64_t* fine_steps, int64_
  * option `ginsu` is read but unused, nonlinearPropElasticShotsGpu_3D.cpp:12). */
 int ela_set_windows(ela_handle* h, int n, const int64_t* y0, const int64_t* ny, const int64_t* nx);
 
-/* Replay the coarse step of ela_nl_forward / ela_nl_adjoint as a HIP graph:
- * the sub fine steps (stepper, injection and recording launches) are captured
- * once per call and the graph is launched nts - 1 times, the coarse-sample
- * index advancing in device memory. Same results; fewer host launches. */
+/* Replay the time loops as HIP graphs: the coarse step of ela_nl_forward /
+ * ela_nl_adjoint (sub fine steps, sub even; launched nts - 1 times, the
+ * coarse-sample index advancing in device memory), pairs of forward steps of
+ * ela_gradient_forward and of the gradient's inner recomputation, and each
+ * full checkpoint segment of ela_gradient_backward (the forward steps with
+ * derivative store, then the transposed steps with imaging; interval even),
+ * their fine-step index and time-filter weights read from device memory.
+ * A graph is captured and instantiated once and kept by the handle; a later
+ * call replays it when its launches (kernels, launch shapes and argument
+ * bytes) are the same, and captures a new one otherwise. Results are
+ * bit-identical to host launches. No counterpart in the reference, which
+ * launches every kernel from the host. */
 int ela_set_graph(ela_handle* h, int on);
 
 /* Tile shape of the stencil kernels: x extent of a 64-wide z tile (4, 8 or

@@ -17,3 +17,18 @@ if os.environ.get("PYTEST_XDIST_WORKER") and "OMP_NUM_THREADS" not in os.environ
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: long CPU test")
+
+
+@pytest.fixture(autouse=True)
+def _release_device_memory(request):
+    """After each GPU test, drop unreferenced objects and torch's cached
+    blocks, so that one test's propagators (tens of GB of checkpoints on the
+    full-size workloads) never starve the next."""
+    yield
+    if request.node.get_closest_marker("gpu") is None:
+        return
+    import gc
+    gc.collect()
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.empty_cache()

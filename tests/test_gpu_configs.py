@@ -138,16 +138,17 @@ def test_c4_gradient_shortened_record():
     per = [float(np.linalg.norm(g32[k] - g64[k]) / np.linalg.norm(g64[k])) for k in range(3)]
     print(f"C4 FP32 (int16 checkpoints) vs FP64 gradient: rel L2 {e:.2e} (rho, lambda, mu {np.round(per, 7)})",
           flush=True)
-    assert e < 1e-3
-    # FP64 gradient against central differences of the FP64 misfit on 4 illuminated cells
+    # FP64 gradient against central differences of the FP64 misfit on 4 illuminated cells: the
+    # shortened record illuminates only part of the grid, so each cell is drawn from the 200 cells
+    # of largest |g m| of its own parameter (rho, lambda, mu, rho)
     p64 = res["f64_exact"][0]
-    score = np.abs(g64 * start).ravel()
-    top = np.argsort(score)[-score.size // 20:]
-    pick = np.random.default_rng(3).choice(top, size=4, replace=False)
+    rng = np.random.default_rng(3)
     h = 1e-3
-    for q in pick:
-        k, rest = divmod(int(q), start[0].size)
-        idx = np.unravel_index(rest, start[0].shape)
+    checks = []
+    for k in (0, 1, 2, 0):
+        score = np.abs(g64[k] * start[k]).ravel()
+        q = int(rng.choice(np.argsort(score)[-200:]))
+        idx = np.unravel_index(q, start[k].shape)
         mp = [m.copy() for m in start]
         mm = [m.copy() for m in start]
         mp[k][idx] *= 1 + h
@@ -156,6 +157,12 @@ def test_c4_gradient_shortened_record():
         an = float(g64[k][idx])
         print(f"C4 FD cell param {k} {tuple(int(i) for i in idx)}: FD {fd:.8e} adjoint {an:.8e} "
               f"rel {abs(fd - an) / abs(an):.2e}", flush=True)
+        checks.append((fd, an))
+    # release the propagators (tens of GB of checkpoints) before any assertion can keep them alive
+    del res, p64
+    torch.cuda.empty_cache()
+    assert e < 1e-3
+    for fd, an in checks:
         assert abs(fd - an) <= 1e-4 * abs(an)
 
 

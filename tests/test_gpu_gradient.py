@@ -133,3 +133,34 @@ def test_gradient_and_born_adjoint_are_reproducible(problem, precision):
     assert np.array_equal(ga, gb)
     assert all(np.array_equal(x, y) for x, y in zip(sa, sb))
     assert np.array_equal(ia, ib)
+
+
+@pytest.mark.parametrize("precision,K,K1,ck", [("f64", 4, 0, "native"), ("f32", 2, 8, "s16"),
+                                                ("f32", 6, 0, "bf16"), ("f64", 3, 3, "native")])
+def test_gradient_graph_replay_matches_eager(problem, precision, K, K1, ck):
+    """The gradient's graph replays (pairs of forward steps, whole checkpoint
+    segments of forward-with-store and transposed steps, the fine-step index
+    and filter weights read from device memory) give the host-launched
+    traces, medium gradient and source gradient bit for bit, on two calls in
+    a row (the second replays the graphs the handle kept). The cases cover
+    one and two checkpoint levels, a last segment shorter than the others
+    (K = 6 over 40 steps, launched from the host) and an odd interval (no
+    segment graphs)."""
+    P = problem[0]
+    prop = make_prop(precision)
+    prop.set_medium(P["med"])
+    prop.set_gradient_checkpoints(K, K1, ck)
+
+    def run():
+        rec = prop.gradient_forward(P["spos"], P["src"], P["rpos"])
+        g8, gs = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
+        return [np.array(r) for r in rec] + [np.array(g8)] + [np.array(x) for x in gs]
+
+    prop.set_graph(False)
+    ref = run()
+    prop.set_graph(True)
+    for call in range(2):
+        out = run()
+        bad = [i for i, (a, b) in enumerate(zip(out, ref)) if not np.array_equal(a, b)]
+        print(f"gradient graph {precision} K={K} K1={K1} {ck} call {call}: differing outputs {bad}", flush=True)
+        assert not bad

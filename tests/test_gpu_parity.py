@@ -292,6 +292,41 @@ def test_graph_replay_matches_eager(problems, precision):
         assert all(np.array_equal(a, b) for a, b in zip(eager, ref_f))
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_graph_recaptured_when_launches_change(problems, precision):
+    """The handle keeps the graph of the last call and replays it only when
+    the launches are the same: calls with other receiver lists, another
+    launch shape and another medium in between give their eager results bit
+    for bit, and a repeat of the first call replays the kept graph to its
+    first result."""
+    g, P = GRIDS["odd"], problems["odd"]
+    prop = make_prop(g, precision)
+    prop.set_medium(P["med"])
+    fewer = [p[:3] for p in P["rpos"]]
+    med2 = P["med"] * 1.01
+
+    def calls():
+        out = [prop.nl_forward(P["spos"], P["src"], P["rpos"]),
+               prop.nl_forward(P["spos"], P["src"], fewer)]
+        prop.set_tiling(8, 3)
+        out.append(prop.nl_forward(P["spos"], P["src"], P["rpos"]))
+        prop.set_medium(med2)
+        out.append(prop.nl_forward(P["spos"], P["src"], P["rpos"]))
+        prop.set_medium(P["med"])
+        prop.set_tiling(16, 8)
+        out.append(prop.nl_forward(P["spos"], P["src"], P["rpos"]))
+        return out
+
+    prop.set_tiling(16, 8)
+    prop.set_graph(False)
+    ref = calls()
+    prop.set_graph(True)
+    out = calls()
+    for i, (a, b) in enumerate(zip(out, ref)):
+        assert all(np.array_equal(x, y) for x, y in zip(a, b)), (precision, i)
+    assert all(np.array_equal(x, y) for x, y in zip(out[-1], ref[0]))
+
+
 def test_stream_probe_reports_bandwidth():
     """The streaming reference of the roofline runs its five mixes and
     rejects others."""
@@ -344,3 +379,33 @@ def test_elastic_parameters_match_scaled_medium(problems, precision, name):
             assert all(np.array_equal(x, y) for x, y in zip(a, b)), what
         else:
             assert err < 2e-6, (what, err)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_derived_medium_equals_host_scaling(problems, precision):
+    """The 8 arrays ela_set_elastic derives on the GPU against the host
+    scaling (ela_scale_medium, FP64) of the same (rho, lambda, mu) values: in
+    FP64 all 8 bit for bit; in FP32 (inputs rounded to FP32 first, the host
+    scaling of those values rounded once to FP32) the buoyancies and normal
+    moduli bit for bit -- the GPU forms them with a correctly rounded FP64
+    division and products -- and the shear moduli, FP32 half-cell sums,
+    within 2 units in the last place."""
+    from elasticisolib_3d_amd.propagator import Propagator, scale_medium
+    g = GRIDS["odd"]
+    P = problems["odd"]
+    dt = np.float64 if precision == "f64" else np.float32
+    rlm = np.stack([P["rho"], P["lam"], P["mu"]]).astype(dt)
+    host = scale_medium(*[x.astype(np.float64) for x in rlm], g.dtw).astype(dt)
+    prop = Propagator(g.nz, g.nx, g.ny, g.nts, g.sub, g.dz, g.dx, g.dy, g.dts, g.minPad, g.alphaCos, precision)
+    prop.set_elastic(rlm, g.dtw)
+    dev = prop.get_medium()
+    exact = range(8) if precision == "f64" else range(5)
+    for k in exact:
+        nbad = int(np.count_nonzero(dev[k] != host[k]))
+        print(f"derived medium {precision} array {k}: {nbad} of {dev[k].size} values differ", flush=True)
+        assert nbad == 0, k
+    if precision == "f32":
+        for k in range(5, 8):
+            ulp = np.spacing(np.abs(host[k])).astype(np.float64)
+            d = np.abs(dev[k].astype(np.float64) - host[k].astype(np.float64))
+            assert np.all(d <= 2 * ulp), (k, float((d / np.maximum(ulp, 1e-300)).max()))
