@@ -62,9 +62,13 @@ def _unpad(g: np.ndarray, axis: int, lo: int, n: int) -> np.ndarray:
 
 class WindowStack:
     """B windows of a model of (nz, nx, ny) cells, spacing d, each padded
-    with `pad` sponge cells (plus FAT) and stacked along y."""
+    with `pad` sponge cells (plus FAT) and stacked along y. `grid` = (nx, ny)
+    places the stack in a larger grid (a propagator shared by several
+    stacks): planes past the last window and columns past a window's width
+    are never stepped."""
 
-    def __init__(self, model_shape: Tuple[int, int, int], d: float, pad: int, windows: Sequence[Window]):
+    def __init__(self, model_shape: Tuple[int, int, int], d: float, pad: int, windows: Sequence[Window],
+                 grid: Optional[Tuple[int, int]] = None):
         nz, nxm, nym = model_shape
         self.nz_model, self.nx_model, self.ny_model = nz, nxm, nym
         self.d, self.pad = float(d), int(pad)
@@ -90,6 +94,10 @@ class WindowStack:
                                         (w.y0 - FAT - pad) * d, d, d, d, FAT, zm, zp, xm, xp, pad))
         self.nx = max(self.nx_w)
         self.ny = y
+        if grid is not None:
+            if grid[0] < self.nx or grid[1] < self.ny:
+                raise ValueError(f"grid {grid} smaller than the stack ({self.nx}, {self.ny})")
+            self.nx, self.ny = int(grid[0]), int(grid[1])
         self.min_pad = min(min(zm, zp), min(min(x) for x in self.xpads))
 
     # -- model ------------------------------------------------------------
@@ -225,10 +233,14 @@ class BatchedFwi:
     (default), bfloat16 or exact checkpoints, FP64 gradient accumulation, as
     FwiProblem). `model_shape` is the [ny, nx, nz] shape of the (unpadded)
     model arrays; `checkpoint` is "s16", "bf16" or "native", and the older
-    `bf16` flag, when given, means "bf16" / "native"; `autotune` times the launch shapes of every stacked grid
-    unless `tiling` fixes them; `medium` ("elastic" or "scaled") chooses how
+    `bf16` flag, when given, means "bf16" / "native"; `autotune` times the launch shapes of the shared grid
+    once unless `tiling` fixes them; `medium` ("elastic" or "scaled") chooses how
     the stacked model reaches the library, as in FwiProblem; `graph` replays
-    the gradient's time steps as HIP graphs (as FwiProblem)."""
+    the gradient's time steps as HIP graphs (as FwiProblem). Every batch runs
+    on one persistent propagator whose grid holds the widest and tallest
+    stack of windows (`shared_grid`); a batch sets its windows and stacked
+    medium on it, so wavefields, checkpoints and graphs are allocated and
+    captured once for all batches and calls (`release` frees them)."""
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
@@ -250,9 +262,43 @@ class BatchedFwi:
         self.medium = medium
         self.graph = bool(graph)   # replay the gradient's step pairs and checkpoint segments as HIP graphs
         self.timing = {}
+        # one propagator for every batch and every call: its grid holds the widest and the tallest
+        # stack, so its wavefields, checkpoints and graphs are allocated, tuned and captured once
+        self._prop = None
+        self._grid = None
+        self._tune_pending = False
 
     def _groups(self):
         return [self.shots[i:i + self.batch] for i in range(0, len(self.shots), self.batch)]
+
+    def _model_zxy(self):
+        return (self.model_shape[2], self.model_shape[1], self.model_shape[0])
+
+    def shared_grid(self) -> Tuple[int, int]:
+        """(nx, ny) of the propagator grid the batches share."""
+        if self._grid is None:
+            stacks = [WindowStack(self._model_zxy(), self.d, self.pad, [s.window for s in g]) for g in self._groups()]
+            self._grid = (max(w.nx for w in stacks), max(w.ny for w in stacks))
+        return self._grid
+
+    def _propagator(self, ws):
+        """The persistent propagator, created and configured on first use."""
+        if self._prop is None:
+            prop = P.Propagator(ws.nz, ws.nx, ws.ny, self.nts, self.sub, self.d, self.d, self.d, self.dts,
+                                ws.min_pad, self.alpha, self.precision, self.device)
+            if self.tiling:
+                prop.set_tiling(*self.tiling[:2])
+                if len(self.tiling) > 2:
+                    prop.set_tile_order(*self.tiling[2:])
+            prop.set_gradient_checkpoints(self.interval, 0, self.checkpoint)
+            prop.set_graph(self.graph)
+            self._prop = prop
+            self._tune_pending = self.autotune and not self.tiling
+        return self._prop
+
+    def release(self) -> None:
+        """Free the persistent propagator's device memory (it is created again when needed)."""
+        self._prop = None
 
     def _model_device(self, model):
         """The full (rho, lambda, mu) model as one FP64 tensor on the GPU."""
@@ -265,28 +311,21 @@ class BatchedFwi:
         operations of the host routines); `meds` are the per-window 8 scaled
         arrays (FP64, device)."""
         import torch
-        ws = WindowStack((self.model_shape[2], self.model_shape[1], self.model_shape[0]), self.d, self.pad,
-                         [s.window for s in group])
+        ws = WindowStack(self._model_zxy(), self.d, self.pad, [s.window for s in group], grid=self.shared_grid())
         wins = [ws.window_model_t(k, model_d) for k in range(len(group))]
         meds = [MT.scale_medium(w[0], w[1], w[2], self.dtw) for w in wins]
-        prop = P.Propagator(ws.nz, ws.nx, ws.ny, self.nts, self.sub, self.d, self.d, self.d, self.dts, ws.min_pad,
-                            self.alpha, self.precision, self.device)
+        prop = self._propagator(ws)
         ws.set_windows(prop)
-        if self.tiling:
-            prop.set_tiling(*self.tiling[:2])
-            if len(self.tiling) > 2:
-                prop.set_tile_order(*self.tiling[2:])
         td = torch.float32 if self.precision == "f32" else torch.float64
         if self.medium == "elastic":
             prop.set_elastic(ws.stack_t(wins, fill=1.0).to(td).contiguous(), self.dtw)
         else:
             prop.set_medium(ws.stack_t(meds).to(td).contiguous())
         del wins
-        if self.autotune and not self.tiling:
+        if self._tune_pending:
             from .autotune import autotune
             autotune(prop, reps=5)
-        prop.set_gradient_checkpoints(self.interval, 0, self.checkpoint)
-        prop.set_graph(self.graph)
+            self._tune_pending = False
         geos = []
         for k, s in enumerate(group):
             ax = ws.axes[k]

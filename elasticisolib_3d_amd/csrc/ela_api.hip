@@ -600,15 +600,26 @@ void set_time(ela::PointArgs<T>& a, int its, double fa, double fb) {
   a.fb = a.step_dev ? T(0) : T(fb);
 }
 
-// The point launch reads its fine step from kdev and its weights from the
-// handle's filter table (offset fo: 0 for filt[i2], 1 for filt[i2 + 1]).
+// The point launch reads its fine step (*kdev + off) from device memory and
+// its weights from the handle's filter table (offset fo: 0 for filt[i2], 1
+// for filt[i2 + 1]).
 template <typename T>
-void set_step(ela_handle* h, ela::PointArgs<T>& a, const int* kdev, int fo) {
+void set_step(ela_handle* h, ela::PointArgs<T>& a, const int* kdev, int off, int fo) {
   a.step_dev = kdev;
+  a.step_off = off;
   a.filt = static_cast<const T*>(h->dfilt.p);
   a.sub = (int)h->cfg.sub;
   a.fo = fo;
 }
+
+// A replayed step's position relative to the device step counter, and
+// whether the step advances the counter afterwards (the last step of a
+// replayed pair).
+struct DevStep {
+  int* k = nullptr;   // device counter, or null (host index)
+  int off = 0;
+  int advance = 0;
+};
 
 template <typename T, bool ACC, bool GRAD = false, bool PM = false>
 void launch_inject(ela_handle* h, ela::PointArgs<T> a, int its, double fa, double fb) {
@@ -1372,11 +1383,12 @@ int load_checkpoint(ela_handle* h, const Wave<T>& w, const void* in, int fmt, co
 }
 
 // One forward fine step k (source injection, optional store of D(c_k)). With
-// kdev the step index is read from device memory and counted up there (graph
-// replays); k is then unused.
+// ds.k the step index is read from device memory (the counter plus ds.off)
+// and the counter advanced by ds.advance after the step (graph replays); k is
+// then unused.
 template <typename T, bool STORE>
 int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k, T* dst,
-                  int* kdev = nullptr) {
+                  DevStep ds = DevStep()) {
   const int64_t sub = h->cfg.sub, hh = sub + 1, np = h->stride;
   const double* f = h->filt.data();
   const int its = (int)(k / sub), i2 = (int)(k % sub);  // forward it2 = i2 + 1
@@ -1388,9 +1400,9 @@ int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_t
     CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
   }
   ela::PointArgs<T> S = make_points<T>(h, sp, w.p0, src_tr, false, 0);
-  if (kdev) set_step(h, S, kdev, 0);
+  if (ds.k) set_step(h, S, ds.k, ds.off, 0);
   launch_inject<T, false>(h, S, its, f[i2], f[hh + i2]);
-  if (kdev) launch(h, ela::k_count, dim3(1), dim3(64), kdev, 1);
+  if (ds.advance) launch(h, ela::k_count, dim3(1), dim3(64), ds.k, ds.advance);
   w.swap();
   return ELA_OK;
 }
@@ -1416,8 +1428,8 @@ int grad_fwd_range(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_
     kst = step_counter(h);
     auto pair = [&]() -> int {
       Wave<T> wl = w;
-      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, kst)));
-      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, kst)));
+      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, DevStep{kst, 0, 0})));
+      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, DevStep{kst, 1, 2})));
       return ELA_OK;
     };
     CHK(graph_for(h, h->graphs[G_FWD], pair, &ex));
@@ -1467,17 +1479,17 @@ int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in
   CHK(fill<T>(h, h->wave.p, 18 * h->stride));
   CHK(fill<T>(h, tm, nrec_tot * nts));
   const double* f = h->filt.data();
-  // one fine step k (host index) or the next one of the device counter kdev
-  auto step = [&](Wave<T>& wv, int64_t k, int* kdev) -> int {
+  // one fine step k (host index), or the step the device counter names (graph replays)
+  auto step = [&](Wave<T>& wv, int64_t k, DevStep ds) -> int {
     const int its = (int)(k / sub), i2 = (int)(k % sub);
     ela::StepArgs<T> a = base_args<T>(h, wv.p0, wv.p1);
     CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
     ela::PointArgs<T> S = make_points<T>(h, sp, wv.p0, src_tr, false, 0);
     ela::PointArgs<T> R = make_points<T>(h, rp, wv.p0, TmViews<T>(tm).v, true, nrec_tot);
-    if (kdev) { set_step(h, S, kdev, 0); set_step(h, R, kdev, 1); }
+    if (ds.k) { set_step(h, S, ds.k, ds.off, 0); set_step(h, R, ds.k, ds.off, 1); }
     launch_inject<T, false>(h, S, its, f[i2], f[hh + i2]);
     launch_record<T>(h, R, its, f[i2 + 1], f[hh + i2 + 1]);
-    if (kdev) launch(h, ela::k_count, dim3(1), dim3(64), kdev, 1);
+    if (ds.advance) launch(h, ela::k_count, dim3(1), dim3(64), ds.k, ds.advance);
     wv.swap();
     return ELA_OK;
   };
@@ -1487,8 +1499,8 @@ int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in
     int* kst = step_counter(h);
     auto pair = [&]() -> int {
       Wave<T> wl = w;
-      CHK(step(wl, 0, kst));
-      CHK(step(wl, 0, kst));
+      CHK(step(wl, 0, DevStep{kst, 0, 0}));
+      CHK(step(wl, 0, DevStep{kst, 1, 2}));
       return ELA_OK;
     };
     CHK(ensure_chunks(h));
@@ -1507,7 +1519,7 @@ int grad_forward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_in
       if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
       k += 2;
     } else {
-      CHK(step(w, k, nullptr));
+      CHK(step(w, k, DevStep()));
       ++k;
     }
   }
@@ -1599,9 +1611,8 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   const float* scales = (const float*)h->ckscale.p;
   int64_t steps = 0;
 
-  // adjoint step k (host index) or the current one of the device counter
-  // kdev, which it counts down
-  auto adj_step = [&](Wave<T>& av, int64_t k, const T* d, int* kdev) -> int {
+  // adjoint step k (host index), or the step the device counter names (graph replays)
+  auto adj_step = [&](Wave<T>& av, int64_t k, const T* d, DevStep ds) -> int {
     const int its = (int)(k / sub), i2 = (int)(k % sub);
     ela::StepArgs<T> a = base_args<T>(h, av.p0, av.p1);
     set_pm<T>(h, a, av);
@@ -1613,10 +1624,10 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
     for (int c = 0; c < 9; ++c) R.drv[c] = a.drv[c];
     for (int m = 0; m < 8; ++m) R.grad[m] = a.grad[m];
     ela::PointArgs<T> S = make_points<T>(h, sp, av.p0, TmViews<T>(tm_src).v, true, nsrc_tot);
-    if (kdev) { set_step(h, R, kdev, 1); set_step(h, S, kdev, 0); }
+    if (ds.k) { set_step(h, R, ds.k, ds.off, 1); set_step(h, S, ds.k, ds.off, 0); }
     launch_inject<T, false, true, true>(h, R, its, f[i2 + 1], f[hh + i2 + 1]);
     launch_record<T>(h, S, its, f[i2], f[hh + i2]);
-    if (kdev) launch(h, ela::k_count, dim3(1), dim3(64), kdev, -1);
+    if (ds.advance) launch(h, ela::k_count, dim3(1), dim3(64), ds.k, ds.advance);
     av.swap();
     return ELA_OK;
   };
@@ -1643,28 +1654,29 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
       else CHK(load_checkpoint<T>(h, bg, c1, plan.fmt, scales + s1 * 18));
       if (seg_graph && j1 - j0 == plan.K) {
         // recompute with derivative store, then the transposed steps, each one replayed graph
+        // the counter holds the segment's first step j0; step j0 + kk reads it at offset kk
         auto store_seg = [&]() -> int {
           Wave<T> wl = bg;
           for (int64_t kk = 0; kk < plan.K; ++kk)
-            CHK((grad_fwd_step<T, true>(h, wl, sp, src_tr, 0, drv + kk * 9 * np, kst)));
+            CHK((grad_fwd_step<T, true>(h, wl, sp, src_tr, 0, drv + kk * 9 * np, DevStep{kst, (int)kk, 0})));
           return ELA_OK;
         };
         auto adj_seg = [&]() -> int {
           Wave<T> wl = ad;
-          for (int64_t kk = plan.K - 1; kk >= 0; --kk) CHK(adj_step(wl, 0, drv + kk * 9 * np, kst));
+          for (int64_t kk = plan.K - 1; kk >= 0; --kk)
+            CHK(adj_step(wl, 0, drv + kk * 9 * np, DevStep{kst, (int)kk, 0}));
           return ELA_OK;
         };
         hipGraphExec_t ex = nullptr;
-        CHK(graph_for(h, h->graphs[G_STORE], store_seg, &ex));
         set_counter(h, kst, j0);
+        CHK(graph_for(h, h->graphs[G_STORE], store_seg, &ex));
         CHK(replay(h, h->graphs[G_STORE], ex, 1));
         CHK(graph_for(h, h->graphs[G_ADJ], adj_seg, &ex));
-        set_counter(h, kst, j1 - 1);
         CHK(replay(h, h->graphs[G_ADJ], ex, 1));
       } else {
         for (int64_t k = j0; k < j1; ++k)
           CHK((grad_fwd_step<T, true>(h, bg, sp, src_tr, k, drv + (k - j0) * 9 * np)));
-        for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(ad, k, drv + (k - j0) * 9 * np, nullptr));
+        for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(ad, k, drv + (k - j0) * 9 * np, DevStep()));
       }
       steps += 2 * (j1 - j0);
       const int64_t n8 = 8 * np;

@@ -865,19 +865,20 @@ struct PointArgs {
   int min_pad, nz, nx, ny, nzi, zoff;
   const int* ytab;         // stacked windows: per plane {window y0, window ny, window nx, 0}, or null
   const int* its_dev;      // coarse sample index read from device memory (graph replays), or null
-  // fine step k read from device memory (graph replays of gradient steps), or
-  // null: the coarse sample is k / sub and the weights are filt[i + fo] and
-  // filt[sub + 1 + i + fo] with i = k % sub (the host's fa / fb of that step)
+  // fine step k = *step_dev + step_off read from device memory (graph replays
+  // of gradient steps), or null: the coarse sample is k / sub and the weights
+  // are filt[i + fo] and filt[sub + 1 + i + fo] with i = k % sub (the host's
+  // fa / fb of that step)
   const int* step_dev;
   const T* filt;
-  int sub, fo;
+  int sub, fo, step_off;
 };
 
 // Coarse sample and time-filter weights of a point launch.
 template <typename T>
 __device__ __forceinline__ void point_time(const PointArgs<T>& a, int& its, T& fa, T& fb) {
   if (a.step_dev) {
-    const int k = *a.step_dev;
+    const int k = *a.step_dev + a.step_off;
     its = k / a.sub;
     const int i = k - its * a.sub + a.fo;
     fa = a.filt[i];

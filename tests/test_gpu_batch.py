@@ -142,3 +142,34 @@ def test_whole_model_window_equals_plain_operator(medium):
           flush=True)
     assert abs(Jp - Jb) <= 1e-12 * Jp
     assert e < 1e-12
+
+
+def test_persistent_propagator_repeats_and_isolates_batches():
+    """The batches of BatchedFwi share one propagator whose grid holds the
+    widest and tallest stack: a second misfit-and-gradient call (buffers,
+    checkpoints and graphs reused) repeats the first bit for bit, and the
+    shared grid (wider than the first stack, taller than the second) gives
+    each batch what a propagator of its own stack gives, in FP32 with exact
+    checkpoints."""
+    from elasticisolib_3d_amd.batch import BatchedFwi, WindowStack
+    m = _model()
+    start = [x * (1 + 0.02 * np.random.default_rng(5 + k).standard_normal(x.shape)) for k, x in enumerate(m)]
+    shots = _shots()
+    kw = dict(precision="f32", interval=4, checkpoint="native")
+    op = BatchedFwi(m[0].shape, D, 5, NTS, DTS, SUB, 0.95, shots, batch=2, **kw)
+    op.synthesize(m)
+    own = [WindowStack(op._model_zxy(), D, 5, [s.window for s in g]) for g in op._groups()]
+    nx, ny = op.shared_grid()
+    assert nx > own[0].nx and ny > own[1].ny
+    J1, g1 = op.misfit_and_gradient(start)
+    J2, g2 = op.misfit_and_gradient(start)
+    assert J1 == J2 and np.array_equal(g1, g2)
+    # the last group holds the whole-model window alone, on the grid of the first (two-window) stack
+    alone = BatchedFwi(m[0].shape, D, 5, NTS, DTS, SUB, 0.95, shots[2:], batch=1, **kw)
+    Ja, ga = alone.misfit_and_gradient(start)
+    parts = BatchedFwi(m[0].shape, D, 5, NTS, DTS, SUB, 0.95, shots[:2], batch=2, **kw)
+    Jp, gp = parts.misfit_and_gradient(start)
+    e = float(np.linalg.norm(g1 - (ga + gp)) / np.linalg.norm(g1))
+    print(f"shared-grid batches vs own-grid batches: J {J1:.10e} vs {Ja + Jp:.10e}, gradient rel {e:.2e}", flush=True)
+    assert abs(J1 - (Ja + Jp)) <= 1e-9 * J1
+    assert e < 1e-6
