@@ -96,3 +96,26 @@ def test_medium_scaling_and_its_transpose_in_torch_match_the_library():
     g3 = P.medium_gradient(g8, med, dtw)
     g3t = MT.medium_gradient(torch.from_numpy(g8), torch.from_numpy(med), dtw).numpy()
     assert np.allclose(g3t, g3, rtol=1e-14, atol=0)
+
+
+def test_stack_in_a_larger_shared_grid():
+    """A stack placed in a larger grid (the propagator several stacks share):
+    flat positions, stacking and unstacking use the grid's extents, the
+    planes past the last window and the columns past a window's width hold
+    the fill value, and a grid smaller than the stack is refused."""
+    natural = _stack()
+    ws = WindowStack((20, 30, 26), 10.0, 5, natural.windows, grid=(natural.nx + 16, natural.ny + 9))
+    assert (ws.nx, ws.ny) == (natural.nx + 16, natural.ny + 9)
+    assert ws.y_off == natural.y_off and ws.ny_w == natural.ny_w and ws.nx_w == natural.nx_w
+    rng = np.random.default_rng(2)
+    per = [rng.standard_normal((ws.ny_w[k], ws.nx_w[k], ws.nz)) for k in range(3)]
+    st = ws.stack(per, fill=1.0)
+    assert st.shape == (ws.ny, ws.nx, ws.nz)
+    assert np.all(st[natural.ny:] == 1.0) and np.all(st[:, natural.nx:] == 1.0)
+    for k in range(3):
+        pos = rng.integers(0, per[k].size, 50)
+        sp = ws.stacked_positions(k, [pos])[0]
+        assert np.array_equal(st.ravel()[sp], per[k].ravel()[pos])
+        assert np.array_equal(ws.unstack(k, st), per[k])
+    with pytest.raises(ValueError):
+        WindowStack((20, 30, 26), 10.0, 5, natural.windows, grid=(natural.nx - 1, natural.ny))
