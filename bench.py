@@ -436,7 +436,9 @@ def fwi_batch_compare(n_shots, batch, nts, checkpoint, autotune, full_shots, smo
             sig[c][0] = cfg["wavelet"]
         shots.append(BatchShot(src, rec, Window(x0, y0, wx, wy), sig))
     shape = cfg["true"][0].shape
-    kw = dict(precision="f32", interval=20, checkpoint=checkpoint, autotune=autotune)
+    # progress every 8th batch on the log (long runs stay visibly alive)
+    kw = dict(precision="f32", interval=20, checkpoint=checkpoint, autotune=autotune,
+              progress=lambda m: log(f"[bench] {m}") if any(f"batch {k}/" in m for k in range(8, 1000, 8)) else None)
     ref = BatchedFwi(shape, cfg["d"], cfg["pad"], nts, cfg["dts"], cfg["sub"], cfg["alpha_cos"], shots, batch=1, **kw)
     t0 = time.perf_counter()
     ref.synthesize(cfg["true"])

@@ -240,12 +240,13 @@ class BatchedFwi:
     on one persistent propagator whose grid holds the widest and tallest
     stack of windows (`shared_grid`); a batch sets its windows and stacked
     medium on it, so wavefields, checkpoints and graphs are allocated and
-    captured once for all batches and calls (`release` frees them)."""
+    captured once for all batches and calls (`release` frees them).
+    `progress`, a callable taking a string, is told after every batch."""
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
                  bf16=None, device: int = 0, tiling=None, checkpoint: str = "s16", autotune: bool = False,
-                 medium: str = "elastic", graph: bool = True):
+                 medium: str = "elastic", graph: bool = True, progress=None):
         self.model_shape, self.d, self.pad = tuple(model_shape), float(d), int(pad)
         self.nts, self.dts, self.sub, self.alpha = int(nts), float(dts), int(sub), float(alpha_cos)
         self.dtw = self.dts / self.sub
@@ -261,6 +262,7 @@ class BatchedFwi:
             raise ValueError("medium must be 'elastic' or 'scaled'")
         self.medium = medium
         self.graph = bool(graph)   # replay the gradient's step pairs and checkpoint segments as HIP graphs
+        self.progress = progress   # callable(str) told after every batch, or None
         self.timing = {}
         # one propagator for every batch and every call: its grid holds the widest and the tallest
         # stack, so its wavefields, checkpoints and graphs are allocated, tuned and captured once
@@ -370,10 +372,16 @@ class BatchedFwi:
             out.append(rg.to_irregular(reg))
         return out
 
+    def _tell(self, what: str, i: int, n: int, t0: float) -> None:
+        if self.progress is not None:
+            self.progress(f"{what}: batch {i + 1}/{n} done ({time.perf_counter() - t0:.1f} s)")
+
     def synthesize(self, model) -> None:
         import torch
         model_d = self._model_device(model)
-        for group in self._groups():
+        groups = self._groups()
+        t0 = time.perf_counter()
+        for gi, group in enumerate(groups):
             ws, meds, prop, geos, td = self._setup(group, model_d)
             src, _ = self._src_traces(ws, meds, geos, group, td)
             rec = prop.nl_forward(self._merged(ws, geos, 0), src, self._merged(ws, geos, 1))
@@ -381,6 +389,7 @@ class BatchedFwi:
             for s, d in zip(group, self._split_rec(geos, rec)):
                 s.observed = d
             del prop
+            self._tell("observed data", gi, len(groups), t0)
 
     def misfit_and_gradient(self, model):
         """(J, dJ/d(rho, lambda, mu) on the full model [3, ny, nx, nz]). The
@@ -393,7 +402,8 @@ class BatchedFwi:
         g3 = torch.zeros_like(model_d)
         J = 0.0
         t_prop = 0.0
-        for group in self._groups():
+        groups = self._groups()
+        for gi, group in enumerate(groups):
             ws, meds, prop, geos, td = self._setup(group, model_d)
             src, raw = self._src_traces(ws, meds, geos, group, td)
             spos, rpos = self._merged(ws, geos, 0), self._merged(ws, geos, 1)
@@ -428,6 +438,7 @@ class BatchedFwi:
                 ws.window_adjoint_t(k, MT.medium_gradient(g8k, meds[k], self.dtw), g3)
             del prop, g8, meds
             torch.cuda.empty_cache()
+            self._tell("gradient", gi, len(groups), t0)
         g3h = g3.cpu().numpy()
         self.timing = {"total_s": time.perf_counter() - t0, "propagation_s": t_prop}
         return J, g3h
