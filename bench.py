@@ -309,7 +309,11 @@ def bench_fwi(args):
     log(f"[bench] model {w.name} padded {ax.nz}x{ax.nx}x{ax.ny} built in {time.time() - t0:.1f}s")
     shots = _fwi_shots(w, ax)
     prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, checkpoint=args.checkpoint,
-                      autotune=not args.no_autotune, medium=args.medium)
+                      autotune=not args.no_autotune and not args.launch, medium=args.medium, graph=not args.no_graph)
+    if args.launch:
+        from elasticisolib_3d_amd.autotune import apply, _parse_shapes
+        tx, yc, order, xg = _parse_shapes(args.launch)[0]
+        apply(prob.prop, {"tile_x": tx, "y_chunk": yc, "order": order, "x_group": xg})
     t0 = time.perf_counter()
     prob.synthesize(w.rho, w.lame, w.mu)
     log(f"[bench] observed data of {len(shots)} shots synthesized in {time.perf_counter() - t0:.1f}s")
