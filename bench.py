@@ -309,6 +309,7 @@ def bench_fwi(args):
     log(f"[bench] model {w.name} padded {ax.nz}x{ax.nx}x{ax.ny} built in {time.time() - t0:.1f}s")
     shots = _fwi_shots(w, ax)
     prob = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, shots, "f32", interval=20, checkpoint=args.checkpoint,
+                      derivatives=args.derivatives,
                       autotune=not args.no_autotune and not args.launch, medium=args.medium, graph=not args.no_graph)
     if args.launch:
         from elasticisolib_3d_amd.autotune import apply, _parse_shapes
@@ -325,6 +326,7 @@ def bench_fwi(args):
         times.append(time.perf_counter() - t0)
         log(f"[bench] gradient {i + 1}: J = {J:.6e}, {times[-1]:.1f}s ({prob.last_timing})")
     plan = prob.prop.gradient_plan_used()
+    drv_fmt = prob.prop.gradient_derivatives_used()
     tuned = dict(given, given=True) if args.launch else prob.tuned
     del prob
     torch.cuda.empty_cache()
@@ -361,11 +363,17 @@ def bench_fwi(args):
         del p64
         torch.cuda.empty_cache()
         res = {}
-        for name, ck in (("f32_bf16", "bf16"), ("f32_s16", "s16"), ("f32_exact", "native")):
-            p = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f32", interval=20, checkpoint=ck)
+        # (checkpoints, derivative combinations): each checkpoint format with the derivative storage of
+        # the run, both exact, and exact checkpoints with 16-bit derivatives alone
+        d16 = "s16"
+        for name, ck, drv in (("f32_bf16", "bf16", args.derivatives), ("f32_s16", "s16", args.derivatives),
+                              ("f32_exact", "native", "native"), (f"f32_exact_ckpt_{d16}_drv", "native", d16)):
+            p = FwiProblem(ax, w.nts, w.dts, w.sub, w.alpha_cos, one, "f32", interval=20, checkpoint=ck,
+                           derivatives=drv)
             t0 = time.perf_counter()
             Jx, gx = p.misfit_and_gradient(*start)
-            res[name] = (Jx, gx, time.perf_counter() - t0, p.prop.gradient_plan_used())
+            res[name] = (Jx, gx, time.perf_counter() - t0, p.prop.gradient_plan_used(),
+                         p.prop.gradient_derivatives_used())
             log(f"[bench] {name} gradient (shot 1): J = {Jx:.9e}, {res[name][2]:.1f}s, plan {res[name][3]}")
             del p
             torch.cuda.empty_cache()
@@ -373,13 +381,14 @@ def bench_fwi(args):
         def rel(a, b):
             return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
-        for name, (Jx, gx, tx, px) in res.items():
+        for name, (Jx, gx, tx, px, dx) in res.items():
             acc[name] = {
                 "grad_rel_l2_vs_fp64": rel(gx, g64),
                 "grad_rel_l2_per_param": [rel(gx[k], g64[k]) for k in range(3)],
                 "misfit_rel_vs_fp64": abs(Jx - J64) / J64,
                 "checkpoint_plan": {"interval": px[0], "outer_interval": px[1], "format": px[2],
                                     "checkpoint_GB": round(px[3] / 1e9, 2)},
+                "derivatives": dx,
             }
             if fd_rows:
                 errs = [abs(float(gx[k][idx]) - fd) / abs(fd) for (k, idx, fd, _) in fd_rows]
@@ -408,6 +417,7 @@ def bench_fwi(args):
                         f"sigma=20 cells, {len(shots)} explosive shots (8 Hz Ricker), {w.nts} coarse samples x {w.sub} "
                         f"fine steps, {len(w.rec_xyz)} surface receivers, L2 misfit, checkpoints every 20 fine steps",
             "gradient_s_per_call": [round(t, 2) for t in times],
+            "derivatives": drv_fmt,
             "checkpoint_plan": {"interval": plan[0], "outer_interval": plan[1], "format": plan[2],
                                 "checkpoint_GB_per_shot": round(plan[3] / 1e9, 2)},
             "launch": None if tuned is None else {k: tuned[k] for k in ("tile_x", "y_chunk", "order", "x_group")},
@@ -533,6 +543,49 @@ def bench_fwi_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_hbm(args):
+    """HBM rates of plain streaming mixes (ela_stream_probe): reads only and
+    read / in-place write mixes, on arrays of the 400x400x250 grid's size
+    (290 MB, partly held by the 256 MB Infinity Cache) and of 1 GiB (HBM's
+    own rate). The stencil's roofline context: what HBM delivers for a
+    byte mix with writes."""
+    import torch
+    from elasticisolib_3d_amd.propagator import Propagator
+
+    prop = Propagator(40, 40, 40, 4, 2, 10.0, 10.0, 10.0, 0.002, 10, 0.95, "f32")
+    mixes = [(2, 0), (12, 0), (1, 1), (10, 3), (11, 6), (12, 3), (14, 6), (12, 9)]
+    sizes = {"grid_400_290MB": 72453888, "1GiB": 1 << 28}
+    table = {}
+    for (r, wr) in mixes:
+        row = {}
+        for name, n in sizes.items():
+            row[name] = round(prop.stream_probe(r, wr, n, reps=10), 1)
+        table[f"{r}r_{wr}w"] = row
+        log(f"[bench] {r:2d} reads {wr} writes: " + ", ".join(f"{k} {v:.0f} GB/s" for k, v in row.items()))
+    prop.close()
+    torch.cuda.synchronize()
+    with_writes = [v["1GiB"] for k, v in table.items() if not k.endswith("_0w")]
+    out = {
+        "metric": "HBM GB/s of streaming read / write mixes",
+        "value": round(min(with_writes), 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": 10,
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (zero-filled arrays)",
+        "config": {"workload": "plain streaming kernels: R arrays read, the first W rewritten in place, 16-byte "
+                               "accesses; value = the slowest mix with writes on 1 GiB arrays",
+                   "peak": HBM_PEAK_GBS, "mixes": table,
+                   "mix_with_writes_range_1GiB": [min(with_writes), max(with_writes)]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -552,11 +605,11 @@ def main():
                     help="skip the streaming-kernel reference of the roofline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for counter calibration)")
-    ap.add_argument("--workload", choices=("nl_forward", "born", "fp64_check", "fwi", "fwi_batch"),
+    ap.add_argument("--workload", choices=("nl_forward", "born", "fp64_check", "fwi", "fwi_batch", "hbm"),
                     default="nl_forward",
                     help="nl_forward (default): the headline line; born: Born forward/adjoint and dot test; "
                          "fp64_check: FP32 vs FP64 traces on the forward workload; fwi: FWI gradient shots/s "
-                         "and gradient accuracy")
+                         "and gradient accuracy; hbm: HBM rates of streaming read / write mixes")
     ap.add_argument("--shots", type=int, default=4, help="fwi_batch: number of shots")
     ap.add_argument("--fwi-slice-shots", type=int, default=4,
                     help="nl_forward: shots of the batched-vs-plain FWI slice added to the line (0: skip)")
@@ -572,6 +625,8 @@ def main():
                     help="fwi: cells checked by central differences (-1: skip the accuracy part)")
     ap.add_argument("--checkpoint", choices=("bf16", "s16", "native"), default="s16",
                     help="fwi, fwi_batch: storage of the gradient's outer checkpoints")
+    ap.add_argument("--derivatives", choices=("s16", "native"), default=None,
+                    help="fwi: storage of the backward's derivative combinations (default: the library's, exact)")
     ap.add_argument("--born-segment", type=int, default=0,
                     help="Born adjoint storage: 0 automatic, -1 all samples, K checkpoint segment")
     args = ap.parse_args()
@@ -596,6 +651,8 @@ def main():
         return bench_fp64_check(args)
     if args.workload == "fwi":
         return bench_fwi(args)
+    if args.workload == "hbm":
+        return bench_hbm(args)
     if args.workload == "fwi_batch":
         return bench_fwi_batch(args)
 

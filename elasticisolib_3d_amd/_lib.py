@@ -28,7 +28,8 @@ EXPORTS = (
     "ela_born_stagger", "ela_born_unstagger", "ela_medium_gradient",
     "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
     "ela_set_windows", "ela_set_graph", "ela_stream_probe", "ela_set_elastic", "ela_set_born_mode",
-    "ela_nl_forward_noalloc", "ela_get_medium",
+    "ela_nl_forward_noalloc", "ela_get_medium", "ela_set_gradient_derivatives", "ela_gradient_derivatives_used",
+    "ela_time_gradient_kernels",
 )
 
 
@@ -123,6 +124,9 @@ def load() -> ctypes.CDLL:
     lib.ela_gradient_forward.argtypes = [vp, P, Tr, P, Tr, i32]
     lib.ela_gradient_backward.argtypes = [vp, P, Tr, P, Tr, vp, Tr, i32]
     lib.ela_set_gradient_checkpoints.argtypes = [vp, i64, i64, i32, i64]
+    lib.ela_set_gradient_derivatives.argtypes = [vp, i32]
+    lib.ela_gradient_derivatives_used.argtypes = [vp, ctypes.POINTER(i32)]
+    lib.ela_time_gradient_kernels.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double)]
     lib.ela_gradient_plan_used.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
                                            ctypes.POINTER(i64)]
     lib.ela_born_storage_used.argtypes = [vp, ctypes.POINTER(i64)]

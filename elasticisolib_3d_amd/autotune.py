@@ -46,10 +46,13 @@ def _device_name() -> str:
         return "unknown"
 
 
-def _key(prop) -> str:
+def _key(prop, target: str = "forward") -> str:
     ny, nx, nz = prop.shape
     kind = getattr(prop, "medium_kind", "scaled")   # the steppers differ with the medium kind
-    return f"{_device_name()}|{prop.precision}|{kind}|{nz}x{nx}x{ny}"
+    key = f"{_device_name()}|{prop.precision}|{kind}|{nz}x{nx}x{ny}"
+    if target == "gradient":   # the gradient's kernel mix, with its derivative storage
+        key += f"|gradient|{prop.gradient_derivatives_used()}"
+    return key
 
 
 def _load() -> Dict[str, dict]:
@@ -78,11 +81,18 @@ def apply(prop, best: dict) -> None:
 
 
 def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = None, use_cache: bool = True,
-             verbose: bool = False) -> dict:
+             verbose: bool = False, target: str = "forward") -> dict:
     """Time every candidate launch shape on `prop` (its medium must be set),
     apply the fastest and return {tile_x, y_chunk, order, x_group, ms_step,
-    table}. The wavefields of the handle are overwritten."""
-    key = _key(prop)
+    table}. The wavefields of the handle are overwritten. `target`
+    "forward" ranks the shapes by a forward step (velocity + stress
+    launches); "gradient" by the three kernel pairs of an adjoint-state
+    gradient step (forward, recomputation with derivative store, transposed
+    step with imaging: ela_time_gradient_kernels), which do not share the
+    forward's best shape."""
+    if target not in ("forward", "gradient"):
+        raise ValueError("target must be 'forward' or 'gradient'")
+    key = _key(prop, target)
     if use_cache and candidates is None:
         hit = _load().get(key)
         if hit:
@@ -95,14 +105,21 @@ def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = N
     for tx, yc, order, xg in cands:
         prop.set_tiling(tx, yc)
         prop.set_tile_order(order, xg)
-        kv, ks = prop.time_step_kernels(reps)
-        table.append((kv + ks, kv, ks, tx, yc, order, xg))
+        if target == "gradient":
+            parts = prop.time_gradient_kernels(reps)
+        else:
+            parts = prop.time_step_kernels(reps)
+        table.append((sum(parts),) + tuple(parts) + (tx, yc, order, xg))
         if verbose:
-            print(f"[autotune] tile {tx}x{yc} order {order}/{xg}: velocity {kv:.4f} ms stress {ks:.4f} ms", flush=True)
+            print(f"[autotune] tile {tx}x{yc} order {order}/{xg}: " + " + ".join(f"{p:.4f}" for p in parts)
+                  + " ms", flush=True)
     table.sort()
-    t, kv, ks, tx, yc, order, xg = table[0]
-    best = {"tile_x": tx, "y_chunk": yc, "order": order, "x_group": xg, "ms_step": t, "ms_velocity": kv,
-            "ms_stress": ks}
+    t, tx, yc, order, xg = table[0][0], *table[0][-4:]
+    best = {"tile_x": tx, "y_chunk": yc, "order": order, "x_group": xg, "ms_step": t, "target": target}
+    if target == "gradient":
+        best.update(ms_forward=table[0][1], ms_recompute=table[0][2], ms_transposed=table[0][3])
+    else:
+        best.update(ms_velocity=table[0][1], ms_stress=table[0][2])
     apply(prop, best)
     if use_cache and candidates is None:
         tab = _load()
@@ -135,6 +152,8 @@ def main(argv=None) -> int:
     ap.add_argument("--shapes", default="", help="tile_x x y_chunk / order / x_group, comma separated")
     ap.add_argument("--medium", default="scaled", choices=("scaled", "elastic"),
                     help="8 scaled medium arrays (ela_set_medium) or rho, lambda, mu (ela_set_elastic)")
+    ap.add_argument("--target", default="forward", choices=("forward", "gradient"),
+                    help="time the forward step, or the three kernel pairs of a gradient step")
     a = ap.parse_args(argv)
     nz, nx, ny = (int(v) for v in a.grid.split(","))
     prop = Propagator(nz, nx, ny, 10, 8, 12.5, 12.5, 12.5, 0.004, 30, 0.99, a.precision)
@@ -160,6 +179,13 @@ def main(argv=None) -> int:
     for tx, yc, order, xg in shapes:
         prop.set_tiling(tx, yc)
         prop.set_tile_order(order, xg)
+        if a.target == "gradient":
+            fw, rc, tr = prop.time_gradient_kernels(a.reps)
+            print(f"[autotune] shape {tx}x{yc}/{order}/{xg}: forward {fw:.4f} ms recompute {rc:.4f} ms "
+                  f"transposed {tr:.4f} ms step {fw + rc + tr:.4f} ms", flush=True)
+            if best is None or fw + rc + tr < best[0]:
+                best = (fw + rc + tr, tx, yc, order, xg)
+            continue
         kv, ks = prop.time_step_kernels(a.reps)
         gbs = bpc * cells / ((kv + ks) * 1e-3) / 1e9
         print(f"[autotune] shape {tx}x{yc}/{order}/{xg}: velocity {kv:.4f} ms stress {ks:.4f} ms "

@@ -241,12 +241,14 @@ class BatchedFwi:
     stack of windows (`shared_grid`); a batch sets its windows and stacked
     medium on it, so wavefields, checkpoints and graphs are allocated and
     captured once for all batches and calls (`release` frees them).
-    `progress`, a callable taking a string, is told after every batch."""
+    `progress`, a callable taking a string, is told after every batch;
+    `derivatives` is the storage of the backward's derivative combinations
+    (as FwiProblem; None: the library's default)."""
 
     def __init__(self, model_shape, d: float, pad: int, nts: int, dts: float, sub: int, alpha_cos: float,
                  shots: Sequence[BatchShot], batch: int = 1, precision: str = "f32", interval: int = 20,
                  bf16=None, device: int = 0, tiling=None, checkpoint: str = "s16", autotune: bool = False,
-                 medium: str = "elastic", graph: bool = True, progress=None):
+                 medium: str = "elastic", graph: bool = True, progress=None, derivatives=None):
         self.model_shape, self.d, self.pad = tuple(model_shape), float(d), int(pad)
         self.nts, self.dts, self.sub, self.alpha = int(nts), float(dts), int(sub), float(alpha_cos)
         self.dtw = self.dts / self.sub
@@ -263,6 +265,7 @@ class BatchedFwi:
         self.medium = medium
         self.graph = bool(graph)   # replay the gradient's step pairs and checkpoint segments as HIP graphs
         self.progress = progress   # callable(str) told after every batch, or None
+        self.derivatives = derivatives
         self.timing = {}
         # one propagator for every batch and every call: its grid holds the widest and the tallest
         # stack, so its wavefields, checkpoints and graphs are allocated, tuned and captured once
@@ -293,6 +296,8 @@ class BatchedFwi:
                 if len(self.tiling) > 2:
                     prop.set_tile_order(*self.tiling[2:])
             prop.set_gradient_checkpoints(self.interval, 0, self.checkpoint)
+            if self.derivatives is not None:
+                prop.set_gradient_derivatives(self.derivatives)
             prop.set_graph(self.graph)
             self._prop = prop
             self._tune_pending = self.autotune and not self.tiling
@@ -326,7 +331,7 @@ class BatchedFwi:
         del wins
         if self._tune_pending:
             from .autotune import autotune
-            autotune(prop, reps=5)
+            autotune(prop, reps=5, target="gradient")
             self._tune_pending = False
         geos = []
         for k, s in enumerate(group):

@@ -215,6 +215,7 @@ struct ela_handle {
   int64_t grad_outer = 0;       // outer interval K1 (0: automatic)
   int grad_fmt = ELA_CK_S16;    // storage of the outer checkpoints (int16 block-scaled: inside 1e-3 of FP64)
   int64_t grad_budget = 0;      // device-memory budget of the automatic plan (0: free memory)
+  int drv_fmt = ELA_CK_NATIVE;  // stored derivative combinations: ELA_CK_NATIVE or ELA_CK_S16
   bool grad_ready = false;      // a gradient forward has stored its checkpoints
   int64_t plan_K = 0, plan_K1 = 0, plan_n1 = 0;
   int plan_fmt = 0;
@@ -1275,7 +1276,41 @@ struct GradPlan {
   int64_t K = 20, K1 = 20, n1 = 0, n2 = 1;
   int fmt = ELA_CK_S16;
   size_t ck1_bytes = 0, ck2_bytes = 0;  // one checkpoint
+  int dfmt = ELA_CK_NATIVE;             // storage of the derivative combinations (ELA_CK_*)
+  size_t drv_bytes = 0;                 // the derivative combinations of one step
 };
+
+// Storage of the derivative combinations that the recomputation writes and
+// the transposed steps read (ela_set_gradient_derivatives); the int16 format
+// is FP32-only, so an FP64 handle always stores them exactly.
+int drv_format(const ela_handle* h) { return h->esize == 4 ? h->drv_fmt : ELA_CK_NATIVE; }
+
+size_t drv_step_bytes(const ela_handle* h, int fmt) {
+  if (fmt == ELA_CK_S16) return (size_t)9 * (h->stride * 2 + h->stride / ela::DQ_BLOCK * 4);
+  return (size_t)9 * h->stride * h->esize;
+}
+
+// Derivative combinations of one step: native arrays [9][stride], or int16
+// [9][stride] followed by the float scales [9][stride / DQ_BLOCK].
+struct DrvSlot {
+  char* p = nullptr;
+  int fmt = ELA_CK_NATIVE;
+};
+
+template <typename T>
+void set_drv(const ela_handle* h, ela::StepArgs<T>& a, DrvSlot d, bool store) {
+  const int64_t np = h->stride;
+  for (int c = 0; c < 9; ++c) {
+    if (d.fmt == ELA_CK_S16) {
+      a.dq16[c] = reinterpret_cast<int16_t*>(d.p) + c * np;
+      a.dqs[c] = reinterpret_cast<float*>(d.p + (size_t)9 * np * 2) + c * (np / ela::DQ_BLOCK);
+    } else if (store) {
+      a.dst[c] = reinterpret_cast<T*>(d.p) + c * np;
+    } else {
+      a.drv[c] = reinterpret_cast<const T*>(d.p) + c * np;
+    }
+  }
+}
 
 // Cells of the checkpoint packing: the grid interior (with stacked windows
 // this includes the windows' FAT planes and the columns beyond a narrower
@@ -1296,10 +1331,12 @@ int grad_plan(ela_handle* h, GradPlan& p) {
   const int64_t nI = interior_count(h);
   p.K = std::max<int64_t>(1, std::min<int64_t>(h->grad_interval, N));
   p.fmt = h->grad_fmt;
+  p.dfmt = drv_format(h);
+  p.drv_bytes = drv_step_bytes(h, p.dfmt);
   p.ck1_bytes = ck_bytes(p.fmt, nI, h->esize);
   p.ck2_bytes = (size_t)18 * nI * h->esize;
   const double arr = double(h->stride) * h->esize;
-  const double fixed = arr * (9.0 * p.K + 8 + 36) + double(h->stride) * 8.0 * 8;  // drv, g32, wave2, g64
+  const double fixed = double(p.drv_bytes) * p.K + arr * (8 + 36) + double(h->stride) * 8.0 * 8;  // drv, g32, wave2, g64
   auto mem_for = [&](int64_t K1) {
     const int64_t n1 = (N + K1 - 1) / K1;
     return fixed + double(n1) * p.ck1_bytes + (K1 > p.K ? double(K1 / p.K) * p.ck2_bytes : 0.0);
@@ -1387,15 +1424,17 @@ int load_checkpoint(ela_handle* h, const Wave<T>& w, const void* in, int fmt, co
 // and the counter advanced by ds.advance after the step (graph replays); k is
 // then unused.
 template <typename T, bool STORE>
-int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k, T* dst,
+int grad_fwd_step(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_tr[9], int64_t k, DrvSlot dst,
                   DevStep ds = DevStep()) {
-  const int64_t sub = h->cfg.sub, hh = sub + 1, np = h->stride;
+  const int64_t sub = h->cfg.sub, hh = sub + 1;
   const double* f = h->filt.data();
   const int its = (int)(k / sub), i2 = (int)(k % sub);  // forward it2 = i2 + 1
   ela::StepArgs<T> a = base_args<T>(h, w.p0, w.p1);
   if (STORE) {
-    for (int c = 0; c < 9; ++c) a.dst[c] = dst + c * np;
-    CHK((launch_step_m<T, ela::MODE_FWD, ela::F_STORE, ela::F_STORE>(h, a)));
+    set_drv<T>(h, a, dst, true);
+    constexpr int FQ = sizeof(T) == 4 ? ela::F_STORE | ela::F_Q16 : ela::F_STORE;
+    if (dst.fmt == ELA_CK_S16) CHK((launch_step_m<T, ela::MODE_FWD, FQ, FQ>(h, a)));
+    else CHK((launch_step_m<T, ela::MODE_FWD, ela::F_STORE, ela::F_STORE>(h, a)));
   } else {
     CHK((launch_step_m<T, ela::MODE_FWD, 0, 0>(h, a)));
   }
@@ -1428,8 +1467,8 @@ int grad_fwd_range(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_
     kst = step_counter(h);
     auto pair = [&]() -> int {
       Wave<T> wl = w;
-      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, DevStep{kst, 0, 0})));
-      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, nullptr, DevStep{kst, 1, 2})));
+      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, DrvSlot(), DevStep{kst, 0, 0})));
+      CHK((grad_fwd_step<T, false>(h, wl, sp, src_tr, 0, DrvSlot(), DevStep{kst, 1, 2})));
       return ELA_OK;
     };
     CHK(graph_for(h, h->graphs[G_FWD], pair, &ex));
@@ -1442,7 +1481,7 @@ int grad_fwd_range(ela_handle* h, Wave<T>& w, const DevPoints& sp, T* const src_
       if (e != hipSuccess) return fail(ELA_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
       k += 2;
     } else {
-      CHK((grad_fwd_step<T, false>(h, w, sp, src_tr, k, nullptr)));
+      CHK((grad_fwd_step<T, false>(h, w, sp, src_tr, k, DrvSlot())));
       ++k;
     }
   }
@@ -1555,6 +1594,8 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   const int64_t nI = interior_count(h);
   plan.ck1_bytes = ck_bytes(plan.fmt, nI, h->esize);
   plan.ck2_bytes = (size_t)18 * nI * h->esize;
+  plan.dfmt = drv_format(h);  // unchanged since the forward (setting it requires a new forward)
+  plan.drv_bytes = drv_step_bytes(h, plan.dfmt);
 
   DevPoints sp, rp;
   CHK(convert_points(h, sp_in, rp_in, mem, sp, rp));
@@ -1595,11 +1636,12 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
     if (h->ck2.bytes != plan.n2 * plan.ck2_bytes) h->ck2.release();
     CHK(h->ck2.alloc(plan.n2 * plan.ck2_bytes));
   }
-  CHK(h->drv.alloc((size_t)plan.K * 9 * np * sizeof(T)));
+  CHK(h->drv.alloc((size_t)plan.K * plan.drv_bytes));
   CHK(h->g32.alloc((size_t)8 * np * sizeof(T)));
   CHK(h->g64.alloc((size_t)8 * np * sizeof(double)));
   CHK(h->wave2.alloc((size_t)36 * np * sizeof(T)));
-  T* drv = (T*)h->drv.p;
+  // derivative combinations of inner step kk of a segment
+  auto drv = [&](int64_t kk) { return DrvSlot{(char*)h->drv.p + kk * plan.drv_bytes, plan.dfmt}; };
   T* g32 = (T*)h->g32.p;
   double* g64 = (double*)h->g64.p;
   CHK(fill<T>(h, g32, 8 * np));
@@ -1612,16 +1654,18 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
   int64_t steps = 0;
 
   // adjoint step k (host index), or the step the device counter names (graph replays)
-  auto adj_step = [&](Wave<T>& av, int64_t k, const T* d, DevStep ds) -> int {
+  auto adj_step = [&](Wave<T>& av, int64_t k, DrvSlot d, DevStep ds) -> int {
     const int its = (int)(k / sub), i2 = (int)(k % sub);
     ela::StepArgs<T> a = base_args<T>(h, av.p0, av.p1);
     set_pm<T>(h, a, av);
-    for (int c = 0; c < 9; ++c) a.drv[c] = d + c * np;
+    set_drv<T>(h, a, d, false);
     for (int m = 0; m < 8; ++m) a.grad[m] = g32 + m * np;
-    CHK((launch_step_m<T, ela::MODE_BORN_ADJ_PM, ela::F_GRAD, ela::F_GRAD>(h, a)));
+    constexpr int FQ = sizeof(T) == 4 ? ela::F_GRAD | ela::F_Q16 : ela::F_GRAD;
+    if (d.fmt == ELA_CK_S16) CHK((launch_step_m<T, ela::MODE_BORN_ADJ_PM, FQ, FQ>(h, a)));
+    else CHK((launch_step_m<T, ela::MODE_BORN_ADJ_PM, ela::F_GRAD, ela::F_GRAD>(h, a)));
     ela::PointArgs<T> R = make_points<T>(h, rp, av.p0, TmViews<T>(tm_adj).v, true, nrec_tot);
     set_pm<T>(h, R, av);
-    for (int c = 0; c < 9; ++c) R.drv[c] = a.drv[c];
+    for (int c = 0; c < 9; ++c) { R.drv[c] = a.drv[c]; R.dq16[c] = a.dq16[c]; R.dqs[c] = a.dqs[c]; }
     for (int m = 0; m < 8; ++m) R.grad[m] = a.grad[m];
     ela::PointArgs<T> S = make_points<T>(h, sp, av.p0, TmViews<T>(tm_src).v, true, nsrc_tot);
     if (ds.k) { set_step(h, R, ds.k, ds.off, 1); set_step(h, S, ds.k, ds.off, 0); }
@@ -1658,13 +1702,13 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
         auto store_seg = [&]() -> int {
           Wave<T> wl = bg;
           for (int64_t kk = 0; kk < plan.K; ++kk)
-            CHK((grad_fwd_step<T, true>(h, wl, sp, src_tr, 0, drv + kk * 9 * np, DevStep{kst, (int)kk, 0})));
+            CHK((grad_fwd_step<T, true>(h, wl, sp, src_tr, 0, drv(kk), DevStep{kst, (int)kk, 0})));
           return ELA_OK;
         };
         auto adj_seg = [&]() -> int {
           Wave<T> wl = ad;
           for (int64_t kk = plan.K - 1; kk >= 0; --kk)
-            CHK(adj_step(wl, 0, drv + kk * 9 * np, DevStep{kst, (int)kk, 0}));
+            CHK(adj_step(wl, 0, drv(kk), DevStep{kst, (int)kk, 0}));
           return ELA_OK;
         };
         hipGraphExec_t ex = nullptr;
@@ -1675,8 +1719,8 @@ int grad_backward(ela_handle* h, const ela_points* sp_in, const ela_traces* st_i
         CHK(replay(h, h->graphs[G_ADJ], ex, 1));
       } else {
         for (int64_t k = j0; k < j1; ++k)
-          CHK((grad_fwd_step<T, true>(h, bg, sp, src_tr, k, drv + (k - j0) * 9 * np)));
-        for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(ad, k, drv + (k - j0) * 9 * np, DevStep()));
+          CHK((grad_fwd_step<T, true>(h, bg, sp, src_tr, k, drv(k - j0))));
+        for (int64_t k = j1 - 1; k >= j0; --k) CHK(adj_step(ad, k, drv(k - j0), DevStep()));
       }
       steps += 2 * (j1 - j0);
       const int64_t n8 = 8 * np;
@@ -1722,6 +1766,56 @@ int time_kernels(ela_handle* h, int reps, double* ms_v, double* ms_s) {
   HIPCHK(hipGetLastError());
   *ms_v = tv / reps;
   *ms_s = ts / reps;
+  return ELA_OK;
+}
+
+// Mean durations of the three kernel pairs of a gradient step on the
+// handle's grid and launch shape: the forward step, the recomputing step
+// with derivative store (in the handle's derivative format) and the
+// transposed step with imaging. Runs on the handle's wavefields and the
+// gradient's own buffers (sized here for at least one step, zero-filled),
+// whose contents do not matter for timing; stored checkpoints are untouched.
+template <typename T>
+int time_gradient_kernels(ela_handle* h, int reps, double* ms3) {
+  const int64_t np = h->stride;
+  const int dfmt = drv_format(h);
+  CHK(h->wave2.alloc((size_t)36 * np * sizeof(T)));
+  CHK(h->drv.alloc(drv_step_bytes(h, dfmt)));
+  CHK(h->g32.alloc((size_t)8 * np * sizeof(T)));
+  CHK(fill<T>(h, h->wave2.p, 36 * np));
+  CHK(fill<T>(h, h->g32.p, 8 * np));
+  CHK(fill<float>(h, h->drv.p, (int64_t)drv_step_bytes(h, dfmt) / 4));  // a multiple of 4 bytes
+  Wave<T> w = wave_ptrs<T>(h->wave, np), ad = wave_ptrs<T>(h->wave2, np, true);
+  const DrvSlot slot{(char*)h->drv.p, dfmt};
+  ela::StepArgs<T> f = base_args<T>(h, w.p0, w.p1);
+  ela::StepArgs<T> r = base_args<T>(h, w.p0, w.p1);
+  set_drv<T>(h, r, slot, true);
+  ela::StepArgs<T> g = base_args<T>(h, ad.p0, ad.p1);
+  set_pm<T>(h, g, ad);
+  set_drv<T>(h, g, slot, false);
+  for (int m = 0; m < 8; ++m) g.grad[m] = (T*)h->g32.p + m * np;
+  constexpr int FQS = sizeof(T) == 4 ? ela::F_STORE | ela::F_Q16 : ela::F_STORE;
+  constexpr int FQG = sizeof(T) == 4 ? ela::F_GRAD | ela::F_Q16 : ela::F_GRAD;
+  auto step = [&](int which) -> int {
+    if (which == 0) return launch_step_m<T, ela::MODE_FWD, 0, 0>(h, f);
+    if (which == 1) {
+      if (dfmt == ELA_CK_S16) return launch_step_m<T, ela::MODE_FWD, FQS, FQS>(h, r);
+      return launch_step_m<T, ela::MODE_FWD, ela::F_STORE, ela::F_STORE>(h, r);
+    }
+    if (dfmt == ELA_CK_S16) return launch_step_m<T, ela::MODE_BORN_ADJ_PM, FQG, FQG>(h, g);
+    return launch_step_m<T, ela::MODE_BORN_ADJ_PM, ela::F_GRAD, ela::F_GRAD>(h, g);
+  };
+  for (int k = 0; k < 3; ++k) {
+    CHK(step(k));  // warm caches and code
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < reps; ++i) CHK(step(k));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    ms3[k] = t / reps;
+  }
+  HIPCHK(hipGetLastError());
   return ELA_OK;
 }
 
@@ -2016,6 +2110,26 @@ int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_
   });
 }
 
+int ela_set_gradient_derivatives(ela_handle* h, int format) {
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (format != ELA_CK_NATIVE && format != ELA_CK_S16)
+      return fail(ELA_ERR_ARG, "derivative format must be ELA_CK_NATIVE or ELA_CK_S16");
+    if (format == ELA_CK_S16 && h->esize != 4) return fail(ELA_ERR_ARG, "ELA_CK_S16 derivatives need an FP32 handle");
+    h->drv_fmt = format;
+    h->grad_ready = false;  // the plan of the next gradient forward sizes the derivative buffers
+    return ELA_OK;
+  });
+}
+
+int ela_gradient_derivatives_used(const ela_handle* h, int* format) {
+  return guarded([&]() -> int {
+    if (!h || !format) return fail(ELA_ERR_ARG, "null argument");
+    *format = drv_format(h);
+    return ELA_OK;
+  });
+}
+
 int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* format,
                            int64_t* checkpoint_bytes) {
   return guarded([&]() -> int {
@@ -2159,7 +2273,10 @@ int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, 
     else if (reads == 10 && writes == 3) kern = ela::k_stream<10, 3>;
     else if (reads == 11 && writes == 6) kern = ela::k_stream<11, 6>;
     else if (reads == 12 && writes == 9) kern = ela::k_stream<12, 9>;
-    if (!kern) return fail(ELA_ERR_ARG, "stream probe mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9)");
+    else if (reads == 12 && writes == 0) kern = ela::k_stream<12, 0>;
+    else if (reads == 2 && writes == 0) kern = ela::k_stream<2, 0>;
+    if (!kern)
+      return fail(ELA_ERR_ARG, "stream probe mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9), (12, 0), (2, 0)");
     DeviceGuard dg(h->cfg.device);
     DevBuf buf, tab;
     CHK(buf.alloc((size_t)reads * n * 4));
@@ -2193,6 +2310,15 @@ int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* 
     DeviceGuard dg(h->cfg.device);
     return h->esize == 4 ? time_kernels<float>(h, reps, ms_velocity, ms_stress)
                          : time_kernels<double>(h, reps, ms_velocity, ms_stress);
+  });
+}
+
+int ela_time_gradient_kernels(ela_handle* h, int reps, double* ms3) {
+  return guarded([&]() -> int {
+    if (!h || reps < 1 || !ms3) return fail(ELA_ERR_ARG, "bad argument");
+    if (!h->have_medium) return fail(ELA_ERR_STATE, "medium not set");
+    DeviceGuard dg(h->cfg.device);
+    return h->esize == 4 ? time_gradient_kernels<float>(h, reps, ms3) : time_gradient_kernels<double>(h, reps, ms3);
   });
 }
 

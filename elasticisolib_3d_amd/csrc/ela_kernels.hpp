@@ -49,8 +49,17 @@ enum { F_SS = 1,     // add the secondary sources ssl/ssr (Born forward)
        F_ACC = 2,    // accumulate the new level into accl/accr (coarse-time interpolation)
        F_STORE = 4,  // write the step's derivative combinations to dst (gradient recomputation)
        F_GRAD = 8,   // gradient imaging: grad[m] += new level x stored derivatives drv
-       F_MED3 = 16   // MODE_FWD / MODE_ADJ: form the 8 staggered, scaled medium values from (rho, lambda, mu)
+       F_MED3 = 16,  // MODE_FWD / MODE_ADJ: form the 8 staggered, scaled medium values from (rho, lambda, mu)
+       F_Q16 = 32    // FP32 with F_STORE / F_GRAD: the derivative combinations as block-scaled int16 (dq16, dqs)
 };
+
+// Derivative combinations stored as int16 (F_Q16): one scale per block of
+// DQ_BLOCK values, the block being one 128-byte FP32 line of a z row, i.e.
+// one half of a tile row's wave (rows and tiles start on a line), so that
+// the scale of element i is dqs[i / DQ_BLOCK] for the steppers and the point
+// kernels alike. q = rint(x / s), s = max |x| / 32767 over the block; the
+// value restored is q s, within s / 2 (2^-16 of the block's largest value).
+constexpr int DQ_BLOCK = 32;
 
 template <typename T>
 struct StepArgs {
@@ -67,6 +76,8 @@ struct StepArgs {
   const T* q[9];        // MODE_BORN_ADJ_PM: M c of the current level
   T* qn[9];             // MODE_BORN_ADJ_PM: M n of the new level (written)
   const T* ep[3];       // F_MED3: rho, lambda, mu
+  int16_t* dq16[9];     // F_Q16: int16 derivative combinations (written by F_STORE, read by F_GRAD)
+  float* dqs[9];        // F_Q16: their scales, one per DQ_BLOCK elements
   T tdt;                // F_MED3: 2 dtw
   double tdt64;         // F_MED3: 2 dtw in FP64 (FP32 steppers form buoyancies and normal moduli in FP64)
   T ssfl, ssfr;         // filter weights for the secondary source
@@ -205,6 +216,45 @@ __device__ __forceinline__ void row_to_lds(const float* base, uint32_t i, float*
   __builtin_amdgcn_global_load_lds(at(base, i), (__attribute__((address_space(3))) void*)row, 4, 0, 0);
 }
 
+// Largest magnitude over each 32-lane half of the wave (one DQ_BLOCK), in
+// every lane: DPP lane moves inside each 16-lane row (swap neighbours, swap
+// pairs, mirror within 8, mirror within 16) and one swizzle that exchanges
+// the two rows of a half (lane ^ 16). All 64 lanes must be active. The
+// magnitudes are compared as unsigned integers (the bit patterns of
+// non-negative floats order as their values), which needs no NaN quieting.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_umax(uint32_t u) {
+  return max(u, (uint32_t)__builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ float half_wave_absmax(float x) {
+  uint32_t u = __float_as_uint(fabsf(x));
+  u = dpp_umax<0xB1>(u);   // quad_perm [1, 0, 3, 2]
+  u = dpp_umax<0x4E>(u);   // quad_perm [2, 3, 0, 1]
+  u = dpp_umax<0x141>(u);  // row_half_mirror
+  u = dpp_umax<0x140>(u);  // row_mirror
+  u = max(u, (uint32_t)__builtin_amdgcn_ds_swizzle((int)u, 0x401F));  // lane ^ 16
+  return __uint_as_float(u);
+}
+
+// F_Q16 store of one derivative combination at element i: every lane of the
+// wave calls it; lanes outside the interior pass x = 0. The last z tile of a
+// row may reach past the row into the next row's leading non-interior line
+// ([0, zoff + FAT) = [0, 32)), which no stepper reads and no other tile
+// writes. The hardware reciprocal (1 ulp) serves for 1 / m: |x| 32767 rcp(m)
+// stays below 32767.5, so q fits int16, and the stored scale is what the
+// reader multiplies by, so the error stays within s / 2 plus a few FP32 ulps.
+__device__ __forceinline__ void store_dq16(int16_t* q, float* sc, uint32_t i, float x) {
+  const float m = half_wave_absmax(x);
+  const float inv = m > 0.f ? 32767.f * __builtin_amdgcn_rcpf(m) : 0.f;
+  st_nt(q, i, (int16_t)__float2int_rn(x * inv));
+  if ((threadIdx.x & (DQ_BLOCK - 1)) == 0) *at(sc, i / DQ_BLOCK) = m * (1.f / 32767.f);
+}
+
+__device__ __forceinline__ float load_dq16(const int16_t* q, const float* sc, uint32_t i) {
+  return (float)*at(q, i) * *at(sc, i / DQ_BLOCK);
+}
+
 // Values that the derivative operators see. In MODE_BORN_ADJ they are the
 // medium-multiplied fields (stepAdjGpuBorn_3D multiplies at every loaded
 // position, kernelsElaGpu_3D.cu:864-1040); MODE_BORN_ADJ_PM reads the same
@@ -274,6 +324,7 @@ struct TileCoord {
   int wy0, wny, wnx;       // the tile's window: first plane, planes, x extent (whole grid without windows)
   bool empty;              // no interior cell in the tile (its window is narrower than the tile's x)
   bool okc, active;        // own column inside the grid / inside the interior
+  bool xin;                // the tile row (wave-uniform) is an interior x row
   bool xh, okxl, okxh;     // x-halo loader (ty < 4) and its low/high validity
   bool zh, okzh;           // z-halo loader (lanes 0-7) and validity
   int colh;                // LDS column of the z-halo value
@@ -330,7 +381,8 @@ __device__ __forceinline__ TileCoord tile_coord(int nbz, int nbx, int nby, int y
   nx = t.wnx;
   t.empty = t.x0 >= nx - FAT;
   t.okc = t.izo < nz && t.ix < nx;
-  t.active = (t.izo < nz - FAT) && (t.ix < nx - FAT);
+  t.xin = t.ix < nx - FAT;
+  t.active = (t.izo < nz - FAT) && t.xin;
   t.xh = t.ty < FAT;
   const int ixlo = t.x0 - FAT + t.ty, ixhi = t.x0 + BX + t.ty;
   t.okxl = t.xh && t.izo < nz;
@@ -418,6 +470,9 @@ __device__ __forceinline__ void normal_moduli(T tdt, double tdt64, T lam, T mu, 
 template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
+  constexpr bool Q16 = (F & F_Q16) != 0, QS = Q16 && (F & F_STORE) != 0;
+  static_assert(!Q16 || sizeof(T) == 4, "F_Q16 is an FP32 format");
+  static_assert(!QS || (!SS && !ACC && (F & F_GRAD) == 0 && MODE == MODE_FWD), "F_Q16 stores: MODE_FWD recomputation");
   constexpr bool HOIST = sizeof(T) == 4;  // per-cell operands loaded ahead of the stores
   static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ || MODE == MODE_BORN_ADJ_PM,
                 "F_MED3 needs MODE_FWD, MODE_ADJ or MODE_BORN_ADJ_PM");
@@ -520,7 +575,9 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
     const bool more = iy + 1 < t.y1;
     VelStage<T> nx_;
     if (more) fetch(nx_, iy + 1, b ^ 1);  // buffer b ^ 1 is free: every wave passed the last barrier
-    if (t.active) {
+    // F_STORE with F_Q16: the whole wave of an interior row takes the branch
+    // (the block maxima are wave reductions) and only the interior lanes store
+    if (QS ? t.xin : t.active) {
       const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
       // FP32: every per-cell operand is loaded before the first store (the
@@ -536,7 +593,11 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
         if constexpr (HOIST) {
           if (SS) ss3[f] = load_ss(f);
           if (ACC) { al3[f] = *at(a.accl[f], i); ar3[f] = *at(a.accr[f], i); }
-          if (F & F_GRAD) { g3[f] = *at(a.grad[f], i); d3[f] = ld_nt(a.drv[f], i); }  // BX_/BY_/BZ_ == VX/VY/VZ
+          if (F & F_GRAD) {  // BX_/BY_/BZ_ == VX/VY/VZ
+            g3[f] = *at(a.grad[f], i);
+            if constexpr (Q16) d3[f] = load_dq16(a.dq16[f], a.dqs[f], i);
+            else d3[f] = ld_nt(a.drv[f], i);
+          }
         }
       }
       if (MED3) {
@@ -548,11 +609,15 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
       const T sx = dmx<T, BX>(l_xx[b], r, q, cx) + dq(qxy, cy) + dpz<T, BX>(l_xz[b], r, q, cz);
       const T sy = dpx<T, BX>(l_xy[b], r, q, cx) + dq(qyy, cy) + dpz<T, BX>(l_yz[b], r, q, cz);
       const T sz = dpx<T, BX>(l_xz[b], r, q, cx) + dq(qyz, cy) + dmz<T, BX>(l_zz[b], r, q, cz);
-      const T w = damp_w(a, t.izo, t.ix, iy, t.wy0, t.wny, t.wnx);
+      T w = T(0);
+      if (!QS || t.active) w = damp_w(a, t.izo, t.ix, iy, t.wy0, t.wny, t.wnx);
       const T s3[3] = {sx, sy, sz};
       if (F & F_STORE) {
 #pragma unroll
-        for (int f = 0; f < 3; ++f) st_nt(a.dst[f], i, s3[f]);
+        for (int f = 0; f < 3; ++f) {
+          if constexpr (QS) store_dq16(a.dq16[f], a.dqs[f], i, t.active ? s3[f] : T(0));
+          else st_nt(a.dst[f], i, s3[f]);
+        }
       }
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
@@ -567,6 +632,7 @@ __global__ void __launch_bounds__(TZ * BX) k_velocity(StepArgs<T> a) {
         }
         if (SS) u += ss3[f];
         u *= w;
+        if (QS && !t.active) continue;
         st_nt(a.o[f], i, u);
         if (MODE == MODE_BORN_ADJ_PM) st_nt(a.qn[f], i, u * m3[f]);
         if (ACC) {
@@ -609,6 +675,9 @@ struct StressStage {
 template <typename T, int BX, int MODE, int F>
 __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
   constexpr bool SS = (F & F_SS) != 0, ACC = (F & F_ACC) != 0, MED3 = (F & F_MED3) != 0;
+  constexpr bool Q16 = (F & F_Q16) != 0, QS = Q16 && (F & F_STORE) != 0;
+  static_assert(!Q16 || sizeof(T) == 4, "F_Q16 is an FP32 format");
+  static_assert(!QS || (!SS && !ACC && (F & F_GRAD) == 0 && MODE == MODE_FWD), "F_Q16 stores: MODE_FWD recomputation");
   constexpr bool HOIST = sizeof(T) == 4;  // per-cell operands loaded ahead of the stores
   static_assert(!MED3 || MODE == MODE_FWD || MODE == MODE_ADJ || MODE == MODE_BORN_ADJ_PM,
                 "F_MED3 needs MODE_FWD, MODE_ADJ or MODE_BORN_ADJ_PM");
@@ -714,7 +783,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       fetch(nx_, iy + 1, b ^ 1);
       if (MED3) fetch_mu(nx_, iy + 1, b ^ 1);
     }
-    if (t.active) {
+    if (QS ? t.xin : t.active) {  // F_Q16 stores: see k_velocity
       const uint32_t i = (uint32_t)iy * ys + t.col;
       const T* cx = a.cx; const T* cy = a.cy; const T* cz = a.cz;
       // FP32: every per-cell operand is loaded before the first store (see k_velocity)
@@ -722,7 +791,10 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       auto load_ss = [&](int f) { return *at(a.ssl[f], i) * a.ssfl + *at(a.ssr[f], i) * a.ssfr; };
       auto load_grad = [&]() {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) d6[k] = ld_nt(a.drv[SXX + k], i);
+        for (int k = 0; k < 6; ++k) {
+          if constexpr (Q16) d6[k] = load_dq16(a.dq16[SXX + k], a.dqs[SXX + k], i);
+          else d6[k] = ld_nt(a.drv[SXX + k], i);
+        }
 #pragma unroll
         for (int m = 0; m < 5; ++m) g5[m] = *at(a.grad[L2M + m], i);
       };
@@ -757,10 +829,17 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
       const T gxy = dq(qx, cy) + dmx<T, BX>(l_vy[b], r, q, cx);
       const T gxz = dmz<T, BX>(l_vx[b], r, q, cz) + dmx<T, BX>(l_vz[b], r, q, cx);
       const T gyz = dq(qz, cy) + dmz<T, BX>(l_vy[b], r, q, cz);
-      const T w = damp_w(a, t.izo, t.ix, iy, t.wy0, t.wny, t.wnx);
+      T w = T(0);
+      if (!QS || t.active) w = damp_w(a, t.izo, t.ix, iy, t.wy0, t.wny, t.wnx);
       if (F & F_STORE) {
-        st_nt(a.dst[SXX], i, exx); st_nt(a.dst[SYY], i, eyy); st_nt(a.dst[SZZ], i, ezz);
-        st_nt(a.dst[SXZ], i, gxz); st_nt(a.dst[SXY], i, gxy); st_nt(a.dst[SYZ], i, gyz);
+        if constexpr (QS) {
+          const T e6[6] = {exx, eyy, ezz, gxz, gxy, gyz};
+#pragma unroll
+          for (int k = 0; k < 6; ++k) store_dq16(a.dq16[SXX + k], a.dqs[SXX + k], i, t.active ? e6[k] : T(0));
+        } else {
+          st_nt(a.dst[SXX], i, exx); st_nt(a.dst[SYY], i, eyy); st_nt(a.dst[SZZ], i, ezz);
+          st_nt(a.dst[SXZ], i, gxz); st_nt(a.dst[SXY], i, gxy); st_nt(a.dst[SYZ], i, gyz);
+        }
       }
       T u[6];
       if (MODE == MODE_BORN_ADJ || MODE == MODE_BORN_ADJ_PM) {
@@ -790,6 +869,7 @@ __global__ void __launch_bounds__(TZ * BX) k_stress(StepArgs<T> a) {
         if (SS) v += ss6[k];
         v *= w;
         u[k] = v;
+        if (QS && !t.active) continue;
         st_nt(a.o[f], i, v);
         if (ACC) {
           *at(a.accl[f], i) = al6[k] + v * a.accfl;
@@ -854,8 +934,11 @@ struct PointArgs {
   T* accl[9];
   T* accr[9];
   T accfl, accfr;
-  // optional gradient imaging of the injected value (adjoint-state gradient)
+  // optional gradient imaging of the injected value (adjoint-state gradient);
+  // with dq16 set the derivative combinations are the F_Q16 int16 arrays
   const T* drv[9];
+  const int16_t* dq16[9];
+  const float* dqs[9];
   T* grad[8];
   // optional premultiplied level to keep consistent (MODE_BORN_ADJ_PM)
   T* q[9];
@@ -959,15 +1042,18 @@ __global__ void __launch_bounds__(256) k_inject(PointArgs<T> a) {
   }
   if (GRAD) {
     // the stepper imaged the level before this injection; add the injected part
+    auto drv = [&](int cc) -> T {
+      return a.dq16[cc] ? T(load_dq16(a.dq16[cc], a.dqs[cc], (uint32_t)p)) : a.drv[cc][p];
+    };
     if (c < 3) {
-      a.grad[c][p] += v * a.drv[c][p];
+      a.grad[c][p] += v * drv(c);
     } else if (c == SXX) {
-      const T dxx = a.drv[SXX][p], dyy = a.drv[SYY][p], dzz = a.drv[SZZ][p];
+      const T dxx = drv(SXX), dyy = drv(SYY), dzz = drv(SZZ);
       const T l2 = vn[0] * dxx + vn[1] * dyy + vn[2] * dzz;
       a.grad[L2M][p] += l2;
       a.grad[LAM][p] += (vn[0] + vn[1] + vn[2]) * (dxx + dyy + dzz) - l2;
     } else if (c >= SXZ) {
-      a.grad[MXZ + (c - SXZ)][p] += v * a.drv[c][p];
+      a.grad[MXZ + (c - SXZ)][p] += v * drv(c);
     }
   }
 }
@@ -1370,7 +1456,10 @@ __global__ void __launch_bounds__(256) k_scale_medium(MedArgs<T> a) {
 // Streaming reference for the roofline: R arrays of n4 16-byte vectors are
 // read and the first W of them rewritten in place (non-temporal stores), the
 // byte mix of a stencil kernel (reads of c, o and the medium, o rewritten)
-// without neighbour accesses. Grid-stride loop over vectors.
+// without neighbour accesses. Grid-stride loop over vectors. W = 0 reads
+// only: the sum of all four lanes of the vector sum is compared with a value
+// the (zero-filled) inputs never give, so every load stays live and nothing
+// is written.
 template <int R, int W>
 __global__ void __launch_bounds__(256) k_stream(float4* const* arrays, int64_t n4) {
   float4* a[R];
@@ -1382,6 +1471,9 @@ __global__ void __launch_bounds__(256) k_stream(float4* const* arrays, int64_t n
     for (int k = 1; k < R; ++k) {
       const float4 v = a[k][i];
       s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+    if constexpr (W == 0) {
+      if (s.x + s.y + s.z + s.w == -1.2345e30f) a[0][i] = s;
     }
 #pragma unroll
     for (int k = 0; k < W; ++k) {

@@ -164,19 +164,23 @@ class FwiProblem:
     (exact); the older `bf16` flag, when given, means "bf16" (True) or
     "native" (False). `components` is the sampled component list of the
     misfit (the reference's par key comp, e.g. "vx,vy,vz,p"; all 9 by
-    default). `autotune` times the stencil launch
-    shapes once when the first medium is set, `medium` chooses between
+    default). `autotune` times the stencil launch shapes once when the
+    first medium is set, ranked by the gradient step's three kernel pairs
+    (forward, recomputation with derivative store, transposed step with
+    imaging), `medium` chooses between
     handing (rho, lambda, mu) to the library ("elastic") and the 8 host-scaled
     arrays ("scaled"); the two agree exactly in FP64. `graph` replays the
     gradient's pairs of forward steps and its checkpoint segments (forward
     with derivative store, transposed steps with imaging) as HIP graphs
     captured once and kept by the propagator; the results are bit-identical
-    to launching every step from the host."""
+    to launching every step from the host. `derivatives` ("native", the
+    library's default, or "s16": half the memory, FP32 only) is the storage
+    of the derivative combinations the backward recomputes and images with."""
 
     def __init__(self, axes: PaddedAxes, nts: int, dts: float, sub: int, alpha_cos: float, shots: Sequence[FwiShot],
                  precision: str = "f32", interval: int = 20, outer_interval: int = 0, bf16: Optional[bool] = None,
                  device: int = 0, checkpoint: str = "s16", autotune: bool = False, medium: str = "elastic",
-                 components: str = ALL_COMPONENTS, graph: bool = True):
+                 components: str = ALL_COMPONENTS, graph: bool = True, derivatives: Optional[str] = None):
         self.axes = axes
         self.components = components
         self.rows = component_rows(components)
@@ -190,6 +194,8 @@ class FwiProblem:
         if bf16 is not None:
             checkpoint = "bf16" if bf16 else "native"
         self.prop.set_gradient_checkpoints(interval, outer_interval, checkpoint)
+        if derivatives is not None:
+            self.prop.set_gradient_derivatives(derivatives)
         self.prop.set_graph(graph)
         self.dxyz = axes.dx * axes.dy * axes.dz
         self.last_timing = {}
@@ -234,7 +240,7 @@ class FwiProblem:
             self.prop.set_medium(torch.tensor(med8, dtype=self._tdtype(), device=self.dev))
         if self._tune and self.tuned is None:
             from .autotune import autotune
-            self.tuned = autotune(self.prop, reps=5)
+            self.tuned = autotune(self.prop, reps=5, target="gradient")
         return med8
 
     def _to_irregular(self, shot, reg):

@@ -293,6 +293,20 @@ class Propagator:
                                                     checkpoint_format(checkpoint), int(budget_bytes)),
               "ela_set_gradient_checkpoints")
 
+    def set_gradient_derivatives(self, fmt="s16") -> None:
+        """Storage of the derivative combinations the gradient's backward
+        recomputes and images with: "native" (exact, the default) or "s16"
+        (FP32 only: int16 with one scale per 32 values, half the memory of
+        the derivative buffers, a slightly slower backward). Takes effect
+        from the next gradient forward."""
+        check(self.lib.ela_set_gradient_derivatives(self.h, checkpoint_format(fmt)), "ela_set_gradient_derivatives")
+
+    def gradient_derivatives_used(self) -> str:
+        """"s16" or "native": the derivative storage of the next gradient."""
+        f = ctypes.c_int()
+        check(self.lib.ela_gradient_derivatives_used(self.h, ctypes.byref(f)), "ela_gradient_derivatives_used")
+        return CHECKPOINT_NAMES[f.value]
+
     def gradient_plan_used(self):
         """(interval, outer_interval, checkpoint format name, checkpoint bytes) of the last gradient forward."""
         k, k1, b, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()
@@ -357,6 +371,16 @@ class Propagator:
         check(self.lib.ela_time_step_kernels(self.h, int(reps), ctypes.byref(mv), ctypes.byref(ms)),
               "ela_time_step_kernels")
         return mv.value, ms.value
+
+    def time_gradient_kernels(self, reps: int = 10):
+        """Mean launch durations [ms] of the forward step, the recomputing
+        step with derivative store and the transposed step with imaging of
+        the adjoint-state gradient (ela_time_gradient_kernels; overwrites the
+        wavefields, keeps stored checkpoints)."""
+        ms = (ctypes.c_double * 3)()
+        self._mem_of()
+        check(self.lib.ela_time_gradient_kernels(self.h, int(reps), ms), "ela_time_gradient_kernels")
+        return ms[0], ms[1], ms[2]
 
     def stream_probe(self, reads: int, writes: int, n: int, reps: int = 20) -> float:
         """GB/s of a plain streaming kernel with the given read / in-place

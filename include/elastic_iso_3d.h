@@ -286,6 +286,20 @@ int ela_set_gradient_checkpoints(ela_handle* h, int64_t interval, int64_t outer_
 /* Plan of the last gradient forward (format = ELA_CK_*) and the bytes its outer checkpoints hold. */
 int ela_gradient_plan_used(const ela_handle* h, int64_t* interval, int64_t* outer_interval, int* format,
                            int64_t* checkpoint_bytes);
+/* Storage of the 9 derivative combinations per cell and step that the
+ * backward's recomputation writes and its transposed steps read: ELA_CK_NATIVE
+ * (exact; the default) or ELA_CK_S16 (FP32 handles only): each value within
+ * 2^-16 of the largest of its 32-value block (one 128-byte line of a z row),
+ * 2.125 bytes per value instead of 4, which halves the backward's largest
+ * buffer (interval x 9 arrays) for grids where memory is short. The values
+ * enter the gradient linearly and are never propagated (measured: 1.9e-6
+ * from the FP64 gradient against 1.1e-6 exact on a 256x256x160 model), but
+ * the block maxima cost the recomputation more than the bytes save (about 5%
+ * slower backward). Takes effect from the next ela_gradient_forward. No
+ * counterpart in the reference. */
+int ela_set_gradient_derivatives(ela_handle* h, int format);
+/* The derivative storage the next gradient uses (ELA_CK_NATIVE or ELA_CK_S16). */
+int ela_gradient_derivatives_used(const ela_handle* h, int* format);
 
 /* Axes of one acquisition grid (its origin already shifted by half a cell on
  * the staggered axes, as python_float/Elastic_iso_float_3D.py:97-110 does),
@@ -383,6 +397,14 @@ int ela_set_tile_order(ela_handle* h, int order, int x_group);
  * handle's stream). The wavefields are overwritten; no counterpart in the
  * reference. */
 int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* ms_stress);
+/* Mean milliseconds of the three kernel pairs of an adjoint-state gradient
+ * step on the handle's grid and launch shape, over `reps` launches each:
+ * ms3[0] the forward step, ms3[1] the recomputing step with derivative store
+ * (in the handle's derivative format), ms3[2] the transposed step with
+ * imaging. Overwrites the wavefields and sizes the gradient's step buffers;
+ * stored checkpoints are kept. For the launch-shape tuner of the gradient.
+ * No counterpart in the reference. */
+int ela_time_gradient_kernels(ela_handle* h, int reps, double* ms3);
 
 /* Diagnostic streaming reference for the roofline: a plain kernel that reads
  * `reads` arrays of n FP32 values (16-byte accesses) and rewrites the first
@@ -390,10 +412,12 @@ int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* 
  * (14, 6) kernels without neighbour accesses, or (10, 3) and (11, 6) when they
  * form the medium from (rho, lambda, mu) -- timed over `reps` launches on the
  * handle's stream; *gbs = (reads + writes) * 4 * n bytes per launch / time.
- * Mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9); n a multiple of 4.
- * (12, 9) is the mix of a step that advances one time sub-lattice of the
- * leapfrog by a velocity and a stress half-step in one pass (v_old, stress,
- * rho, lambda, mu read; velocity and stress rewritten).
+ * Mixes: (1, 1), (12, 3), (14, 6), (10, 3), (11, 6), (12, 9), (12, 0), (2, 0);
+ * n a multiple of 4. (12, 9) is the mix of a step that advances one time
+ * sub-lattice of the leapfrog by a velocity and a stress half-step in one pass
+ * (v_old, stress, rho, lambda, mu read; velocity and stress rewritten);
+ * (12, 0) and (2, 0) only read. With reads * n * 4 bytes well above the
+ * 256 MB Infinity Cache the figures are HBM's own.
  * Allocates and frees reads * n * 4 bytes. No counterpart in the reference. */
 int ela_stream_probe(ela_handle* h, int reads, int writes, int64_t n, int reps, double* gbs);
 

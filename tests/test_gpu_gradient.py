@@ -25,31 +25,96 @@ def make_prop(precision):
 
 
 CASES = [
-    # precision, interval, outer, checkpoint format, tolerance (relative L2 vs the FP64 oracle)
-    ("f64", 3, 3, "native", 1e-11),   # one level, exact checkpoints
-    ("f64", 2, 6, "native", 1e-11),   # two levels (outer segment recomputed)
-    ("f64", 20, 0, "native", 1e-11),  # one segment longer than the run
-    ("f32", 4, 4, "native", 1e-5),
-    ("f32", 2, 8, "bf16", 5e-3),      # bfloat16 outer checkpoints
-    ("f32", 2, 8, "s16", 5e-5),       # block-scaled int16 outer checkpoints
-    ("f64", 3, 3, "s16", 5e-5),
+    # precision, interval, outer, checkpoint format, derivative storage,
+    # tolerance (relative L2 vs the FP64 oracle)
+    ("f64", 3, 3, "native", "native", 1e-11),   # one level, exact checkpoints
+    ("f64", 2, 6, "native", "native", 1e-11),   # two levels (outer segment recomputed)
+    ("f64", 20, 0, "native", "native", 1e-11),  # one segment longer than the run
+    ("f32", 4, 4, "native", "native", 1e-5),
+    ("f32", 2, 8, "bf16", "native", 5e-3),      # bfloat16 outer checkpoints
+    ("f32", 2, 8, "s16", "native", 5e-5),       # block-scaled int16 outer checkpoints
+    ("f64", 3, 3, "s16", "native", 5e-5),
+    ("f32", 4, 4, "native", "s16", 5e-5),       # block-scaled int16 derivative combinations
+    ("f32", 2, 8, "s16", "s16", 5e-5),          # both, two levels
 ]
 
 
-@pytest.mark.parametrize("precision,K,K1,ck,tol", CASES)
-def test_gradient_matches_oracle(problem, precision, K, K1, ck, tol):
+@pytest.mark.parametrize("precision,K,K1,ck,drv,tol", CASES)
+def test_gradient_matches_oracle(problem, precision, K, K1, ck, drv, tol):
     P, g8_ref, gsrc_ref = problem
     prop = make_prop(precision)
     prop.set_medium(P["med"])
     prop.set_gradient_checkpoints(K, K1, ck)
+    prop.set_gradient_derivatives(drv)
     rec = prop.gradient_forward(P["spos"], P["src"], P["rpos"])
     fwd = prop.nl_forward(P["spos"], P["src"], P["rpos"])
     assert all(np.array_equal(a, b) for a, b in zip(rec, fwd)), "gradient forward data differ from nl_forward"
     g8, gsrc = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
     e8, es = rel(g8, g8_ref), rel(gsrc, gsrc_ref)
-    print(f"gradient {precision} K={K} K1={K1} {ck}: g8 rel={e8:.3e} src rel={es:.3e} "
+    print(f"gradient {precision} K={K} K1={K1} {ck}/{drv}: g8 rel={e8:.3e} src rel={es:.3e} "
           f"plan={prop.gradient_plan_used()}", flush=True)
     assert e8 < tol and es < tol
+
+
+def test_derivative_storage_formats(problem):
+    """The derivative combinations the backward recomputes and images with
+    are stored exactly by default; FP32 handles also take block-scaled int16
+    (FP64 handles refuse it). The int16 gradient differs from the
+    exact-storage one by what the format allows (each value within 2^-16 of
+    its 32-value block's largest magnitude) and the source gradient, which
+    does not image, not at all; switching back restores the exact result."""
+    from elasticisolib_3d_amd._lib import ElaError
+    P, _, _ = problem
+    p64 = make_prop("f64")
+    assert p64.gradient_derivatives_used() == "native"
+    for fmt in ("s16", "bf16"):
+        with pytest.raises(ElaError):
+            p64.set_gradient_derivatives(fmt)
+    p64.close()
+    prop = make_prop("f32")
+    with pytest.raises(ElaError):
+        prop.set_gradient_derivatives("bf16")
+    assert prop.gradient_derivatives_used() == "native"
+    prop.set_medium(P["med"])
+    prop.set_gradient_checkpoints(4, 4, "native")
+    out = {}
+    for fmt in ("native", "s16", "native"):
+        prop.set_gradient_derivatives(fmt)
+        assert prop.gradient_derivatives_used() == fmt
+        prop.gradient_forward(P["spos"], P["src"], P["rpos"])
+        out.setdefault(fmt, []).append(prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"]))
+    (n1, ns1), (n2, ns2) = out["native"]
+    assert np.array_equal(n1, n2) and all(np.array_equal(x, y) for x, y in zip(ns1, ns2))
+    q, qs = out["s16"][0]
+    d8 = rel(q, n1)
+    print(f"int16 derivatives vs exact: g8 rel={d8:.3e}", flush=True)
+    assert 0 < d8 < 5e-5
+    assert all(np.array_equal(x, y) for x, y in zip(qs, ns1))
+
+
+@pytest.mark.parametrize("precision,drv", [("f32", "native"), ("f32", "s16"), ("f64", "native")])
+def test_gradient_kernel_timing_keeps_checkpoints(problem, precision, drv):
+    """ela_time_gradient_kernels times the gradient step's three kernel
+    pairs on the handle (for the launch-shape tuner) between a gradient
+    forward and its backward without disturbing the stored checkpoints: the
+    backward gives the result of an undisturbed run bit for bit."""
+    P = problem[0]
+    prop = make_prop(precision)
+    prop.set_medium(P["med"])
+    prop.set_gradient_checkpoints(4, 0, "native")
+    prop.set_gradient_derivatives(drv)
+
+    def grad(timed):
+        prop.gradient_forward(P["spos"], P["src"], P["rpos"])
+        t = prop.time_gradient_kernels(2) if timed else None
+        g8, gs = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
+        return t, np.array(g8), [np.array(x) for x in gs]
+
+    _, g_ref, s_ref = grad(False)
+    t, g, s = grad(True)
+    print(f"gradient kernel pairs {precision}/{drv}: " + ", ".join(f"{x:.4f} ms" for x in t), flush=True)
+    assert all(x > 0 for x in t)
+    assert np.array_equal(g, g_ref) and all(np.array_equal(x, y) for x, y in zip(s, s_ref))
 
 
 def test_gradient_accumulates_and_requires_forward(problem):
@@ -77,6 +142,7 @@ def test_gradient_device_tensors(problem):
     src = [torch.tensor(s, dtype=torch.float32, device=dev) for s in P["src"]]
     adj = [torch.tensor(s, dtype=torch.float32, device=dev) for s in P["rec"]]
     prop.set_gradient_checkpoints(4, 0, False)
+    prop.set_gradient_derivatives("native")  # exact storage: the bound is FP32 rounding
     prop.gradient_forward(spos, src, rpos)
     g8, gsrc = prop.gradient_backward(spos, src, rpos, adj)
     torch.cuda.synchronize()

@@ -203,6 +203,53 @@ def test_autotuner_picks_fastest_and_caches(tmp_path, monkeypatch):
     assert again["cached"] and q.calls == 0 and q.tiling == (16, 16) and q.order == (3, 2)
 
 
+def test_autotuner_gradient_target(tmp_path, monkeypatch):
+    """With target="gradient" the autotuner ranks shapes by the sum of the
+    gradient step's three kernel pairs, which can prefer another shape than
+    the forward pair, and caches that choice under its own key."""
+    from elasticisolib_3d_amd import autotune as A
+    monkeypatch.setenv("ELA_TUNE_CACHE", str(tmp_path / "t.json"))
+
+    class FakeProp:
+        precision = "f32"
+        shape = (40, 50, 60)
+
+        def __init__(self):
+            self.tiling, self.order = (16, 8), (0, 1)
+            self.grad_calls = 0
+
+        def set_tiling(self, tx, yc):
+            self.tiling = (tx, yc)
+
+        def set_tile_order(self, o, g):
+            self.order = (o, g)
+
+        def gradient_derivatives_used(self):
+            return "native"
+
+        def time_step_kernels(self, reps):   # the forward pair prefers tile 8
+            tx, yc = self.tiling
+            return 0.5 + 0.01 * abs(tx - 8) + 0.01 * abs(yc - 8), 0.5
+
+        def time_gradient_kernels(self, reps):   # the transposed pair prefers tile 4 more strongly
+            self.grad_calls += 1
+            tx, yc = self.tiling
+            fw = sum(self.time_step_kernels(reps))
+            return fw, 1.2 * fw, 2.0 + 0.05 * abs(tx - 4) + 0.01 * abs(yc - 8)
+
+    p = FakeProp()
+    fwd = A.autotune(p, candidates=[(4, 8, 3, 2), (8, 8, 3, 1), (16, 8, 3, 1)])
+    assert (fwd["tile_x"], fwd["target"]) == (8, "forward")
+    grad = A.autotune(p, target="gradient")
+    assert (grad["tile_x"], grad["y_chunk"], grad["target"]) == (4, 8, "gradient") and not grad["cached"]
+    assert grad["ms_step"] == grad["ms_forward"] + grad["ms_recompute"] + grad["ms_transposed"]
+    q = FakeProp()
+    again = A.autotune(q, target="gradient")
+    assert again["cached"] and q.grad_calls == 0 and q.tiling == (4, 8)
+    with pytest.raises(ValueError):
+        A.autotune(q, target="adjoint")
+
+
 def test_positions_follow_the_memory_kind_of_the_call():
     """Position lists are handed to the library in the memory kind of the
     call (host here: torch CPU tensors become numpy arrays), so that a
