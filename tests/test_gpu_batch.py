@@ -173,3 +173,31 @@ def test_persistent_propagator_repeats_and_isolates_batches():
     print(f"shared-grid batches vs own-grid batches: J {J1:.10e} vs {Ja + Jp:.10e}, gradient rel {e:.2e}", flush=True)
     assert abs(J1 - (Ja + Jp)) <= 1e-9 * J1
     assert e < 1e-6
+
+
+def test_batched_int16_derivatives_equal_single_shot():
+    """With the int16 derivative storage (one scale per 32-value z line) the
+    stacked windows give each shot's gradient as its own propagation does:
+    the windows are stacked along y, so every line holds the same cells in
+    both layouts (rows beyond a narrower window are outside every interior
+    and store nothing that is read) and the results differ at most by the
+    order of the FP64 sums over shots."""
+    from elasticisolib_3d_amd.batch import BatchedFwi
+    m = _model()
+    start = [x * (1 + 0.02 * np.random.default_rng(5 + k).standard_normal(x.shape)) for k, x in enumerate(m)]
+    shots = _shots()
+    kw = dict(precision="f32", interval=4, checkpoint="native", derivatives="s16")
+    one = BatchedFwi(m[0].shape, D, 5, NTS, DTS, SUB, 0.95, shots, batch=1, **kw)
+    one.synthesize(m)
+    J1, g1 = one.misfit_and_gradient(start)
+    many = BatchedFwi(m[0].shape, D, 5, NTS, DTS, SUB, 0.95, shots, batch=3, **kw)
+    J3, g3 = many.misfit_and_gradient(start)
+    exact = BatchedFwi(m[0].shape, D, 5, NTS, DTS, SUB, 0.95, shots, batch=3, precision="f32", interval=4,
+                       checkpoint="native")
+    Je, ge = exact.misfit_and_gradient(start)
+    e = float(np.linalg.norm(g3 - ge) / np.linalg.norm(ge))
+    print(f"int16 derivatives: batched vs single-shot J {J1:.10e} {J3:.10e}; vs exact storage rel {e:.2e}",
+          flush=True)
+    d = float(np.linalg.norm(g3 - g1) / np.linalg.norm(g1))
+    assert abs(J1 - J3) <= 1e-12 * J1 and d < 1e-12
+    assert J3 == Je and 0 < e < 5e-5
