@@ -213,7 +213,8 @@ def bench_born(args):
             "adjoint_gcell_updates_per_s": round(ga, 3),
             "adjoint_storage": "all coarse samples in HBM" if seg < 0 else f"exact checkpoints every {seg} coarse samples",
             "dot_product": {"lhs": lhs, "rhs": rhs, "rel_err": dot_rel, "reductions": "fp64"},
-            "launch": None if tuned is None else {k: tuned[k] for k in ("tile_x", "y_chunk", "order", "x_group")},
+            "launch": None if tuned is None else {k: tuned[k] for k in ("tile_x", "y_chunk", "order", "x_group",
+                                                                          "tile_x_transposed") if k in tuned},
         },
     }
     print(json.dumps(out), flush=True)
@@ -420,7 +421,8 @@ def bench_fwi(args):
             "derivatives": drv_fmt,
             "checkpoint_plan": {"interval": plan[0], "outer_interval": plan[1], "format": plan[2],
                                 "checkpoint_GB_per_shot": round(plan[3] / 1e9, 2)},
-            "launch": None if tuned is None else {k: tuned[k] for k in ("tile_x", "y_chunk", "order", "x_group")},
+            "launch": None if tuned is None else {k: tuned[k] for k in ("tile_x", "y_chunk", "order", "x_group",
+                                                                          "tile_x_transposed") if k in tuned},
             "interior_cells": w.interior_cells,
             "fine_steps_per_propagation": steps,
             "propagation_gcell_updates_per_s": round(3 * cells * steps * len(shots) / t_plain / 1e9, 2),

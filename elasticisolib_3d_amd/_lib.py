@@ -29,7 +29,7 @@ EXPORTS = (
     "ela_gradient_forward", "ela_gradient_backward", "ela_set_gradient_checkpoints", "ela_gradient_plan_used",
     "ela_set_windows", "ela_set_graph", "ela_stream_probe", "ela_set_elastic", "ela_set_born_mode",
     "ela_nl_forward_noalloc", "ela_get_medium", "ela_set_gradient_derivatives", "ela_gradient_derivatives_used",
-    "ela_time_gradient_kernels",
+    "ela_time_gradient_kernels", "ela_set_tiling_transposed", "ela_get_tiling_transposed",
 )
 
 
@@ -127,6 +127,8 @@ def load() -> ctypes.CDLL:
     lib.ela_set_gradient_derivatives.argtypes = [vp, i32]
     lib.ela_gradient_derivatives_used.argtypes = [vp, ctypes.POINTER(i32)]
     lib.ela_time_gradient_kernels.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double)]
+    lib.ela_set_tiling_transposed.argtypes = [vp, i32]
+    lib.ela_get_tiling_transposed.argtypes = [vp, ctypes.POINTER(i32)]
     lib.ela_gradient_plan_used.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32),
                                            ctypes.POINTER(i64)]
     lib.ela_born_storage_used.argtypes = [vp, ctypes.POINTER(i64)]

@@ -78,6 +78,8 @@ def _store(table: Dict[str, dict]) -> None:
 def apply(prop, best: dict) -> None:
     prop.set_tiling(best["tile_x"], best["y_chunk"])
     prop.set_tile_order(best["order"], best["x_group"])
+    if hasattr(prop, "set_tiling_transposed"):
+        prop.set_tiling_transposed(best.get("tile_x_transposed", 0))
 
 
 def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = None, use_cache: bool = True,
@@ -89,7 +91,10 @@ def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = N
     launches); "gradient" by the three kernel pairs of an adjoint-state
     gradient step (forward, recomputation with derivative store, transposed
     step with imaging: ela_time_gradient_kernels), which do not share the
-    forward's best shape."""
+    forward's best shape: for every y chunk and block order it takes the
+    x tile that is fastest for the forward and recomputing pairs and, for
+    the transposed pair (ela_set_tiling_transposed), the x tile fastest for
+    that pair, from the same measurements."""
     if target not in ("forward", "gradient"):
         raise ValueError("target must be 'forward' or 'gradient'")
     key = _key(prop, target)
@@ -99,6 +104,8 @@ def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = N
             apply(prop, hit)
             return dict(hit, cached=True)
     cands = list(candidates) if candidates is not None else list(default_candidates(prop.precision))
+    if hasattr(prop, "set_tiling_transposed"):
+        prop.set_tiling_transposed(0)   # candidates are timed with one tile for every step
     prop.set_tiling(*cands[0][:2])
     prop.time_step_kernels(2)  # warm the code objects
     table = []
@@ -117,7 +124,19 @@ def autotune(prop, reps: int = 10, candidates: Optional[Iterable[Candidate]] = N
     t, tx, yc, order, xg = table[0][0], *table[0][-4:]
     best = {"tile_x": tx, "y_chunk": yc, "order": order, "x_group": xg, "ms_step": t, "target": target}
     if target == "gradient":
-        best.update(ms_forward=table[0][1], ms_recompute=table[0][2], ms_transposed=table[0][3])
+        best.update(ms_forward=table[0][1], ms_recompute=table[0][2], ms_transposed=table[0][3],
+                    tile_x_transposed=tx)
+        # per (y chunk, order): the best tile of the forward + recomputing pairs and of the transposed pair
+        groups = {}
+        for row in table:
+            groups.setdefault(row[-3:], []).append(row)
+        for (yc_, o_, g_), rows in groups.items():
+            f = min(rows, key=lambda r: r[1] + r[2])
+            tr = min(rows, key=lambda r: r[3])
+            split = f[1] + f[2] + tr[3]
+            if split < best["ms_step"]:
+                best.update(tile_x=f[4], y_chunk=yc_, order=o_, x_group=g_, ms_step=split, ms_forward=f[1],
+                            ms_recompute=f[2], ms_transposed=tr[3], tile_x_transposed=tr[4])
     else:
         best.update(ms_velocity=table[0][1], ms_stress=table[0][2])
     apply(prop, best)

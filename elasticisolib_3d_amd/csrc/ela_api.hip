@@ -196,6 +196,7 @@ struct ela_handle {
   std::vector<double> filt; // 2*(sub+1)
   int64_t last_steps = 0;
   int by_x = 16;            // x extent of a tile
+  int by_x_t = 0;           // x extent of the premultiplied transposed steps' tiles (0: by_x)
   int ychunk = 8;           // y planes per block
   int remap = 0;            // tile order (ela_set_tile_order)
   int xgroup = 1;           // x columns per group of the y-major order
@@ -446,7 +447,9 @@ int launch_step_bx(ela_handle* h, ela::StepArgs<T> a, int which) {
 
 template <typename T, int MODE, int FV, int FS>
 int launch_step(ela_handle* h, const ela::StepArgs<T>& a, int which = 3) {
-  switch (h->by_x) {
+  // the premultiplied transposed steps (Born adjoint, gradient) may take a tile of their own
+  const int bx = MODE == ela::MODE_BORN_ADJ_PM && h->by_x_t ? h->by_x_t : h->by_x;
+  switch (bx) {
     case 4: return launch_step_bx<T, 4, MODE, FV, FS>(h, a, which);
     case 8: return launch_step_bx<T, 8, MODE, FV, FS>(h, a, which);
     default: return launch_step_bx<T, 16, MODE, FV, FS>(h, a, which);
@@ -2210,6 +2213,24 @@ int ela_set_tiling(ela_handle* h, int tile_x, int y_chunk) {
     if (y_chunk < 1) return fail(ELA_ERR_ARG, "y_chunk must be >= 1");
     h->by_x = tile_x;
     h->ychunk = y_chunk;
+    return ELA_OK;
+  });
+}
+
+int ela_set_tiling_transposed(ela_handle* h, int tile_x) {
+  return guarded([&]() -> int {
+    if (!h) return fail(ELA_ERR_ARG, "null handle");
+    if (tile_x != 0 && tile_x != 4 && tile_x != 8 && tile_x != 16)
+      return fail(ELA_ERR_ARG, "tile_x must be 0 (the forward tile), 4, 8 or 16");
+    h->by_x_t = tile_x;
+    return ELA_OK;
+  });
+}
+
+int ela_get_tiling_transposed(const ela_handle* h, int* tile_x) {
+  return guarded([&]() -> int {
+    if (!h || !tile_x) return fail(ELA_ERR_ARG, "null argument");
+    *tile_x = h->by_x_t ? h->by_x_t : h->by_x;
     return ELA_OK;
   });
 }

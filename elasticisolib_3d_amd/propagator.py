@@ -363,6 +363,16 @@ class Propagator:
         check(self.lib.ela_get_tiling(self.h, ctypes.byref(tx), ctypes.byref(yc)), "ela_get_tiling")
         return tx.value, yc.value
 
+    def set_tiling_transposed(self, tile_x: int = 0) -> None:
+        """x tile of the premultiplied transposed steps (Born adjoint,
+        gradient backward); 0 uses the forward tile (speed only)."""
+        check(self.lib.ela_set_tiling_transposed(self.h, int(tile_x)), "ela_set_tiling_transposed")
+
+    def get_tiling_transposed(self) -> int:
+        tx = ctypes.c_int()
+        check(self.lib.ela_get_tiling_transposed(self.h, ctypes.byref(tx)), "ela_get_tiling_transposed")
+        return tx.value
+
     def time_step_kernels(self, reps: int = 50):
         """Mean launch duration [ms] of the velocity and stress stencil kernels
         (HIP events on the handle's stream; overwrites the wavefields)."""

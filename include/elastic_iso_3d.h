@@ -405,6 +405,14 @@ int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* 
  * stored checkpoints are kept. For the launch-shape tuner of the gradient.
  * No counterpart in the reference. */
 int ela_time_gradient_kernels(ela_handle* h, int reps, double* ms3);
+/* x extent (4, 8 or 16) of the tiles of the premultiplied transposed steps
+ * (Born adjoint and gradient backward), which prefer another tile than the
+ * forward steps on some grids; 0 (the default) uses the forward tile of
+ * ela_set_tiling. The y chunk and block order are shared; results do not
+ * depend on it. The getter returns the tile in use. No counterpart in the
+ * reference. */
+int ela_set_tiling_transposed(ela_handle* h, int tile_x);
+int ela_get_tiling_transposed(const ela_handle* h, int* tile_x);
 
 /* Diagnostic streaming reference for the roofline: a plain kernel that reads
  * `reads` arrays of n FP32 values (16-byte accesses) and rewrites the first

@@ -117,6 +117,30 @@ def test_gradient_kernel_timing_keeps_checkpoints(problem, precision, drv):
     assert np.array_equal(g, g_ref) and all(np.array_equal(x, y) for x, y in zip(s, s_ref))
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_transposed_tile_does_not_change_results(problem, precision):
+    """The premultiplied transposed steps (gradient backward, Born adjoint)
+    may run with an x tile of their own (ela_set_tiling_transposed); the
+    gradient, the source gradient and the Born image stay bit-identical."""
+    P = problem[0]
+    prop = make_prop(precision)
+    prop.set_medium(P["med"])
+    prop.set_gradient_checkpoints(4, 0, "native")
+    prop.set_tiling(8, 8)
+    out = []
+    for tt in (0, 4, 16):
+        prop.set_tiling_transposed(tt)
+        assert prop.get_tiling_transposed() == (tt or 8)
+        prop.gradient_forward(P["spos"], P["src"], P["rpos"])
+        g8, gs = prop.gradient_backward(P["spos"], P["src"], P["rpos"], P["rec"])
+        img = prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])
+        out.append((np.array(g8), [np.array(x) for x in gs], np.array(img)))
+    (ga, sa, ia) = out[0]
+    for gb, sb, ib in out[1:]:
+        assert np.array_equal(ga, gb) and np.array_equal(ia, ib)
+        assert all(np.array_equal(x, y) for x, y in zip(sa, sb))
+
+
 def test_gradient_accumulates_and_requires_forward(problem):
     from elasticisolib_3d_amd._lib import ElaError
     P, g8_ref, _ = problem

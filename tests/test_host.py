@@ -206,7 +206,9 @@ def test_autotuner_picks_fastest_and_caches(tmp_path, monkeypatch):
 def test_autotuner_gradient_target(tmp_path, monkeypatch):
     """With target="gradient" the autotuner ranks shapes by the sum of the
     gradient step's three kernel pairs, which can prefer another shape than
-    the forward pair, and caches that choice under its own key."""
+    the forward pair: the forward and recomputing pairs take the x tile best
+    for them, the transposed pair its own (ela_set_tiling_transposed), all
+    from the same measurements, and the choice is cached under its own key."""
     from elasticisolib_3d_amd import autotune as A
     monkeypatch.setenv("ELA_TUNE_CACHE", str(tmp_path / "t.json"))
 
@@ -217,6 +219,10 @@ def test_autotuner_gradient_target(tmp_path, monkeypatch):
         def __init__(self):
             self.tiling, self.order = (16, 8), (0, 1)
             self.grad_calls = 0
+            self.tile_t = 0
+
+        def set_tiling_transposed(self, tx):
+            self.tile_t = tx
 
         def set_tiling(self, tx, yc):
             self.tiling = (tx, yc)
@@ -239,13 +245,15 @@ def test_autotuner_gradient_target(tmp_path, monkeypatch):
 
     p = FakeProp()
     fwd = A.autotune(p, candidates=[(4, 8, 3, 2), (8, 8, 3, 1), (16, 8, 3, 1)])
-    assert (fwd["tile_x"], fwd["target"]) == (8, "forward")
+    assert (fwd["tile_x"], fwd["target"]) == (8, "forward") and p.tile_t == 0
     grad = A.autotune(p, target="gradient")
-    assert (grad["tile_x"], grad["y_chunk"], grad["target"]) == (4, 8, "gradient") and not grad["cached"]
+    assert (grad["tile_x"], grad["tile_x_transposed"], grad["y_chunk"]) == (8, 4, 8) and not grad["cached"]
     assert grad["ms_step"] == grad["ms_forward"] + grad["ms_recompute"] + grad["ms_transposed"]
+    assert abs(grad["ms_step"] - (1.0 + 1.2 + 2.0)) < 1e-12    # split beats every common tile (4.288 at tile 4)
+    assert p.tiling == (8, 8) and p.tile_t == 4
     q = FakeProp()
     again = A.autotune(q, target="gradient")
-    assert again["cached"] and q.grad_calls == 0 and q.tiling == (4, 8)
+    assert again["cached"] and q.grad_calls == 0 and q.tiling == (8, 8) and q.tile_t == 4
     with pytest.raises(ValueError):
         A.autotune(q, target="adjoint")
 
