@@ -156,8 +156,9 @@ def bench_born(args):
     tuned = None
     if not args.no_autotune:
         from elasticisolib_3d_amd.autotune import autotune
-        tuned = autotune(prop, reps=5)
-        log(f"[bench] autotune: tile {tuned['tile_x']}x{tuned['y_chunk']} order {tuned['order']}/{tuned['x_group']}")
+        tuned = autotune(prop, reps=5, target=args.tune_target)
+        log(f"[bench] autotune ({args.tune_target}): tile {tuned['tile_x']}x{tuned['y_chunk']} order "
+            f"{tuned['order']}/{tuned['x_group']}, transposed tile {prop.get_tiling_transposed()}")
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -600,6 +601,9 @@ def main():
     ap.add_argument("--launch", default="", help="fixed launch shape tile_x x y_chunk / order / x_group, "
                                                  "e.g. 4x8/3/4 (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
+    ap.add_argument("--tune-target", choices=("forward", "gradient"), default="forward",
+                    help="born: rank launch shapes by the forward step or by the gradient step's kernel pairs "
+                         "(which also picks a tile of their own for the transposed steps)")
     ap.add_argument("--medium", choices=("elastic", "scaled"), default="elastic",
                     help="nl_forward, born, fwi, fp64_check: rho, lambda, mu with the medium formed in the steppers (120 B per cell-update), "
                          "or the 8 scaled arrays of the reference's interface (140 B)")

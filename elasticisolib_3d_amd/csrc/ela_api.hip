@@ -1782,6 +1782,9 @@ template <typename T>
 int time_gradient_kernels(ela_handle* h, int reps, double* ms3) {
   const int64_t np = h->stride;
   const int dfmt = drv_format(h);
+  // buffers that no gradient or Born call holds yet are released again afterwards, so that
+  // timing does not shrink the memory the next call's storage plan sees
+  const bool had_w2 = h->wave2.p != nullptr, had_drv = h->drv.p != nullptr, had_g32 = h->g32.p != nullptr;
   CHK(h->wave2.alloc((size_t)36 * np * sizeof(T)));
   CHK(h->drv.alloc(drv_step_bytes(h, dfmt)));
   CHK(h->g32.alloc((size_t)8 * np * sizeof(T)));
@@ -1819,6 +1822,10 @@ int time_gradient_kernels(ela_handle* h, int reps, double* ms3) {
     ms3[k] = t / reps;
   }
   HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (!had_w2) h->wave2.release();
+  if (!had_drv) h->drv.release();
+  if (!had_g32) h->g32.release();
   return ELA_OK;
 }
 
