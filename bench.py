@@ -601,9 +601,10 @@ def main():
     ap.add_argument("--launch", default="", help="fixed launch shape tile_x x y_chunk / order / x_group, "
                                                  "e.g. 4x8/3/4 (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="launch every fine step from the host")
-    ap.add_argument("--tune-target", choices=("forward", "gradient"), default="forward",
-                    help="born: rank launch shapes by the forward step or by the gradient step's kernel pairs "
-                         "(which also picks a tile of their own for the transposed steps)")
+    ap.add_argument("--tune-target", choices=("forward", "gradient"), default="gradient",
+                    help="born: rank launch shapes by the nonlinear forward step, or (default) by the gradient "
+                         "step's kernel pairs -- forward, forward with extra per-cell streams, transposed -- "
+                         "the kernel mix of the Born passes (also picks the transposed steps' own tile)")
     ap.add_argument("--medium", choices=("elastic", "scaled"), default="elastic",
                     help="nl_forward, born, fwi, fp64_check: rho, lambda, mu with the medium formed in the steppers (120 B per cell-update), "
                          "or the 8 scaled arrays of the reference's interface (140 B)")

@@ -1012,9 +1012,10 @@ int born_plan(ela_handle* h, BornPlan& p) {
   } else {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
+    // the gradient's step buffers count as free: the Born adjoint releases them before it allocates
     budget = double(fr) + double(h->coarse.bytes) + double(h->ckpt.bytes) + double(h->wave2.bytes) +
              double(h->ss.bytes) + double(h->dmod.bytes) + double(h->dmod64.bytes) + double(h->wave.bytes) +
-             double(h->med.bytes) -
+             double(h->med.bytes) + double(h->drv.bytes) + double(h->g32.bytes) + double(h->g64.bytes) -
              double(2ull << 30);  // 2 GiB margin for traces and the runtime
   }
   const double avail = budget / arr_b - double(born_fixed_arrays(h));
@@ -1084,11 +1085,17 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
   BornPlan plan;
   CHK(born_plan(h, plan));
   h->born_last_segment = plan.full ? -1 : plan.K;
-  // release what the new plan does not reuse before allocating (peak memory)
+  // release what the new plan does not reuse before allocating (peak memory); the gradient's
+  // step buffers, which the plan counted as free, are allocated again by the next gradient
   const size_t ck_bytes = plan.full ? 0 : (size_t)plan.S * 21 * np * sizeof(T);
   const size_t v_bytes = (size_t)plan.slots * 3 * np * sizeof(T);
   if (h->ckpt.bytes != ck_bytes) h->ckpt.release();
   if (h->coarse.bytes != v_bytes) h->coarse.release();
+  if (h->born_segment == 0 && h->born_budget <= 0) {  // the automatic plan counted them as free
+    h->drv.release();
+    h->g32.release();
+    h->g64.release();
+  }
   CHK(h->ckpt.alloc(ck_bytes));
   CHK(h->coarse.alloc(v_bytes));
   CHK(h->wave2.alloc(36 * np * sizeof(T)));
@@ -1376,6 +1383,17 @@ int grad_plan(ela_handle* h, GradPlan& p) {
   return ELA_OK;
 }
 
+// Multiplier and shifts of ela::fdiv for divisor d >= 1.
+ela::FastDiv fast_div(uint32_t d) {
+  int l = 0;
+  while (l < 32 && ((uint64_t)1 << l) < d) ++l;
+  ela::FastDiv f;
+  f.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+  f.s1 = l < 1 ? l : 1;
+  f.s2 = l > 1 ? l - 1 : 0;
+  return f;
+}
+
 template <typename T>
 ela::CkArgs<T> ck_args(ela_handle* h, const Wave<T>& w) {
   ela::CkArgs<T> a;
@@ -1387,6 +1405,8 @@ ela::CkArgs<T> ck_args(ela_handle* h, const Wave<T>& w) {
   a.zoff = (int)h->zoff;
   a.ys = h->cfg.nx * h->nzi;
   a.nI = interior_count(h);
+  a.dz = fast_div((uint32_t)a.nzI);
+  a.dx = fast_div((uint32_t)a.nxI);
   return a;
 }
 
@@ -1776,15 +1796,13 @@ int time_kernels(ela_handle* h, int reps, double* ms_v, double* ms_s) {
 // handle's grid and launch shape: the forward step, the recomputing step
 // with derivative store (in the handle's derivative format) and the
 // transposed step with imaging. Runs on the handle's wavefields and the
-// gradient's own buffers (sized here for at least one step, zero-filled),
+// gradient's own buffers (sized here for at least one step, zero-filled, and
+// kept, as a gradient keeps them; the storage plans count them as free),
 // whose contents do not matter for timing; stored checkpoints are untouched.
 template <typename T>
 int time_gradient_kernels(ela_handle* h, int reps, double* ms3) {
   const int64_t np = h->stride;
   const int dfmt = drv_format(h);
-  // buffers that no gradient or Born call holds yet are released again afterwards, so that
-  // timing does not shrink the memory the next call's storage plan sees
-  const bool had_w2 = h->wave2.p != nullptr, had_drv = h->drv.p != nullptr, had_g32 = h->g32.p != nullptr;
   CHK(h->wave2.alloc((size_t)36 * np * sizeof(T)));
   CHK(h->drv.alloc(drv_step_bytes(h, dfmt)));
   CHK(h->g32.alloc((size_t)8 * np * sizeof(T)));
@@ -1822,10 +1840,6 @@ int time_gradient_kernels(ela_handle* h, int reps, double* ms3) {
     ms3[k] = t / reps;
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(h->stream));
-  if (!had_w2) h->wave2.release();
-  if (!had_drv) h->drv.release();
-  if (!had_g32) h->g32.release();
   return ELA_OK;
 }
 

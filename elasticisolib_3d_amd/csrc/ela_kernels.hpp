@@ -1308,6 +1308,23 @@ __global__ void __launch_bounds__(256) k_fill(T* p, int64_t n, T v) {
 // 2^e >= max |x| (the scaling is exact; bfloat16 keeps 8 significant bits,
 // rounded to nearest even).
 // ---------------------------------------------------------------------------
+// Unsigned 32-bit division by an invariant divisor d with a multiply-high,
+// a subtraction and two shifts (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, section 4): with
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(m, n),
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0) for every 32-bit n.
+// The host forms (m, l) once per grid (fast_div); the device avoids the
+// 30-odd instructions of a general 32-bit division per element.
+struct FastDiv {
+  uint32_t m;
+  int s1, s2;
+};
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+  const uint32_t t = __umulhi(f.m, n);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 template <typename T>
 struct CkArgs {
   T* f[18];
@@ -1315,16 +1332,16 @@ struct CkArgs {
   int nzi, zoff;         // internal row pitch and z origin
   int64_t ys;            // nx * nzi
   int64_t nI;            // nzI * nxI * nyI
+  FastDiv dz, dx;        // division by nzI and nxI
 };
 
-// Library offset of packed interior index q. 32-bit division: nI and every
-// offset fit in 32 bits (arrays < 4 GiB), and a 64-bit division costs the
-// pack / unpack kernels more than their memory traffic.
+// Library offset of packed interior index q (32-bit: nI and every offset fit
+// in 32 bits, the arrays being smaller than 4 GiB).
 template <typename T>
 __device__ __forceinline__ int64_t ck_offset(const CkArgs<T>& a, int64_t q) {
   const uint32_t q32 = (uint32_t)q, nz = (uint32_t)a.nzI, nx = (uint32_t)a.nxI;
-  const uint32_t r = q32 / nz, iz = q32 - r * nz;
-  const uint32_t iy = r / nx, ix = r - iy * nx;
+  const uint32_t r = fdiv(q32, a.dz), iz = q32 - r * nz;
+  const uint32_t iy = fdiv(r, a.dx), ix = r - iy * nx;
   return (int64_t)((iy + FAT) * (uint32_t)a.ys + (ix + FAT) * (uint32_t)a.nzi + (uint32_t)(a.zoff + FAT) + iz);
 }
 
@@ -1501,14 +1518,15 @@ __global__ void __launch_bounds__(256) k_ck_pack_s16(CkArgs<T> a, int16_t* out) 
   float* sc = reinterpret_cast<float*>(out + 18 * a.nI) + f * nb;
   const int lane = threadIdx.x & (CK_BLOCK - 1);
   const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / CK_BLOCK);
-  // b is uniform over a wave (blockDim.x is a multiple of 64), so every lane joins the reduction
+  // b is uniform over a wave (blockDim.x is a multiple of 64), so every lane joins the reduction:
+  // the maximum of each 32-lane half (DPP row moves and a swizzle), then of the two halves
   for (int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CK_BLOCK; b < nb; b += wstride) {
     const int64_t q = b * CK_BLOCK + lane;
     const bool ok = q < a.nI;
     const float x = ok ? (float)a.f[f][ck_offset(a, q)] : 0.f;
-    float m = fabsf(x);
-#pragma unroll
-    for (int o = CK_BLOCK / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, CK_BLOCK));
+    const float h = half_wave_absmax(x);
+    const float m = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(h), 0)),
+                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h), 32)));
     const float inv = m > 0.f ? 32767.f / m : 0.f;
     if (ok) q16[q] = (int16_t)__float2int_rn(x * inv);
     if (lane == 0) sc[b] = m / 32767.f;

@@ -401,10 +401,11 @@ int ela_time_step_kernels(ela_handle* h, int reps, double* ms_velocity, double* 
  * step on the handle's grid and launch shape, over `reps` launches each:
  * ms3[0] the forward step, ms3[1] the recomputing step with derivative store
  * (in the handle's derivative format), ms3[2] the transposed step with
- * imaging. Overwrites the wavefields; the gradient's step buffers it needs
- * and the handle does not hold yet are released again; stored checkpoints
- * are kept. For the launch-shape tuner of the gradient. No counterpart in
- * the reference. */
+ * imaging. Overwrites the wavefields and allocates the gradient's step
+ * buffers (kept, as after a gradient; the Born adjoint's automatic storage
+ * plan counts them as free and releases them); stored checkpoints are kept.
+ * For the launch-shape tuner of the gradient. No counterpart in the
+ * reference. */
 int ela_time_gradient_kernels(ela_handle* h, int reps, double* ms3);
 /* x extent (4, 8 or 16) of the tiles of the premultiplied transposed steps
  * (Born adjoint and gradient backward), which prefer another tile than the
