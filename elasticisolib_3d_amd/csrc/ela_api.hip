@@ -1412,11 +1412,25 @@ ela::CkArgs<T> ck_args(ela_handle* h, const Wave<T>& w) {
 
 dim3 ck_grid(int64_t nI) { return dim3((unsigned)std::min<int64_t>((nI + 255) / 256, 4096), 18); }
 
+// The vectorised int16 checkpoint kernels: FP32, interior z extent a multiple
+// of 8, checkpoint 16-byte aligned (the int16 arrays of all 18 fields then are).
+bool ck_vec8(const ela_handle* h, const void* ck) {
+  return h->esize == 4 && (h->cfg.nz - 2 * ela::FAT) % 8 == 0 && reinterpret_cast<uintptr_t>(ck) % 16 == 0;
+}
+
+dim3 ck_grid8(int64_t nI) { return dim3((unsigned)std::min<int64_t>((nI / 8 + 255) / 256, 4096), 18); }
+
 template <typename T>
 int save_checkpoint(ela_handle* h, const Wave<T>& w, void* out, int fmt, float* scale) {
   ela::CkArgs<T> a = ck_args<T>(h, w);
   const dim3 grid = ck_grid(a.nI);
   if (fmt == ELA_CK_S16) {
+    if constexpr (sizeof(T) == 4) {
+      if (ck_vec8(h, out)) {
+        launch(h, ela::k_ck_pack_s16_v8, ck_grid8(a.nI), dim3(256), a, static_cast<int16_t*>(out));
+        return ELA_OK;
+      }
+    }
     launch(h, ela::k_ck_pack_s16<T>, grid, dim3(256), a, static_cast<int16_t*>(out));
     return ELA_OK;
   }
@@ -1434,6 +1448,12 @@ int save_checkpoint(ela_handle* h, const Wave<T>& w, void* out, int fmt, float* 
 template <typename T>
 int load_checkpoint(ela_handle* h, const Wave<T>& w, const void* in, int fmt, const float* scale) {
   ela::CkArgs<T> a = ck_args<T>(h, w);
+  if constexpr (sizeof(T) == 4) {
+    if (fmt == ELA_CK_S16 && ck_vec8(h, in)) {
+      launch(h, ela::k_ck_unpack_s16_v8, ck_grid8(a.nI), dim3(256), a, static_cast<const int16_t*>(in));
+      return ELA_OK;
+    }
+  }
   if (fmt == ELA_CK_S16)
     launch(h, ela::k_ck_unpack_s16<T>, ck_grid(a.nI), dim3(256), a,
                        static_cast<const int16_t*>(in));

@@ -1543,6 +1543,66 @@ __global__ void __launch_bounds__(256) k_ck_unpack_s16(CkArgs<T> a, const int16_
     a.f[f][ck_offset(a, q)] = (T)((float)q16[q] * sc[q / CK_BLOCK]);
 }
 
+// Vectorised k_ck_pack_s16 / k_ck_unpack_s16 for FP32 grids whose interior
+// z extent is a multiple of 8 (so are nI and every row start): a lane moves 8
+// consecutive values -- 32 bytes of one z row (two 16-byte loads or stores)
+// and 16 bytes of the packed int16 -- instead of one 4-byte and one 2-byte
+// access, which kept too few bytes in flight per lane to reach the HBM rate.
+// The 8 lanes of a 64-value block form its maximum with DPP moves inside
+// their 8-lane group. Same blocks, scales and values as the scalar kernels.
+__global__ void __launch_bounds__(256) k_ck_pack_s16_v8(CkArgs<float> a, int16_t* out) {
+  const int f = blockIdx.y;
+  const int64_t nb = (a.nI + CK_BLOCK - 1) / CK_BLOCK, n8 = a.nI / 8;
+  int16_t* q16 = out + f * a.nI;
+  float* sc = reinterpret_cast<float*>(out + 18 * a.nI) + f * nb;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // g0 is uniform over the block, so every lane joins the reductions (lanes past the end hold zeros)
+  for (int64_t g0 = (int64_t)blockIdx.x * blockDim.x; g0 < n8; g0 += stride) {
+    const int64_t g = g0 + threadIdx.x;
+    const bool ok = g < n8;
+    float4 u = make_float4(0.f, 0.f, 0.f, 0.f), v = u;
+    if (ok) {
+      const float4* p = reinterpret_cast<const float4*>(a.f[f] + ck_offset(a, g * 8));
+      u = p[0];
+      v = p[1];
+    }
+    const float x[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
+    uint32_t mu = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mu = max(mu, __float_as_uint(fabsf(x[k])));
+    mu = dpp_umax<0xB1>(mu);   // quad_perm [1, 0, 3, 2]
+    mu = dpp_umax<0x4E>(mu);   // quad_perm [2, 3, 0, 1]
+    mu = dpp_umax<0x141>(mu);  // row_half_mirror: the 8-lane group's maximum
+    const float m = __uint_as_float(mu);
+    const float inv = m > 0.f ? 32767.f / m : 0.f;
+    if (ok) {
+      uint32_t w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        w[k] = (uint32_t)(uint16_t)(int16_t)__float2int_rn(x[2 * k] * inv) |
+               ((uint32_t)(uint16_t)(int16_t)__float2int_rn(x[2 * k + 1] * inv) << 16);
+      *reinterpret_cast<uint4*>(q16 + g * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+      if ((threadIdx.x & 7) == 0) sc[g / 8] = m / 32767.f;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ck_unpack_s16_v8(CkArgs<float> a, const int16_t* in) {
+  const int f = blockIdx.y;
+  const int64_t nb = (a.nI + CK_BLOCK - 1) / CK_BLOCK, n8 = a.nI / 8;
+  const int16_t* q16 = in + f * a.nI;
+  const float* sc = reinterpret_cast<const float*>(in + 18 * a.nI) + f * nb;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n8; g += (int64_t)gridDim.x * blockDim.x) {
+    const uint4 w = *reinterpret_cast<const uint4*>(q16 + g * 8);
+    const float s = sc[g / 8];
+    auto lo = [&](uint32_t b) { return (float)(int16_t)(uint16_t)(b & 0xFFFFu) * s; };
+    auto hi = [&](uint32_t b) { return (float)(int16_t)(uint16_t)(b >> 16) * s; };
+    float4* p = reinterpret_cast<float4*>(a.f[f] + ck_offset(a, g * 8));
+    p[0] = make_float4(lo(w.x), hi(w.x), lo(w.y), hi(w.y));
+    p[1] = make_float4(lo(w.z), hi(w.z), lo(w.w), hi(w.w));
+  }
+}
+
 // out[i] = (T) in[i] (the Born adjoint's FP64 image handed back in T).
 template <typename T>
 __global__ void __launch_bounds__(256) k_narrow(const double* in, T* out, int64_t n) {
