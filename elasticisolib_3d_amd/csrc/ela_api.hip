@@ -1383,8 +1383,10 @@ int grad_plan(ela_handle* h, GradPlan& p) {
   return ELA_OK;
 }
 
-// Multiplier and shifts of ela::fdiv for divisor d >= 1.
+// Multiplier and shifts of ela::fdiv for divisor d >= 1 (d = 0, an empty
+// interior, gives a divisor nothing is divided by).
 ela::FastDiv fast_div(uint32_t d) {
+  if (d == 0) return ela::FastDiv{0, 0, 0};
   int l = 0;
   while (l < 32 && ((uint64_t)1 << l) < d) ++l;
   ela::FastDiv f;
