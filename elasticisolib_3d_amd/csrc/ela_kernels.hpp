@@ -1603,6 +1603,30 @@ __global__ void __launch_bounds__(256) k_ck_unpack_s16_v8(CkArgs<float> a, const
   }
 }
 
+// The gradient's FP32 partial sums folded into its FP64 sums over the FD
+// interior only (the steppers and injections write nothing else): four
+// consecutive z values per lane (interior z extent a multiple of 4, rows
+// 16-byte aligned), array m = blockIdx.y of a.f (partial sums) and of acc
+// (FP64 sums, stride np); the partial sums are cleared.
+__global__ void __launch_bounds__(256) k_fold_v4(CkArgs<float> a, double* acc, int64_t np) {
+  const int m = blockIdx.y;
+  float* part = a.f[m];
+  double* ac = acc + m * np;
+  const int64_t n4 = a.nI / 4;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n4; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = ck_offset(a, g * 4);
+    float4* pp = reinterpret_cast<float4*>(part + o);
+    double2* d = reinterpret_cast<double2*>(ac + o);
+    const float4 p = *pp;
+    double2 d0 = d[0], d1 = d[1];
+    d0.x += (double)p.x; d0.y += (double)p.y;
+    d1.x += (double)p.z; d1.y += (double)p.w;
+    d[0] = d0;
+    d[1] = d1;
+    *pp = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
 // out[i] = (T) in[i] (the Born adjoint's FP64 image handed back in T).
 template <typename T>
 __global__ void __launch_bounds__(256) k_narrow(const double* in, T* out, int64_t n) {
