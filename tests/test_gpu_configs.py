@@ -6,8 +6,11 @@ C1  100^3 homogeneous model, point force, the full 0.6 s record (151 coarse
 C2  400x400x250 layered model, explosive source, the full 2.5 s record (5000
     fine steps, 200x200 receivers, 9 components): FP32 against FP64 on the GPU
     (the bar for FP32 forward data is 1e-5).
-C3  the Born pair at this size is covered by tests/test_gpu_fullsize.py (FP64
-    dot test on the padded 328x472x468 grid).
+C3  the Born pair on the 400x400x250 model smoothed with a 15-cell Gaussian,
+    1 % sparse perturbations, 150x150 receivers, record shortened to 0.5 s
+    (126 coarse samples, 1000 fine steps): FP32 dot test of the shot-level
+    operator (perturbation staggering + propagation) with FP64 inner
+    products, bar 1e-6 (the FP64 pair at this size: tests/test_gpu_fullsize.py).
 C4  256x256x160 FWI gradient, record shortened to 0.5 s (126 coarse samples,
     1000 fine steps): FP32 with the default int16 checkpoints against the FP64
     gradient with exact checkpoints (bar 1e-3), and the FP64 gradient against
@@ -103,6 +106,43 @@ def test_c2_full_record_fp32_against_fp64():
     print(f"C2 FP32 vs FP64: rel L2 {e:.2e}, per component {np.round(per, 8)}", flush=True)
     assert e < 1e-5
     assert all(np.isfinite(per))
+
+
+# ---- C3 -------------------------------------------------------------------
+
+def test_c3_born_dot_test_fp32_shortened_record():
+    import torch
+    from elasticisolib_3d_amd import propagator as P
+    from elasticisolib_3d_amd.acquisition import ShotGeometry, scale_regular_traces
+    from elasticisolib_3d_amd.models import born_layered_400, source_traces
+    w = born_layered_400(nts=126)
+    a = w.axes
+    med = P.scale_medium(w.rho, w.lame, w.mu, w.dtw)
+    sg = ShotGeometry(a, w.src_xyz[:, 2], w.src_xyz[:, 0], w.src_xyz[:, 1])
+    rg = ShotGeometry(a, w.rec_xyz[:, 2], w.rec_xyz[:, 0], w.rec_xyz[:, 1])
+    src = scale_regular_traces(med, sg, sg.to_regular(source_traces(w)), a.dx * a.dy * a.dz)
+    dm3 = w.extra["dm3"]
+    dm8 = P.born_stagger(dm3, med, w.dtw)
+    print(f"C3 FP32: {a.nz}x{a.nx}x{a.ny}, {w.fine_steps} fine steps, {len(rg.positions()[1])} vx points", flush=True)
+    prop = P.Propagator(a.nz, a.nx, a.ny, w.nts, w.sub, a.dz, a.dx, a.dy, w.dts, a.min_pad, w.alpha_cos, "f32")
+    prop.set_elastic(torch.tensor(np.stack([w.rho, w.lame, w.mu]), dtype=torch.float32, device="cuda"), w.dtw)
+    spos = [torch.tensor(p, device="cuda") for p in sg.positions()]
+    rpos = [torch.tensor(p, device="cuda") for p in rg.positions()]
+    srcd = [torch.tensor(x, dtype=torch.float32, device="cuda") for x in src]
+    rec = prop.born_forward(spos, srcd, torch.tensor(dm8, dtype=torch.float32, device="cuda"), rpos)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    data = [torch.randn(r.shape, generator=gen, device="cuda", dtype=torch.float32) for r in rec]
+    img = torch.zeros((8,) + tuple(w.rho.shape), dtype=torch.float32, device="cuda")
+    prop.born_adjoint(spos, srcd, data, rpos, img)
+    torch.cuda.synchronize()
+    lhs = float(sum(float((r.double() * d.double()).sum()) for r, d in zip(rec, data)))
+    rhs = float(np.sum(dm3 * P.born_unstagger(img.cpu().numpy().astype(np.float64), med, w.dtw)))
+    e = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"C3 FP32 dot test: <B m, d> = {lhs:.9e}, <m, B* d> = {rhs:.9e}, rel {e:.2e} "
+          f"(storage plan {prop.born_storage_used()})", flush=True)
+    del prop, rec, data, img
+    torch.cuda.empty_cache()
+    assert abs(lhs) > 0 and e < 1e-6
 
 
 # ---- C4 -------------------------------------------------------------------
