@@ -187,6 +187,74 @@ def test_shots_operator_refuses_domain_decomposition(monkeypatch):
         OP._device_of(PB.make_par(domDec=1))
 
 
+class _NearlyAdjoint(OP.Operator):
+    """Identity forward; the adjoint also adds `eps` x data[0] to model[0], so
+    the pair is adjoint up to a relative error of order eps / sqrt(n) for
+    random vectors of n values."""
+
+    def __init__(self, vec, eps):
+        self.setDomainRange(vec, vec)
+        self.eps = eps
+
+    def forward(self, add, model, data):
+        out = model.getNdArray().astype(data.dtype)
+        data.getNdArray()[...] = data.getNdArray() + out if add else out
+
+    def adjoint(self, add, model, data):
+        out = data.getNdArray().astype(model.dtype)
+        out.ravel()[0] += model.dtype.type(self.eps) * out.ravel()[0]
+        model.getNdArray()[...] = model.getNdArray() + out if add else out
+
+
+def _dot_error_in_float(a, b):
+    f = np.float32
+    return abs((f(a) - f(b)) / f(b))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dot_test_precision_follows_the_vectors(dtype):
+    """operator.h of the float build truncates the inner products to float and
+    compares in float (elastic_iso_float/include/operator.h:65-66, :78-79); the
+    double build keeps double. An FP64 error of ~1e-10 that float cannot
+    resolve passes the FP32 test at maxError 1e-12 and fails the FP64 one; an
+    error of ~1e-4 fails both. The expected outcome is derived from the FP64
+    inner products independently of dotTestCpp's code path."""
+    vec = Vector(Hypercube([Axis(1000), Axis(1000)]), dtype=dtype)
+    small = _NearlyAdjoint(vec, 1e-6)
+    (a, b, e1), (c, d, e2) = small._dot_errors(seed=11)
+    assert 1e-12 < e1 < 1e-8 and 1e-12 < e2 < 1e-8   # FP64 sees the defect
+    in_float = _dot_error_in_float(a, b) == 0 and _dot_error_in_float(c, d) == 0
+    assert in_float  # float cannot (for this seed)
+    assert small.dotTestCpp(maxError=1e-12, seed=11) == (dtype == np.float32)
+    large = _NearlyAdjoint(vec, 1e3)
+    assert not large.dotTestCpp(maxError=1e-5, seed=11)
+    assert _NearlyAdjoint(vec, 0.0).dotTestCpp(maxError=0.0, seed=11)
+
+
+def test_dot_test_zero_inner_product_is_not_an_exception():
+    """A zero operator makes 0/0: the reference's float comparison of a NaN
+    error with maxError is false, so its dotTest returns true; the FP64
+    errors are NaN rather than an exception."""
+
+    class Zero(OP.Operator):
+        def __init__(self, vec):
+            self.setDomainRange(vec, vec)
+
+        def forward(self, add, model, data):
+            if not add:
+                data.zero()
+
+        def adjoint(self, add, model, data):
+            if not add:
+                model.zero()
+
+    for dtype in (np.float32, np.float64):
+        op = Zero(Vector(Hypercube([Axis(8)]), dtype=dtype))
+        (_, _, e1), (_, _, e2) = op._dot_errors(seed=1)
+        assert np.isnan(e1) and np.isnan(e2)
+        assert op.dotTestCpp(seed=1)
+
+
 def test_data_component_sampling():
     hyp = Hypercube([Axis(7), Axis(5), Axis(9), Axis(2)])
     model = Vector(hyp, np.random.default_rng(3).standard_normal(hyp.shape))
