@@ -985,11 +985,19 @@ struct BornPlan {
   int64_t S = 0;         // number of segments
   int64_t slots = 0;     // V slices held (full: nts; checkpointed: K + 3)
 };
+// A checkpointed plan cuts the nts - 1 coarse steps into S segments aligned
+// to the end of the record (segment s starts at born_seg_start; all but the
+// first hold K steps) and stores S - 1 checkpoints: the forward sweep leaves
+// the last segment's V slices in the window, where the adjoint sweep starts.
 
 // Arrays (of h->stride elements) the Born adjoint needs besides V and the
 // checkpoints: medium 8, background wavefield 18, adjoint wavefield with its
 // premultiplied levels 36, secondary sources 18, images 8 (FP32 handles: and
 // their FP64 accumulator, 16 more).
+int64_t born_seg_start(const BornPlan& p, int64_t ncoarse, int64_t s) {
+  return std::max<int64_t>(0, ncoarse - (p.S - s) * p.K);
+}
+
 int64_t born_fixed_arrays(const ela_handle* h) { return 8 + 18 + 36 + 18 + 8 + (h->esize == 4 ? 16 : 0); }
 
 int born_plan(ela_handle* h, BornPlan& p) {
@@ -1022,7 +1030,7 @@ int born_plan(ela_handle* h, BornPlan& p) {
   if (double(3 * nts) <= avail) { p.full = true; p.slots = nts; return ELA_OK; }
   for (int64_t K = ncoarse; K >= 1; --K) {
     const int64_t S = (ncoarse + K - 1) / K;
-    if (double(21 * S + 3 * (K + 3)) <= avail) {
+    if (double(21 * (S - 1) + 3 * (K + 3)) <= avail) {
       p.full = false; p.K = K; p.S = S; p.slots = K + 3;
       return ELA_OK;
     }
@@ -1087,7 +1095,7 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
   h->born_last_segment = plan.full ? -1 : plan.K;
   // release what the new plan does not reuse before allocating (peak memory); the gradient's
   // step buffers, which the plan counted as free, are allocated again by the next gradient
-  const size_t ck_bytes = plan.full ? 0 : (size_t)plan.S * 21 * np * sizeof(T);
+  const size_t ck_bytes = plan.full ? 0 : (size_t)(plan.S - 1) * 21 * np * sizeof(T);
   const size_t v_bytes = (size_t)plan.slots * 3 * np * sizeof(T);
   if (h->ckpt.bytes != ck_bytes) h->ckpt.release();
   if (h->coarse.bytes != v_bytes) h->coarse.release();
@@ -1121,22 +1129,34 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
     for (int64_t k = 0; k <= nts - 2; ++k) CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, V(k), V(k + 1)));
     steps += (nts - 1) * sub;
   } else {
-    // forward sweep: checkpoint s holds (p0, p1) and the partial V[s K] at the
-    // start of coarse step s K; two rotating slices carry the accumulation
+    // forward sweep: checkpoint s < S - 1 holds (p0, p1) and the partial
+    // V[k0] at the start of segment s (first coarse step k0); two rotating
+    // slices carry the accumulation. From the last segment's first step kl on,
+    // the samples accumulate in the window of base kl instead and stay there:
+    // the adjoint sweep starts with that segment, which therefore needs no
+    // recomputation.
     T* rot[2] = {pool, pool + 3 * np};
+    const int64_t ncoarse = nts - 1, kl = born_seg_start(plan, ncoarse, plan.S - 1);
     CHK(fill<T>(h, pool, 6 * np));
-    for (int64_t k = 0; k <= nts - 2; ++k) {
-      if (k % plan.K == 0) {
-        T* c = ck + (k / plan.K) * 21 * np;
-        for (int f9 = 0; f9 < 9; ++f9) {
-          HIPCHK(hipMemcpyAsync(c + f9 * np, bg.p0[f9], np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
-          HIPCHK(hipMemcpyAsync(c + (9 + f9) * np, bg.p1[f9], np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
-        }
-        HIPCHK(hipMemcpyAsync(c + 18 * np, rot[k % 2], 3 * np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+    for (int64_t s = 0; s < plan.S - 1; ++s) {
+      const int64_t k0 = born_seg_start(plan, ncoarse, s), k1 = born_seg_start(plan, ncoarse, s + 1);
+      T* c = ck + s * 21 * np;
+      for (int f9 = 0; f9 < 9; ++f9) {
+        HIPCHK(hipMemcpyAsync(c + f9 * np, bg.p0[f9], np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(c + (9 + f9) * np, bg.p1[f9], np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
       }
-      CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, rot[k % 2], rot[(k + 1) % 2]));
-      CHK(fill<T>(h, rot[k % 2], 3 * np));
+      HIPCHK(hipMemcpyAsync(c + 18 * np, rot[k0 % 2], 3 * np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+      for (int64_t k = k0; k < k1; ++k) {
+        CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, rot[k % 2], rot[(k + 1) % 2]));
+        CHK(fill<T>(h, rot[k % 2], 3 * np));
+      }
     }
+    // the partial V[kl] (in rot[kl % 2]) to slot 0, the slots of V[kl + 1 .. nts - 1] cleared
+    base = kl;
+    if (rot[kl % 2] != V(kl))
+      HIPCHK(hipMemcpyAsync(V(kl), rot[kl % 2], 3 * np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+    CHK(fill<T>(h, V(kl + 1), (nts - 1 - kl) * 3 * np));
+    for (int64_t k = kl; k <= nts - 2; ++k) CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, V(k), V(k + 1)));
     steps += (nts - 1) * sub;
   }
 
@@ -1168,19 +1188,21 @@ int born_adjoint_run(ela_handle* h, const DevPoints& sp, T* const src_tr[9], con
     for (int64_t its = nts - 2; its >= 0; --its) CHK(adjoint_step(its));
   } else {
     for (int64_t s = plan.S - 1; s >= 0; --s) {
-      const int64_t k0 = s * plan.K, k1 = std::min(k0 + plan.K, nts - 1);
+      const int64_t k0 = born_seg_start(plan, nts - 1, s), k1 = born_seg_start(plan, nts - 1, s + 1);
       const int64_t kend = std::min(k1 + 1, nts - 2);  // last recomputed coarse step
       base = k0;
-      // restore the background state and the partial V[k0]
-      const T* c = ck + s * 21 * np;
-      for (int f9 = 0; f9 < 9; ++f9) {
-        HIPCHK(hipMemcpyAsync(bg.p0[f9], c + f9 * np, np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(bg.p1[f9], c + (9 + f9) * np, np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+      if (s < plan.S - 1) {  // the last segment's samples are in the window already
+        // restore the background state and the partial V[k0]
+        const T* c = ck + s * 21 * np;
+        for (int f9 = 0; f9 < 9; ++f9) {
+          HIPCHK(hipMemcpyAsync(bg.p0[f9], c + f9 * np, np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+          HIPCHK(hipMemcpyAsync(bg.p1[f9], c + (9 + f9) * np, np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+        }
+        HIPCHK(hipMemcpyAsync(V(k0), c + 18 * np, 3 * np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+        CHK(fill<T>(h, V(k0 + 1), (kend + 1 - k0) * 3 * np));
+        for (int64_t k = k0; k <= kend; ++k) CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, V(k), V(k + 1)));
+        steps += (kend + 1 - k0) * sub;
       }
-      HIPCHK(hipMemcpyAsync(V(k0), c + 18 * np, 3 * np * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
-      CHK(fill<T>(h, V(k0 + 1), (kend + 1 - k0) * 3 * np));
-      for (int64_t k = k0; k <= kend; ++k) CHK(bg_coarse_step<T>(h, bg, sp, src_tr, k, V(k), V(k + 1)));
-      steps += (kend + 1 - k0) * sub;
       // V[k0 .. min(k1 + 1, nts - 1)] are final
       for (int64_t its = k1 - 1; its >= k0; --its) CHK(adjoint_step(its));
     }

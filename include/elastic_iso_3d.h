@@ -211,8 +211,9 @@ int ela_born_adjoint(ela_handle* h, const ela_points* src_pos, const ela_traces*
  * in pinned host memory (BornElasticGpuFunctions_3D.cu:2147-2200), 3*nts
  * arrays; here it stays in HBM when it fits, otherwise the forward sweep
  * stores exact checkpoints of the background state every `segment` coarse
- * samples and each segment is recomputed before its adjoint sweep (one extra
- * background propagation; results are bit-identical).
+ * samples and keeps the last segment's samples, and every other segment is
+ * recomputed before its adjoint sweep (one extra background propagation
+ * less the last segment; results are bit-identical).
  *   segment = 0   automatic: all samples if they fit, else the longest
  *                 segment that fits, within budget_bytes of device memory
  *                 (0: the free device memory at call time);
