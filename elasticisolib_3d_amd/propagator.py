@@ -343,7 +343,9 @@ class Propagator:
     def set_born_storage(self, segment: int = 0, budget_bytes: int = 0) -> None:
         """Coarse-wavefield storage of the Born adjoint: 0 automatic (within
         budget_bytes, 0 = free device memory), -1 every coarse sample kept,
-        K > 0 exact checkpoints every K coarse samples with recomputation."""
+        K > 0 segments of K coarse samples counted back from the end of the
+        record, exact checkpoints at the start of all but the last, whose
+        samples the forward sweep keeps; the others are recomputed."""
         check(self.lib.ela_set_born_storage(self.h, int(segment), int(budget_bytes)), "ela_set_born_storage")
 
     def set_born_mode(self, physics: bool) -> None:
