@@ -146,14 +146,20 @@ LONG = O.Grid(nz=29, nx=27, ny=25, nts=11, sub=8, dz=10.0, dx=10.0, dy=10.0, dts
 def test_born_adjoint_checkpointed_is_bit_identical(precision):
     """Checkpointed storage of the background coarse wavefield (exact state
     checkpoints + recomputation) gives the same bits as keeping every coarse
-    sample, for segment lengths that do and do not divide nts - 1."""
+    sample, for segment lengths that do and do not divide nts - 1. The fine
+    steps run: 80 forward + 80 adjoint, plus the recomputation of every
+    segment but the last (segments of K counted back from coarse step 10,
+    each recomputed through two steps past its end, at most to step 9),
+    e.g. K = 3: [0, 1) -> steps 0-2, [1, 4) -> 1-5, [4, 7) -> 4-8, [7, 10)
+    kept: 13 coarse steps = 104 fine steps."""
+    steps_expected = {1: 368, 2: 288, 3: 264, 4: 240, 9: 184, 10: 160, 50: 160}
     g = LONG
     P = random_problem(g, seed=5)
     prop = make_prop(g, precision)
     prop.set_medium(P["med"])
     prop.set_born_storage(-1)
     full = prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])
-    assert prop.born_storage_used() == -1
+    assert prop.born_storage_used() == -1 and prop.last_timing()[1] == 160
     ref = O.lib_born_adjoint(g, P["med"], P["src"], P["spos"], P["rec"], P["rpos"])
     print(f"born_adjoint long {precision}: rel={rel(full, ref):.3e}", flush=True)
     assert rel(full, ref) < TOL[precision]
@@ -162,8 +168,9 @@ def test_born_adjoint_checkpointed_is_bit_identical(precision):
         out = prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])
         assert prop.born_storage_used() == min(K, g.nts - 1)
         same = np.array_equal(out, full)
-        print(f"  segment {K}: bit-identical={same}", flush=True)
-        assert same
+        steps = prop.last_timing()[1]
+        print(f"  segment {K}: bit-identical={same}, {steps} fine steps", flush=True)
+        assert same and steps == steps_expected[K]
     prop.set_born_storage(0)
     out = prop.born_adjoint(P["spos"], P["src"], P["rec"], P["rpos"])
     assert prop.born_storage_used() == -1 and np.array_equal(out, full)
