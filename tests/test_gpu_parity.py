@@ -416,3 +416,30 @@ def test_derived_medium_equals_host_scaling(problems, precision):
             ulp = np.spacing(np.abs(host[k])).astype(np.float64)
             d = np.abs(dev[k].astype(np.float64) - host[k].astype(np.float64))
             assert np.all(d <= 2 * ulp), (k, float((d / np.maximum(ulp, 1e-300)).max()))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_empty_point_lists(problems, precision):
+    """No receiver on any grid, or no source on any grid: the calls succeed,
+    return zero-length traces or exact zeros (nothing is injected, so every
+    field stays 0), and leave the handle as it was: the next full call still
+    matches the oracle."""
+    g, P = GRIDS["odd"], problems["odd"]
+    none = [np.zeros(0, np.int64) for _ in range(7)]
+    empty = [np.zeros((0, g.nts)) for _ in range(9)]
+    prop = make_prop(g, precision)
+    prop.set_medium(P["med"])
+    print(f"empty point lists {precision}", flush=True)
+    out = prop.nl_forward(P["spos"], P["src"], none)
+    assert len(out) == 9 and all(np.asarray(x).size == 0 for x in out)
+    quiet = prop.nl_forward(none, empty, P["rpos"])
+    assert sum(np.asarray(x).size for x in quiet) > 0
+    assert all(np.count_nonzero(np.asarray(x)) == 0 for x in quiet)
+    back = prop.nl_adjoint(P["spos"], empty, none)
+    assert all(np.count_nonzero(np.asarray(x)) == 0 for x in back)
+    img = prop.born_adjoint(P["spos"], P["src"], empty, none)
+    assert np.count_nonzero(np.asarray(img)) == 0
+    ref = O.lib_nl_forward(g, P["med"], P["src"], P["spos"], P["rpos"])
+    full = prop.nl_forward(P["spos"], P["src"], P["rpos"])
+    print(f"  after the empty calls: rel={rel(full, ref):.3e}", flush=True)
+    assert rel(full, ref) < TOL[precision]
