@@ -443,3 +443,30 @@ def test_empty_point_lists(problems, precision):
     full = prop.nl_forward(P["spos"], P["src"], P["rpos"])
     print(f"  after the empty calls: rel={rel(full, ref):.3e}", flush=True)
     assert rel(full, ref) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_positions_outside_the_interior_are_rejected(problems, precision):
+    """A source or receiver in the FAT layer, at a negative index or past the
+    grid is an ELA_ERR_ARG error (the conversion kernel flags it before any
+    kernel uses a position); the handle stays usable."""
+    from elasticisolib_3d_amd._lib import ElaError
+    g, P = GRIDS["odd"], problems["odd"]
+    c = (g.ny // 2) * g.nz * g.nx + (g.nx // 2) * g.nz
+    prop = make_prop(g, precision)
+    prop.set_medium(P["med"])
+    print(f"positions outside the interior {precision}", flush=True)
+    for bad in (c + 0, c + g.nz - 1, -5, g.nz * g.nx * g.ny):   # z = 0 and z = nz - 1 lie in the FAT layer
+        spos = [p.copy() for p in P["spos"]]
+        spos[0] = np.concatenate([spos[0], [bad]]).astype(np.int64)
+        src = [np.vstack([s, np.ones((1, g.nts))]) if O.COMP_GRID[k] == 0 else s for k, s in enumerate(P["src"])]
+        with pytest.raises(ElaError):
+            prop.nl_forward(spos, src, P["rpos"])
+        rpos = [p.copy() for p in P["rpos"]]
+        rpos[3] = np.concatenate([rpos[3], [bad]]).astype(np.int64)
+        with pytest.raises(ElaError):
+            prop.nl_forward(P["spos"], P["src"], rpos)
+    ref = O.lib_nl_forward(g, P["med"], P["src"], P["spos"], P["rpos"])
+    full = prop.nl_forward(P["spos"], P["src"], P["rpos"])
+    print(f"  after the rejected calls: rel={rel(full, ref):.3e}", flush=True)
+    assert rel(full, ref) < TOL[precision]
