@@ -470,3 +470,15 @@ def test_positions_outside_the_interior_are_rejected(problems, precision):
     full = prop.nl_forward(P["spos"], P["src"], P["rpos"])
     print(f"  after the rejected calls: rel={rel(full, ref):.3e}", flush=True)
     assert rel(full, ref) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision,shape", [("f32", (1100, 1000, 1000)), ("f64", (560, 1000, 1000))])
+def test_grid_beyond_32_bit_offsets_is_rejected(precision, shape):
+    """Kernels index an array with 32-bit offsets: a grid whose arrays would
+    exceed 4 GiB is ELA_ERR_ARG at creation, before anything is allocated."""
+    from elasticisolib_3d_amd._lib import ElaError
+    from elasticisolib_3d_amd.propagator import Propagator
+    nz, nx, ny = shape
+    print(f"grid {nz}x{nx}x{ny} {precision}", flush=True)
+    with pytest.raises(ElaError, match="4 GiB"):
+        Propagator(nz, nx, ny, 5, 8, 10.0, 10.0, 10.0, 0.004, 5, 0.95, precision)

@@ -115,6 +115,22 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
         _lib.load()
 
 
+@pytest.mark.parametrize("bad,match", [
+    (dict(nz=8), "too small"), (dict(ny=7), "too small"), (dict(nts=1), "nts must be"), (dict(sub=0), "sub must be"),
+    (dict(dx=0.0), "non-positive spacing"), (dict(dts=-1e-3), "non-positive spacing"),
+    (dict(min_pad=-1), "negative min_pad"), (dict(min_pad=11), "larger than half the grid")])
+def test_create_rejects_bad_configurations(bad, match):
+    """ela_create validates its configuration before any HIP call, so these
+    run without a GPU: each bad field is ELA_ERR_ARG with its message."""
+    from elasticisolib_3d_amd import _lib
+    from elasticisolib_3d_amd.propagator import Propagator
+    cfg = dict(nz=21, nx=23, ny=25, nts=5, sub=8, dz=10.0, dx=10.0, dy=10.0, dts=0.004, min_pad=5)
+    cfg.update(bad)
+    with pytest.raises(_lib.ElaError, match=match):
+        Propagator(cfg["nz"], cfg["nx"], cfg["ny"], cfg["nts"], cfg["sub"], cfg["dz"], cfg["dx"], cfg["dy"],
+                   cfg["dts"], cfg["min_pad"], 0.95, "f32")
+
+
 def _rand_model(shape, rng):
     rho = 1800 + 400 * rng.random(shape)
     lam = 1e9 * (3 + rng.random(shape))
